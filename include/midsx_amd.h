@@ -1,0 +1,227 @@
+/* C-ABI of the MI355X photon-transport library (libmidsx_amd.so).
+ *
+ * The reference has no C or FFI layer: its boundary is the C++ class API of
+ * include/MIDSX/Core.h, linked as the static library MIDSX::MIDSX.  Each entry
+ * point below names the reference interface it stands in for (paths relative
+ * to the reference tree).  Plain pointers and sizes only; host buffers stay
+ * caller-owned, device copies belong to the context.
+ *
+ * Errors: every call returns 0 (MDX_OK) or a negative status; the message of
+ * the last failure on the calling thread is returned by mdx_last_error().
+ * Status classes map the reference's exception types.
+ *
+ * Threading: one context per device; calls on one context must come from one
+ * host thread at a time.
+ */
+#ifndef MIDSX_AMD_H
+#define MIDSX_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MDX_OK 0
+#define MDX_ERR_INVALID (-1) /* std::invalid_argument */
+#define MDX_ERR_RANGE (-2)   /* std::out_of_range */
+#define MDX_ERR_RUNTIME (-3) /* std::runtime_error, DatabaseException */
+#define MDX_ERR_DEVICE (-4)  /* HIP runtime failure */
+#define MDX_ERR_NOMEM (-5)   /* device memory / admission budget exceeded */
+
+/* Source kinds: BeamDirectionality, IsotropicDirectionality,
+ * RectangularIsotropicDirectionality, DiscIsotropicDirectionality
+ * (include/MIDSX/Core/source.h:86-196); Monoenergetic / Polyenergetic
+ * spectra (source.h:23-80); PointGeometry (source.h:205-238). */
+#define MDX_DIR_BEAM 0
+#define MDX_DIR_ISOTROPIC 1
+#define MDX_DIR_RECT_ISOTROPIC 2
+#define MDX_DIR_DISC_ISOTROPIC 3
+#define MDX_SPEC_MONO 0
+#define MDX_SPEC_POLY 1
+
+/* Tally kinds: AACuboidVolumeTally (volume_tally.h:59), DiscSurfaceTally
+ * (surface_tally.h:58), RectangularSurfaceTally (surface_tally.h:82), and
+ * an image plane (a rectangle split into nx x ny pixels recording incident
+ * energy; equivalent to nx*ny RectangularSurfaceTally pixels). */
+#define MDX_TALLY_AACUBOID 0
+#define MDX_TALLY_DISC 1
+#define MDX_TALLY_RECT 2
+#define MDX_TALLY_IMAGE 3
+
+/* Tally result layout: [quantity][bucket][field], doubles.
+ * Volume quantities (volume_quantity.h:13-28): 0 EnergyDeposition,
+ * 1 IncidentEnergy, 2 NumberOfPhotons, 3 NumberOfInteractions.
+ * Surface quantities (surface_quantity.h:13-28): 0 IncidentEnergy,
+ * 1 EntranceCosine, 2 NumberOfPhotons.
+ * Buckets: 0 primary, 1 single incoherent, 2 single coherent, 3 multiple.
+ * Fields: 0 number of recorded events n, 1 sum of values, 2 sum of squares
+ * (VectorValue::getSum / getVariance, quantity.cpp:27-94, follow from these). */
+#define MDX_TALLY_NQ 4
+#define MDX_TALLY_NBUCKET 4
+#define MDX_TALLY_NFIELD 3
+#define MDX_TALLY_ACC_SIZE (MDX_TALLY_NQ * MDX_TALLY_NBUCKET * MDX_TALLY_NFIELD)
+
+/* Interpolated table selectors for mdx_interpolate (material_data.h:95-151). */
+#define MDX_XS_TOTAL 0
+#define MDX_XS_COHERENT 1
+#define MDX_XS_INCOHERENT 2
+#define MDX_XS_PHOTOELECTRIC 3
+#define MDX_SF_INCOHERENT 4
+#define MDX_FF_COHERENT 5 /* host tables only */
+
+/* Event counters reported by mdx_run (index into mdx_stats.counters). */
+#define MDX_STAT_HISTORIES 0
+#define MDX_STAT_STEPS 1
+#define MDX_STAT_REAL 2
+#define MDX_STAT_RAYLEIGH 3
+#define MDX_STAT_COMPTON 4
+#define MDX_STAT_PHOTO 5
+#define MDX_STAT_KAHN_TRIALS 6
+#define MDX_STAT_KAHN_S_TESTS 7
+#define MDX_STAT_DEPOSITS 8
+#define MDX_STAT_SURFACE_HITS 9
+#define MDX_STAT_ESCAPES 10
+#define MDX_STAT_NCOUNTERS 12
+
+/* Transport kernels. */
+#define MDX_KERNEL_PLAIN 0 /* one history per thread, one simulation per launch */
+
+typedef struct mdx_ctx mdx_ctx;
+
+typedef struct {
+  int direction;      /* MDX_DIR_* */
+  int spectrum;       /* MDX_SPEC_* */
+  double position[3]; /* PointGeometry position (cm) */
+  double a[3];        /* beam: pass-through point; rect: corner; disc: centre */
+  double b[3];        /* rect: edge1; disc: normal */
+  double c[3];        /* rect: edge2 */
+  double radius;      /* disc radius (cm) */
+  double energy;      /* monoenergetic energy (eV) */
+  int n_bins;         /* polyenergetic: number of (energy, weight) bins */
+  const double* bin_energy; /* eV */
+  const double* bin_weight; /* normalised internally (probability_dist.cpp:34-39) */
+} mdx_source;
+
+typedef struct {
+  int kind;          /* MDX_TALLY_* */
+  double p0[3];      /* cuboid min corner / disc centre / rect and image corner */
+  double p1[3];      /* cuboid max corner / rect and image edge1 */
+  double p2[3];      /* rect and image edge2 */
+  double normal[3];  /* disc normal (rect / image: normalize(edge1 x edge2)) */
+  double radius;     /* disc radius */
+  int nx, ny;        /* image pixels along edge1 / edge2 */
+  int hist_bins;     /* surface tallies: IncidentEnergy histogram bins, 0 = off */
+  double hist_lo;    /* eV */
+  double hist_width; /* eV */
+} mdx_tally;
+
+typedef struct {
+  uint64_t counters[MDX_STAT_NCOUNTERS];
+  double kernel_ms; /* device time of the transport launch(es), HIP events */
+} mdx_stats;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* mdx_last_error(void);
+
+/* One context per device (stands in for the state PhysicsEngine,
+ * physics_engine.h:30, and InteractionData, interaction_data.h:37, hold). */
+int mdx_create(int device, mdx_ctx** out);
+int mdx_destroy(mdx_ctx* ctx);
+
+/* Directory holding epdl/NNN.endf.gz (or full EPDL NNN.txt files) and
+ * materials.csv.  Replaces the configure-time MIDSX_DB_PATH
+ * (include/MIDSX/Core/config.h.in:3, CMakeLists.txt:65-73). */
+int mdx_set_data_dir(mdx_ctx* ctx, const char* dir);
+
+/* InteractionDataHelpers::convertMaterialNameToId / convertMaterialIdToName
+ * (include/MIDSX/Core/material_helpers.h:110-121). */
+int mdx_material_id(mdx_ctx* ctx, const char* name, int* id);
+int mdx_material_name(mdx_ctx* ctx, int id, char* buf, int buflen);
+
+/* InteractionData(material names) (interaction_data.cpp:3-73): builds the
+ * FP64 tables of each material, the majorant and the RITA tables, and
+ * uploads their FP32 device form. */
+int mdx_build_tables(mdx_ctx* ctx, const int* material_ids, int n_materials);
+/* ComputationalDomain::getInteractionData (computational_domain.cpp:11-14,
+ * 109-123): builds the tables for the background and every grid material. */
+int mdx_build_tables_for_domain(mdx_ctx* ctx);
+
+/* ComputationalDomain (computational_domain.cpp:84-107): domain box
+ * [0, dim_space] and background material; grids are added in JSON order
+ * (first grid containing a point wins).  ids are x-fastest uint8 material
+ * IDs (voxel_grid.cpp:119-121, 175-181). */
+int mdx_set_domain(mdx_ctx* ctx, const double dim_space[3], int background_id);
+int mdx_add_grid(mdx_ctx* ctx, const uint8_t* ids, const int dims[3], const double spacing[3],
+                 const double origin[3]);
+/* ComputationalDomain(json path) with NIfTI-1 grids (.nii / .nii.gz). */
+int mdx_load_scene(mdx_ctx* ctx, const char* json_path);
+int mdx_num_grids(mdx_ctx* ctx, int* n);
+int mdx_grid_info(mdx_ctx* ctx, int grid, int dims[3], double spacing[3], double origin[3]);
+int mdx_get_grid_ids(mdx_ctx* ctx, int grid, uint8_t* out);
+
+/* PhotonSource(spectrum, directionality, geometry) (source.cpp:143-155). */
+int mdx_set_source(mdx_ctx* ctx, const mdx_source* src);
+
+/* The tally factories of runSimulation (run_simulation.h:24-27). */
+int mdx_set_tallies(mdx_ctx* ctx, const mdx_tally* tallies, int n);
+/* Per-voxel energy deposit of every grid (Voxel::dose, voxel.h:12-15,
+ * PhysicsEngine::addVoxelDataToComputationalDomain physics_engine.cpp:160-168). */
+int mdx_enable_voxel_dose(mdx_ctx* ctx, int enable);
+int mdx_reset_tallies(mdx_ctx* ctx);
+
+/* runSimulation(source, engine, surface_init, volume_init, N, run_time)
+ * (run_simulation.cpp:6-41): transports histories [first, first + n) with
+ * RNG streams keyed by (seed, history).  Blocking.  Tallies accumulate across
+ * calls until mdx_reset_tallies (physics_engine.cpp:111-122). */
+int mdx_run(mdx_ctx* ctx, uint64_t n_histories, uint64_t seed, uint64_t first_history, int kernel,
+            mdx_stats* stats);
+
+/* PhysicsEngine::getVolumeQuantityContainers / getSurfaceQuantityContainers
+ * (physics_engine.cpp:124-158).  acc: MDX_TALLY_ACC_SIZE doubles; hist:
+ * 4 * hist_bins doubles (may be NULL); image: 3 * nx * ny doubles (sum E,
+ * sum E^2, count per pixel; may be NULL). */
+int mdx_get_tally(mdx_ctx* ctx, int tally, double* acc, double* hist, double* image);
+/* VoxelGrid::getVoxel(i).dose (voxel_grid.cpp:43-45, 86-104): per voxel
+ * (sum, sum^2) interleaved and the deposit count; either may be NULL. */
+int mdx_get_voxel_dose(mdx_ctx* ctx, int grid, double* sum_sumsq, uint32_t* count);
+/* ComputationalDomain background voxel dose: sum, sum^2, count. */
+int mdx_get_background_dose(mdx_ctx* ctx, double out[3]);
+
+/* MaterialData::interpolate*CrossSection / interpolateIncoherentScattering-
+ * Function (material_data.h:95-151) and InteractionData::
+ * interpolateMaxTotalCrossSection (interaction_data.h:63, material_id < 0),
+ * evaluated on the device in FP32 from the packed tables. */
+int mdx_interpolate(mdx_ctx* ctx, int material_id, int which, int n, const float* x, float* out);
+/* The same functions on the FP64 host tables (the values the device tables
+ * are packed from). */
+int mdx_interpolate_host(mdx_ctx* ctx, int material_id, int which, int n, const double* x, double* out);
+/* FP64 host table nodes: size query with x == NULL. */
+int mdx_get_table(mdx_ctx* ctx, int material_id, int which, int* n, double* x, double* y);
+/* MaterialData::sampleCoherentScatteringDCS (material_data.h:151) with the
+ * uniform given: device FP32 and host FP64. */
+int mdx_sample_rayleigh(mdx_ctx* ctx, int material_id, int n, const float* energy, const float* y, float* mu);
+int mdx_sample_rayleigh_host(mdx_ctx* ctx, int material_id, int n, const double* energy, const double* y,
+                             double* mu);
+/* RITA table of one energy row (probability_dist.cpp:121-136): size query
+ * with x == NULL; a and b have rows-1 valid entries. */
+int mdx_get_rita_row(mdx_ctx* ctx, int material_id, int energy_index, int* rows, double* energy, double* x,
+                     double* y, double* a, double* b);
+/* IncoherentScattering::interact sampling (photon_interactions.cpp:62-141)
+ * on the device: n Compton events at the given energies. */
+int mdx_sample_compton(mdx_ctx* ctx, int material_id, int n, const float* energy, uint64_t seed, float* mu,
+                       float* energy_out);
+/* ComputationalDomain::getVoxel (computational_domain.cpp:23-31) on the
+ * device: grid index (-1 background), linear voxel index, material ID. */
+int mdx_voxel_lookup(mdx_ctx* ctx, int n, const float* xyz, int* grid, long long* linear, int* material_id);
+/* PhysicsEngine::setInteractionType decision (physics_engine.cpp:82-109) on
+ * the device: 0 photoelectric, 1 Rayleigh, 2 Compton. */
+int mdx_select_interaction(mdx_ctx* ctx, int n, const float* s_coh, const float* s_inc, const float* s_tot,
+                           const float* u, int* type);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MIDSX_AMD_H */

@@ -1,0 +1,347 @@
+"""Python host mirror of the MI355X photon-transport library.
+
+The reference's surface is its C++ class API (include/MIDSX/Core.h); this
+module exposes the same objects -- a domain, interaction tables, a photon
+source, volume / surface tallies and runSimulation -- over the C-ABI of
+libmidsx_amd.so (include/midsx_amd.h) with ctypes.  Nothing here computes
+physics: every number comes from the HIP library, and importing the module
+fails loudly when the library has not been built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libmidsx_amd.so")
+DATA_DIR = os.path.join(HERE, "data")
+
+# include/midsx_amd.h
+OK, ERR_INVALID, ERR_RANGE, ERR_RUNTIME, ERR_DEVICE, ERR_NOMEM = 0, -1, -2, -3, -4, -5
+DIR_BEAM, DIR_ISOTROPIC, DIR_RECT_ISOTROPIC, DIR_DISC_ISOTROPIC = 0, 1, 2, 3
+SPEC_MONO, SPEC_POLY = 0, 1
+TALLY_AACUBOID, TALLY_DISC, TALLY_RECT, TALLY_IMAGE = 0, 1, 2, 3
+XS_TOTAL, XS_COHERENT, XS_INCOHERENT, XS_PHOTOELECTRIC, SF_INCOHERENT, FF_COHERENT = 0, 1, 2, 3, 4, 5
+KERNEL_PLAIN = 0
+NQ, NBUCKET, NFIELD = 4, 4, 3
+STAT_NAMES = ["histories", "steps", "real", "rayleigh", "compton", "photo", "kahn_trials", "kahn_s_tests",
+              "deposits", "surface_hits", "escapes", "reserved"]
+VOLUME_QUANTITIES = ["EnergyDeposition", "IncidentEnergy", "NumberOfPhotons", "NumberOfInteractions"]
+SURFACE_QUANTITIES = ["IncidentEnergy", "EntranceCosine", "NumberOfPhotons"]
+BUCKETS = ["primary", "single_incoherent", "single_coherent", "multiple"]
+
+EXPORTS = [
+    "mdx_last_error", "mdx_create", "mdx_destroy", "mdx_set_data_dir", "mdx_material_id", "mdx_material_name",
+    "mdx_build_tables", "mdx_build_tables_for_domain", "mdx_set_domain", "mdx_add_grid", "mdx_load_scene",
+    "mdx_num_grids", "mdx_grid_info", "mdx_get_grid_ids", "mdx_set_source", "mdx_set_tallies",
+    "mdx_enable_voxel_dose", "mdx_reset_tallies", "mdx_run", "mdx_get_tally", "mdx_get_voxel_dose",
+    "mdx_get_background_dose", "mdx_interpolate", "mdx_interpolate_host", "mdx_get_table", "mdx_sample_rayleigh",
+    "mdx_sample_rayleigh_host", "mdx_get_rita_row", "mdx_sample_compton", "mdx_voxel_lookup",
+    "mdx_select_interaction",
+]
+
+
+class MdxError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("%s (status %d)" % (msg, status))
+        self.status = status
+
+
+class _Source(ctypes.Structure):
+    _fields_ = [("direction", ctypes.c_int), ("spectrum", ctypes.c_int), ("position", ctypes.c_double * 3),
+                ("a", ctypes.c_double * 3), ("b", ctypes.c_double * 3), ("c", ctypes.c_double * 3),
+                ("radius", ctypes.c_double), ("energy", ctypes.c_double), ("n_bins", ctypes.c_int),
+                ("bin_energy", ctypes.POINTER(ctypes.c_double)), ("bin_weight", ctypes.POINTER(ctypes.c_double))]
+
+
+class _Tally(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("p0", ctypes.c_double * 3), ("p1", ctypes.c_double * 3),
+                ("p2", ctypes.c_double * 3), ("normal", ctypes.c_double * 3), ("radius", ctypes.c_double),
+                ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("hist_bins", ctypes.c_int),
+                ("hist_lo", ctypes.c_double), ("hist_width", ctypes.c_double)]
+
+
+class _Stats(ctypes.Structure):
+    _fields_ = [("counters", ctypes.c_uint64 * 12), ("kernel_ms", ctypes.c_double)]
+
+
+_lib = None
+
+
+def load_library(path=None):
+    """Loads libmidsx_amd.so; raises if it is missing (no fallback path)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise ImportError("libmidsx_amd.so not built (%s); run python -m midsx_amd.build" % path)
+    lib = ctypes.CDLL(path)
+    lib.mdx_last_error.restype = ctypes.c_char_p
+    for name in EXPORTS:
+        getattr(lib, name)  # every declared entry point must resolve
+    _lib = lib
+    return lib
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+def _v3(v):
+    return (ctypes.c_double * 3)(*[float(x) for x in v])
+
+
+class Context:
+    """One device context (device=-1: host-only, tables and registry only)."""
+
+    def __init__(self, device=0, data_dir=None):
+        self.lib = load_library()
+        self._h = ctypes.c_void_p()
+        self._call(self.lib.mdx_create, ctypes.c_int(device), ctypes.byref(self._h))
+        self.device = device
+        if data_dir:
+            self._call(self.lib.mdx_set_data_dir, self._h, data_dir.encode())
+        self.tallies = []
+
+    def _call(self, fn, *args):
+        st = fn(*args)
+        if st != OK:
+            raise MdxError(st, self.lib.mdx_last_error().decode())
+
+    def close(self):
+        if self._h:
+            self.lib.mdx_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------- materials / tables
+    def material_id(self, name):
+        out = ctypes.c_int()
+        self._call(self.lib.mdx_material_id, self._h, name.encode(), ctypes.byref(out))
+        return out.value
+
+    def material_name(self, mid):
+        buf = ctypes.create_string_buffer(256)
+        self._call(self.lib.mdx_material_name, self._h, ctypes.c_int(mid), buf, ctypes.c_int(256))
+        return buf.value.decode()
+
+    def build_tables(self, material_ids=None):
+        if material_ids is None:
+            self._call(self.lib.mdx_build_tables_for_domain, self._h)
+        else:
+            ids = np.ascontiguousarray(material_ids, dtype=np.int32)
+            self._call(self.lib.mdx_build_tables, self._h, _ptr(ids, ctypes.c_int), ctypes.c_int(len(ids)))
+
+    def table(self, material_id, which):
+        n = ctypes.c_int()
+        self._call(self.lib.mdx_get_table, self._h, ctypes.c_int(material_id), ctypes.c_int(which), ctypes.byref(n),
+                   None, None)
+        x = np.empty(n.value)
+        y = np.empty(n.value)
+        self._call(self.lib.mdx_get_table, self._h, ctypes.c_int(material_id), ctypes.c_int(which), ctypes.byref(n),
+                   _ptr(x, ctypes.c_double), _ptr(y, ctypes.c_double))
+        return x, y
+
+    def interpolate(self, material_id, which, x):
+        x = _f32(x)
+        out = np.empty_like(x)
+        self._call(self.lib.mdx_interpolate, self._h, ctypes.c_int(material_id), ctypes.c_int(which),
+                   ctypes.c_int(x.size), _ptr(x, ctypes.c_float), _ptr(out, ctypes.c_float))
+        return out
+
+    def interpolate_host(self, material_id, which, x):
+        x = _f64(x)
+        out = np.empty_like(x)
+        self._call(self.lib.mdx_interpolate_host, self._h, ctypes.c_int(material_id), ctypes.c_int(which),
+                   ctypes.c_int(x.size), _ptr(x, ctypes.c_double), _ptr(out, ctypes.c_double))
+        return out
+
+    def rita_row(self, material_id, energy_index):
+        rows = ctypes.c_int()
+        e = ctypes.c_double()
+        self._call(self.lib.mdx_get_rita_row, self._h, ctypes.c_int(material_id), ctypes.c_int(energy_index),
+                   ctypes.byref(rows), ctypes.byref(e), None, None, None, None)
+        x, y, a, b = (np.empty(rows.value) for _ in range(4))
+        self._call(self.lib.mdx_get_rita_row, self._h, ctypes.c_int(material_id), ctypes.c_int(energy_index),
+                   ctypes.byref(rows), ctypes.byref(e), _ptr(x, ctypes.c_double), _ptr(y, ctypes.c_double),
+                   _ptr(a, ctypes.c_double), _ptr(b, ctypes.c_double))
+        return e.value, x, y, a[:-1], b[:-1]
+
+    def sample_rayleigh(self, material_id, energy, y):
+        e, y = _f32(energy), _f32(y)
+        out = np.empty_like(e)
+        self._call(self.lib.mdx_sample_rayleigh, self._h, ctypes.c_int(material_id), ctypes.c_int(e.size),
+                   _ptr(e, ctypes.c_float), _ptr(y, ctypes.c_float), _ptr(out, ctypes.c_float))
+        return out
+
+    def sample_rayleigh_host(self, material_id, energy, y):
+        e, y = _f64(energy), _f64(y)
+        out = np.empty_like(e)
+        self._call(self.lib.mdx_sample_rayleigh_host, self._h, ctypes.c_int(material_id), ctypes.c_int(e.size),
+                   _ptr(e, ctypes.c_double), _ptr(y, ctypes.c_double), _ptr(out, ctypes.c_double))
+        return out
+
+    def sample_compton(self, material_id, energy, seed=1):
+        e = _f32(energy)
+        mu = np.empty_like(e)
+        ep = np.empty_like(e)
+        self._call(self.lib.mdx_sample_compton, self._h, ctypes.c_int(material_id), ctypes.c_int(e.size),
+                   _ptr(e, ctypes.c_float), ctypes.c_uint64(seed), _ptr(mu, ctypes.c_float), _ptr(ep, ctypes.c_float))
+        return mu, ep
+
+    def select_interaction(self, s_coh, s_inc, s_tot, u):
+        a, b, c, d = _f32(s_coh), _f32(s_inc), _f32(s_tot), _f32(u)
+        out = np.empty(a.size, dtype=np.int32)
+        self._call(self.lib.mdx_select_interaction, self._h, ctypes.c_int(a.size), _ptr(a, ctypes.c_float),
+                   _ptr(b, ctypes.c_float), _ptr(c, ctypes.c_float), _ptr(d, ctypes.c_float), _ptr(out, ctypes.c_int))
+        return out
+
+    # ---------------------------------------------------------- domain
+    def set_domain(self, dim_space, background_id):
+        self._call(self.lib.mdx_set_domain, self._h, _v3(dim_space), ctypes.c_int(background_id))
+
+    def add_grid(self, ids, spacing, origin):
+        """ids indexed [x, y, z] (stored x-fastest)."""
+        ids = np.asarray(ids, dtype=np.uint8)
+        dims = (ctypes.c_int * 3)(*ids.shape)
+        flat = np.ascontiguousarray(ids.transpose(2, 1, 0)).ravel()
+        self._call(self.lib.mdx_add_grid, self._h, _ptr(flat, ctypes.c_uint8), dims, _v3(spacing), _v3(origin))
+
+    def load_scene(self, json_path):
+        self._call(self.lib.mdx_load_scene, self._h, os.fspath(json_path).encode())
+
+    def num_grids(self):
+        n = ctypes.c_int()
+        self._call(self.lib.mdx_num_grids, self._h, ctypes.byref(n))
+        return n.value
+
+    def grid_info(self, g):
+        dims = (ctypes.c_int * 3)()
+        sp = (ctypes.c_double * 3)()
+        org = (ctypes.c_double * 3)()
+        self._call(self.lib.mdx_grid_info, self._h, ctypes.c_int(g), dims, sp, org)
+        return tuple(dims), tuple(sp), tuple(org)
+
+    def grid_ids(self, g):
+        dims, _, _ = self.grid_info(g)
+        out = np.empty(dims[0] * dims[1] * dims[2], dtype=np.uint8)
+        self._call(self.lib.mdx_get_grid_ids, self._h, ctypes.c_int(g), _ptr(out, ctypes.c_uint8))
+        return out.reshape(dims[2], dims[1], dims[0]).transpose(2, 1, 0)
+
+    def voxel_lookup(self, xyz):
+        p = _f32(xyz).reshape(-1, 3)
+        n = p.shape[0]
+        g = np.empty(n, dtype=np.int32)
+        lin = np.empty(n, dtype=np.int64)
+        m = np.empty(n, dtype=np.int32)
+        self._call(self.lib.mdx_voxel_lookup, self._h, ctypes.c_int(n), _ptr(p, ctypes.c_float),
+                   _ptr(g, ctypes.c_int), _ptr(lin, ctypes.c_longlong), _ptr(m, ctypes.c_int))
+        return g, lin, m
+
+    # ---------------------------------------------------------- source / tallies / run
+    def set_source(self, direction=DIR_BEAM, position=(0, 0, 0), a=(0, 0, 1), b=(0, 0, 0), c=(0, 0, 0),
+                   radius=0.0, energy=None, spectrum=None):
+        s = _Source()
+        s.direction = direction
+        s.position = _v3(position)
+        s.a, s.b, s.c = _v3(a), _v3(b), _v3(c)
+        s.radius = radius
+        keep = []
+        if spectrum is not None:
+            e = _f64(spectrum[0])
+            w = _f64(spectrum[1])
+            keep = [e, w]
+            s.spectrum = SPEC_POLY
+            s.n_bins = e.size
+            s.bin_energy = _ptr(e, ctypes.c_double)
+            s.bin_weight = _ptr(w, ctypes.c_double)
+        else:
+            s.spectrum = SPEC_MONO
+            s.energy = float(energy)
+        self._call(self.lib.mdx_set_source, self._h, ctypes.byref(s))
+        del keep
+
+    def set_tallies(self, tallies):
+        arr = (_Tally * max(1, len(tallies)))()
+        for i, t in enumerate(tallies):
+            q = arr[i]
+            q.kind = t["kind"]
+            q.p0 = _v3(t.get("p0", (0, 0, 0)))
+            q.p1 = _v3(t.get("p1", (0, 0, 0)))
+            q.p2 = _v3(t.get("p2", (0, 0, 0)))
+            q.normal = _v3(t.get("normal", (0, 0, 1)))
+            q.radius = t.get("radius", 0.0)
+            q.nx, q.ny = t.get("nx", 0), t.get("ny", 0)
+            q.hist_bins = t.get("hist_bins", 0)
+            q.hist_lo = t.get("hist_lo", 0.0)
+            q.hist_width = t.get("hist_width", 0.0)
+        self._call(self.lib.mdx_set_tallies, self._h, arr, ctypes.c_int(len(tallies)))
+        self.tallies = [dict(t) for t in tallies]
+
+    def enable_voxel_dose(self, on=True):
+        self._call(self.lib.mdx_enable_voxel_dose, self._h, ctypes.c_int(1 if on else 0))
+
+    def reset_tallies(self):
+        self._call(self.lib.mdx_reset_tallies, self._h)
+
+    def run(self, n, seed=1, first=0, kernel=KERNEL_PLAIN):
+        st = _Stats()
+        self._call(self.lib.mdx_run, self._h, ctypes.c_uint64(n), ctypes.c_uint64(seed), ctypes.c_uint64(first),
+                   ctypes.c_int(kernel), ctypes.byref(st))
+        out = {name: int(st.counters[i]) for i, name in enumerate(STAT_NAMES)}
+        out["kernel_ms"] = st.kernel_ms
+        return out
+
+    def tally(self, i):
+        t = self.tallies[i]
+        acc = np.zeros(NQ * NBUCKET * NFIELD)
+        hist = np.zeros(4 * t.get("hist_bins", 0)) if t.get("hist_bins", 0) and t["kind"] != TALLY_AACUBOID else None
+        image = np.zeros(3 * t.get("nx", 0) * t.get("ny", 0)) if t["kind"] == TALLY_IMAGE else None
+        self._call(self.lib.mdx_get_tally, self._h, ctypes.c_int(i), _ptr(acc, ctypes.c_double),
+                   None if hist is None else _ptr(hist, ctypes.c_double),
+                   None if image is None else _ptr(image, ctypes.c_double))
+        out = {"acc": acc.reshape(NQ, NBUCKET, NFIELD)}
+        if hist is not None:
+            out["hist"] = hist.reshape(4, -1)
+        if image is not None:
+            out["image"] = image.reshape(t["ny"], t["nx"], 3)
+        return out
+
+    def voxel_dose(self, g):
+        dims, _, _ = self.grid_info(g)
+        nv = dims[0] * dims[1] * dims[2]
+        ss = np.zeros(2 * nv)
+        cnt = np.zeros(nv, dtype=np.uint32)
+        self._call(self.lib.mdx_get_voxel_dose, self._h, ctypes.c_int(g), _ptr(ss, ctypes.c_double),
+                   _ptr(cnt, ctypes.c_uint32))
+        return ss[0::2], ss[1::2], cnt
+
+    def background_dose(self):
+        out = (ctypes.c_double * 3)()
+        self._call(self.lib.mdx_get_background_dose, self._h, out)
+        return tuple(out)
+
+
+def vector_stats(acc_qb):
+    """VectorValue statistics (quantity.cpp:27-94) from (n, sum, sum^2):
+    sum, sample variance, sigma of the sum sqrt(n s^2), mean and its sigma."""
+    n, s, s2 = acc_qb
+    if n < 2:
+        return {"count": n, "sum": s, "variance": float("nan"), "sum_std": float("nan"), "mean": s / n if n else 0.0}
+    var = (s2 - s * s / n) / (n - 1)
+    return {"count": n, "sum": s, "variance": var, "sum_std": (n * var) ** 0.5, "mean": s / n,
+            "mean_std": (var / n) ** 0.5}

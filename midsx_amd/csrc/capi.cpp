@@ -1,0 +1,929 @@
+// C-ABI of libmidsx_amd.so (declared in include/midsx_amd.h): context,
+// table build and upload, domain, source, tallies, runs and read-back.
+#include <dlfcn.h>
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/midsx_amd.h"
+#include "kernels.h"
+#include "mdx_types.h"
+#include "scene.h"
+#include "tables.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct DeviceError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct NoMemError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return;
+  if (e == hipErrorOutOfMemory) throw NoMemError(std::string(what) + ": " + hipGetErrorString(e));
+  throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Device allocation owned by the context.
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void alloc(size_t n) {
+    release();
+    if (n == 0) return;
+    hip_check(hipMalloc(&p, n), "hipMalloc");
+    bytes = n;
+  }
+  template <class T>
+  void upload(const T* src, size_t count, hipStream_t st) {
+    alloc(count * sizeof(T));
+    if (count) hip_check(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, st), "upload");
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+std::string default_data_dir() {
+  if (const char* e = std::getenv("MIDSX_AMD_DATA")) return e;
+  Dl_info info;
+  if (dladdr((void*)&default_data_dir, &info) && info.dli_fname) {
+    std::string so = info.dli_fname;
+    size_t k = so.rfind('/');
+    std::string dir = k == std::string::npos ? "." : so.substr(0, k);
+    return dir + "/../data";  // midsx_amd/_lib/libmidsx_amd.so -> midsx_amd/data
+  }
+  return "midsx_amd/data";
+}
+
+struct GridHost {
+  std::vector<uint8_t> ids;
+  int dims[3];
+  double spacing[3], origin[3];
+  DBuf d_ids, d_dose, d_dose_n;
+};
+
+struct TallyHost {
+  mdx_tally t;
+  DBuf acc, hist, image;
+};
+
+}  // namespace
+
+struct mdx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int num_cu = 256;
+  std::string data_dir;
+  bool registry_loaded = false;
+  std::map<int, mdx::MaterialDef> registry;
+
+  // tables
+  std::vector<std::unique_ptr<mdx::MaterialTables>> mats;
+  mdx::Table majorant;
+  mdx::Packed packed;
+  DBuf d_xs, d_nodes, d_hash, d_rE, d_rinv, d_roff, d_rrows, d_tables;
+  bool tables_ready = false;
+
+  // domain
+  bool domain_set = false;
+  double dim_space[3] = {0, 0, 0};
+  int background_id = -1;
+  std::vector<std::unique_ptr<GridHost>> grids;
+  bool voxel_dose = false;
+  DBuf d_bg, d_domain;
+
+  // source
+  bool source_set = false;
+  mdx_source_dev src{};
+  DBuf d_cdf, d_energies, d_source;
+
+  // tallies
+  std::vector<std::unique_ptr<TallyHost>> tallies;
+  DBuf d_tallies;
+  DBuf d_counters;
+
+  void ensure_registry() {
+    if (registry_loaded) return;
+    registry = mdx::load_registry(data_dir + "/materials.csv");
+    registry_loaded = true;
+  }
+  int slot_of(int id) const { return (id >= 0 && id < 256) ? packed.id_to_slot[id] : 255; }
+};
+
+namespace {
+
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    g_last_error.clear();
+    return MDX_OK;
+  } catch (const std::invalid_argument& e) {
+    g_last_error = e.what();
+    return MDX_ERR_INVALID;
+  } catch (const std::out_of_range& e) {
+    g_last_error = e.what();
+    return MDX_ERR_RANGE;
+  } catch (const DeviceError& e) {
+    g_last_error = e.what();
+    return MDX_ERR_DEVICE;
+  } catch (const NoMemError& e) {
+    g_last_error = e.what();
+    return MDX_ERR_NOMEM;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return MDX_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return MDX_ERR_RUNTIME;
+  } catch (...) {
+    g_last_error = "unknown error";
+    return MDX_ERR_RUNTIME;
+  }
+}
+
+void need(bool ok, const char* msg) {
+  if (!ok) throw std::invalid_argument(msg);
+}
+
+void set_device(mdx_ctx* c) {
+  if (c->device < 0) throw std::invalid_argument("host-only context (device -1) cannot run device work");
+  hip_check(hipSetDevice(c->device), "hipSetDevice");
+}
+
+void upload_domain(mdx_ctx* c) {
+  mdx_domain_dev D{};
+  for (int k = 0; k < 3; ++k) D.dim_space[k] = (float)c->dim_space[k];
+  D.bg_slot = c->tables_ready ? c->slot_of(c->background_id) : 0;
+  D.n_grids = (int)c->grids.size();
+  for (size_t g = 0; g < c->grids.size(); ++g) {
+    GridHost& G = *c->grids[g];
+    mdx_grid_dev& d = D.grids[g];
+    size_t nv = (size_t)G.dims[0] * G.dims[1] * G.dims[2];
+    for (int k = 0; k < 3; ++k) {
+      d.origin[k] = (float)G.origin[k];
+      d.spacing[k] = (float)G.spacing[k];
+      d.dim[k] = (float)(G.dims[k] * G.spacing[k]);  // voxel_grid.cpp:175-177
+      d.n[k] = G.dims[k];
+    }
+    d.ids = G.d_ids.as<uint8_t>();
+    if (c->voxel_dose) {
+      if (G.d_dose.bytes != nv * 16) {
+        G.d_dose.alloc(nv * 16);
+        G.d_dose_n.alloc(nv * 4);
+        hip_check(mdx_launch::zero(G.d_dose.as<double>(), 2 * nv, c->stream), "zero dose");
+        hip_check(mdx_launch::zero_u32(G.d_dose_n.as<uint32_t>(), nv, c->stream), "zero dose count");
+      }
+      d.dose = G.d_dose.as<double>();
+      d.dose_n = G.d_dose_n.as<uint32_t>();
+    } else {
+      G.d_dose.release();
+      G.d_dose_n.release();
+    }
+  }
+  if (!c->d_bg.p) {
+    c->d_bg.alloc(3 * sizeof(double));
+    hip_check(mdx_launch::zero(c->d_bg.as<double>(), 3, c->stream), "zero bg");
+  }
+  D.bg_dose = c->d_bg.as<double>();
+  c->d_domain.upload(&D, 1, c->stream);
+  hip_check(hipStreamSynchronize(c->stream), "upload domain");
+}
+
+void check_materials(mdx_ctx* c) {
+  need(c->tables_ready, "physics tables not built (mdx_build_tables)");
+  need(c->domain_set, "domain not set (mdx_set_domain / mdx_load_scene)");
+  if (c->slot_of(c->background_id) == 255)
+    throw std::out_of_range("background material " + std::to_string(c->background_id) + " has no tables");
+  for (auto& g : c->grids) {
+    bool seen[256] = {false};
+    for (uint8_t v : g->ids) seen[v] = true;
+    for (int v = 0; v < 256; ++v)
+      if (seen[v] && c->slot_of(v) == 255)
+        throw std::out_of_range("grid material " + std::to_string(v) + " has no tables");
+  }
+}
+
+void build_and_upload(mdx_ctx* c, const std::vector<int>& ids) {
+  c->ensure_registry();
+  need(!ids.empty(), "no materials given");
+  need(ids.size() <= MDX_MAX_MATERIALS, "too many materials");
+  std::vector<int> uniq;
+  for (int id : ids)
+    if (std::find(uniq.begin(), uniq.end(), id) == uniq.end()) uniq.push_back(id);
+  int threads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::unique_ptr<mdx::MaterialTables>> mats;
+  for (int id : uniq) {
+    auto it = c->registry.find(id);
+    if (it == c->registry.end())
+      throw std::runtime_error("material ID " + std::to_string(id) + " not in the registry");
+    mats.push_back(std::make_unique<mdx::MaterialTables>(
+        mdx::build_material(it->second, c->data_dir + "/epdl", true, threads)));
+  }
+  std::vector<const mdx::MaterialTables*> mp;
+  for (auto& m : mats) mp.push_back(m.get());
+  c->majorant = mdx::build_majorant(mp);
+  mdx::pack_all(c->packed, mp, c->majorant, 1.0e3, 5.0e5);
+  c->mats = std::move(mats);
+  if (c->device < 0) {  // host-only context: FP64 tables and packing, no upload
+    c->tables_ready = true;
+    return;
+  }
+  set_device(c);
+  mdx::Packed& P = c->packed;
+  c->d_xs.upload(P.xs.data(), P.xs.size(), c->stream);
+  c->d_nodes.upload(P.nodes.data(), P.nodes.size(), c->stream);
+  c->d_hash.upload(P.hash.data(), P.hash.size(), c->stream);
+  c->d_rE.upload(P.rita_E.data(), P.rita_E.size(), c->stream);
+  c->d_rinv.upload(P.rita_inv_dlnE.data(), P.rita_inv_dlnE.size(), c->stream);
+  c->d_roff.upload(P.rita_row_off.data(), P.rita_row_off.size(), c->stream);
+  c->d_rrows.upload(P.rita_rows.data(), P.rita_rows.size(), c->stream);
+  mdx_tables_dev T{};
+  T.xs = c->d_xs.as<float>();
+  T.nodes = c->d_nodes.as<mdx_node>();
+  T.hash = c->d_hash.as<int32_t>();
+  T.rita_E = c->d_rE.as<float>();
+  T.rita_inv_dlnE = c->d_rinv.as<float>();
+  T.rita_row_off = c->d_roff.as<int32_t>();
+  T.rita_rows = c->d_rrows.as<mdx_f4>();
+  T.majorant = P.majorant;
+  T.n_mat = (int)P.mats.size();
+  for (size_t s = 0; s < P.mats.size(); ++s) T.mat[s] = P.mats[s];
+  std::memcpy(T.id_to_slot, P.id_to_slot, 256);
+  c->d_tables.upload(&T, 1, c->stream);
+  hip_check(hipStreamSynchronize(c->stream), "upload tables");
+  c->tables_ready = true;
+  if (c->domain_set) upload_domain(c);
+}
+
+void upload_tallies(mdx_ctx* c) {
+  mdx_tallies_dev TL{};
+  TL.n = (int)c->tallies.size();
+  for (size_t i = 0; i < c->tallies.size(); ++i) {
+    TallyHost& H = *c->tallies[i];
+    const mdx_tally& t = H.t;
+    mdx_tally_dev& d = TL.t[i];
+    d.kind = t.kind;
+    for (int k = 0; k < 3; ++k) {
+      d.p0[k] = (float)t.p0[k];
+      d.p1[k] = (float)t.p1[k];
+      d.p2[k] = (float)t.p2[k];
+      d.nrm[k] = (float)t.normal[k];
+    }
+    d.radius = (float)t.radius;
+    d.e1e1 = (float)(t.p1[0] * t.p1[0] + t.p1[1] * t.p1[1] + t.p1[2] * t.p1[2]);
+    d.e2e2 = (float)(t.p2[0] * t.p2[0] + t.p2[1] * t.p2[1] + t.p2[2] * t.p2[2]);
+    d.nx = t.nx;
+    d.ny = t.ny;
+    d.hist_bins = t.kind == MDX_TALLY_AACUBOID ? 0 : t.hist_bins;
+    d.hist_lo = (float)t.hist_lo;
+    d.hist_inv_w = t.hist_width > 0 ? (float)(1.0 / t.hist_width) : 0.0f;
+    d.acc = H.acc.as<double>();
+    d.hist = H.hist.as<double>();
+    d.image = H.image.as<double>();
+  }
+  c->d_tallies.upload(&TL, 1, c->stream);
+  hip_check(hipStreamSynchronize(c->stream), "upload tallies");
+}
+
+void zero_tallies(mdx_ctx* c) {
+  for (auto& h : c->tallies) {
+    hip_check(mdx_launch::zero(h->acc.as<double>(), h->acc.bytes / 8, c->stream), "zero acc");
+    hip_check(mdx_launch::zero(h->hist.as<double>(), h->hist.bytes / 8, c->stream), "zero hist");
+    hip_check(mdx_launch::zero(h->image.as<double>(), h->image.bytes / 8, c->stream), "zero image");
+  }
+  for (auto& g : c->grids) {
+    hip_check(mdx_launch::zero(g->d_dose.as<double>(), g->d_dose.bytes / 8, c->stream), "zero dose");
+    hip_check(mdx_launch::zero_u32(g->d_dose_n.as<uint32_t>(), g->d_dose_n.bytes / 4, c->stream), "zero n");
+  }
+  if (c->d_bg.p) hip_check(mdx_launch::zero(c->d_bg.as<double>(), 3, c->stream), "zero bg");
+  hip_check(hipStreamSynchronize(c->stream), "reset tallies");
+}
+
+const mdx::MaterialTables& host_material(mdx_ctx* c, int id) {
+  need(c->tables_ready, "physics tables not built");
+  for (auto& m : c->mats)
+    if (m->def.id == id) return *m;
+  throw std::out_of_range("material " + std::to_string(id) + " has no tables");
+}
+
+template <class T>
+struct DevArray {
+  DBuf b;
+  DevArray(const T* src, size_t n, hipStream_t st) {
+    if (src) b.upload(src, n, st);
+    else b.alloc(n * sizeof(T));
+  }
+  T* p() { return b.as<T>(); }
+};
+
+}  // namespace
+
+extern "C" {
+
+const char* mdx_last_error(void) { return g_last_error.c_str(); }
+
+int mdx_create(int device, mdx_ctx** out) {
+  return guard([&] {
+    need(out != nullptr, "out is NULL");
+    *out = nullptr;
+    if (device == -1) {  // host-only context: registry and FP64 tables only
+      auto c = std::make_unique<mdx_ctx>();
+      c->device = -1;
+      c->data_dir = default_data_dir();
+      *out = c.release();
+      return;
+    }
+    int n = 0;
+    hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    if (device < 0 || device >= n) throw std::out_of_range("device index out of range");
+    auto c = std::make_unique<mdx_ctx>();
+    c->device = device;
+    c->data_dir = default_data_dir();
+    set_device(c.get());
+    hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreate(&c->ev0), "hipEventCreate");
+    hip_check(hipEventCreate(&c->ev1), "hipEventCreate");
+    hipDeviceProp_t prop;
+    hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+    c->num_cu = prop.multiProcessorCount;
+    c->d_counters.alloc(MDX_NCOUNTER * sizeof(unsigned long long));
+    *out = c.release();
+  });
+}
+
+int mdx_destroy(mdx_ctx* c) {
+  return guard([&] {
+    if (!c) return;
+    if (c->device < 0) {
+      delete c;
+      return;
+    }
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    hipStream_t st = c->stream;
+    hipEvent_t e0 = c->ev0, e1 = c->ev1;
+    delete c;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+  });
+}
+
+int mdx_set_data_dir(mdx_ctx* c, const char* dir) {
+  return guard([&] {
+    need(c && dir, "NULL argument");
+    c->data_dir = dir;
+    c->registry_loaded = false;
+    c->registry.clear();
+  });
+}
+
+int mdx_material_id(mdx_ctx* c, const char* name, int* id) {
+  return guard([&] {
+    need(c && name && id, "NULL argument");
+    c->ensure_registry();
+    for (auto& kv : c->registry)
+      if (kv.second.name == name) {
+        *id = kv.first;
+        return;
+      }
+    throw std::runtime_error(std::string("unknown material name '") + name + "'");
+  });
+}
+
+int mdx_material_name(mdx_ctx* c, int id, char* buf, int buflen) {
+  return guard([&] {
+    need(c && buf && buflen > 0, "NULL argument");
+    c->ensure_registry();
+    auto it = c->registry.find(id);
+    if (it == c->registry.end()) throw std::runtime_error("unknown material ID " + std::to_string(id));
+    std::snprintf(buf, (size_t)buflen, "%s", it->second.name.c_str());
+  });
+}
+
+int mdx_build_tables(mdx_ctx* c, const int* ids, int n) {
+  return guard([&] {
+    need(c && ids && n > 0, "bad material list");
+    build_and_upload(c, std::vector<int>(ids, ids + n));
+  });
+}
+
+int mdx_build_tables_for_domain(mdx_ctx* c) {
+  return guard([&] {
+    need(c && c->domain_set, "domain not set");
+    std::vector<int> ids{c->background_id};
+    for (auto& g : c->grids) {
+      bool seen[256] = {false};
+      for (uint8_t v : g->ids) seen[v] = true;
+      for (int v = 0; v < 256; ++v)
+        if (seen[v] && std::find(ids.begin(), ids.end(), v) == ids.end()) ids.push_back(v);
+    }
+    build_and_upload(c, ids);
+  });
+}
+
+int mdx_set_domain(mdx_ctx* c, const double dim_space[3], int background_id) {
+  return guard([&] {
+    need(c && dim_space, "NULL argument");
+    need(background_id >= 0 && background_id < 256, "background material ID must be in [0, 255]");
+    for (int k = 0; k < 3; ++k) {
+      need(dim_space[k] > 0, "dim_space must be positive");
+      c->dim_space[k] = dim_space[k];
+    }
+    c->background_id = background_id;
+    c->grids.clear();
+    c->domain_set = true;
+    if (c->device >= 0) {
+      set_device(c);
+      upload_domain(c);
+    }
+  });
+}
+
+int mdx_add_grid(mdx_ctx* c, const uint8_t* ids, const int dims[3], const double spacing[3],
+                 const double origin[3]) {
+  return guard([&] {
+    need(c && ids && dims && spacing && origin, "NULL argument");
+    need(c->domain_set, "mdx_set_domain must come first");
+    need((int)c->grids.size() < MDX_MAX_GRIDS, "too many grids");
+    auto g = std::make_unique<GridHost>();
+    size_t nv = 1;
+    for (int k = 0; k < 3; ++k) {
+      need(dims[k] > 0 && spacing[k] > 0, "grid dims and spacing must be positive");
+      g->dims[k] = dims[k];
+      g->spacing[k] = spacing[k];
+      g->origin[k] = origin[k];
+      nv *= (size_t)dims[k];
+    }
+    g->ids.assign(ids, ids + nv);
+    if (c->device >= 0) {
+      set_device(c);
+      g->d_ids.upload(g->ids.data(), nv, c->stream);
+    }
+    c->grids.push_back(std::move(g));
+    if (c->device >= 0) upload_domain(c);
+  });
+}
+
+int mdx_load_scene(mdx_ctx* c, const char* json_path) {
+  return guard([&] {
+    need(c && json_path, "NULL argument");
+    mdx::Scene sc = mdx::read_scene(json_path);
+    c->ensure_registry();
+    int bg = -1;
+    for (auto& kv : c->registry)
+      if (kv.second.name == sc.background_material_name) bg = kv.first;
+    if (bg < 0) throw std::runtime_error("unknown background material '" + sc.background_material_name + "'");
+    std::vector<mdx::NiftiGrid> grids;
+    for (auto& g : sc.grids) grids.push_back(mdx::read_nifti(g.path));
+    double ds[3] = {sc.dim_space[0], sc.dim_space[1], sc.dim_space[2]};
+    for (int k = 0; k < 3; ++k) need(ds[k] > 0, "dim_space must be positive");
+    c->dim_space[0] = ds[0];
+    c->dim_space[1] = ds[1];
+    c->dim_space[2] = ds[2];
+    c->background_id = bg;
+    c->grids.clear();
+    c->domain_set = true;
+    for (size_t i = 0; i < grids.size(); ++i) {
+      need(i < MDX_MAX_GRIDS, "too many grids");
+      auto g = std::make_unique<GridHost>();
+      for (int k = 0; k < 3; ++k) {
+        g->dims[k] = grids[i].dims[k];
+        g->spacing[k] = grids[i].spacing[k];
+        g->origin[k] = sc.grids[i].origin[k];
+      }
+      g->ids = std::move(grids[i].ids);
+      if (c->device >= 0) {
+        set_device(c);
+        g->d_ids.upload(g->ids.data(), g->ids.size(), c->stream);
+      }
+      c->grids.push_back(std::move(g));
+    }
+    if (c->device >= 0) upload_domain(c);
+  });
+}
+
+int mdx_num_grids(mdx_ctx* c, int* n) {
+  return guard([&] {
+    need(c && n, "NULL argument");
+    *n = (int)c->grids.size();
+  });
+}
+
+int mdx_grid_info(mdx_ctx* c, int grid, int dims[3], double spacing[3], double origin[3]) {
+  return guard([&] {
+    need(c, "NULL context");
+    if (grid < 0 || grid >= (int)c->grids.size()) throw std::out_of_range("grid index out of range");
+    for (int k = 0; k < 3; ++k) {
+      if (dims) dims[k] = c->grids[grid]->dims[k];
+      if (spacing) spacing[k] = c->grids[grid]->spacing[k];
+      if (origin) origin[k] = c->grids[grid]->origin[k];
+    }
+  });
+}
+
+int mdx_get_grid_ids(mdx_ctx* c, int grid, uint8_t* out) {
+  return guard([&] {
+    need(c && out, "NULL argument");
+    if (grid < 0 || grid >= (int)c->grids.size()) throw std::out_of_range("grid index out of range");
+    std::memcpy(out, c->grids[grid]->ids.data(), c->grids[grid]->ids.size());
+  });
+}
+
+int mdx_set_source(mdx_ctx* c, const mdx_source* s) {
+  return guard([&] {
+    need(c && s, "NULL argument");
+    mdx_source_dev d{};
+    d.dir_kind = s->direction;
+    d.spec_kind = s->spectrum;
+    for (int k = 0; k < 3; ++k) d.pos[k] = (float)s->position[k];
+    switch (s->direction) {
+      case MDX_DIR_BEAM:
+        for (int k = 0; k < 3; ++k) d.a[k] = (float)s->a[k];
+        break;
+      case MDX_DIR_ISOTROPIC:
+        break;
+      case MDX_DIR_RECT_ISOTROPIC: {
+        double dot = s->b[0] * s->c[0] + s->b[1] * s->c[1] + s->b[2] * s->c[2];
+        if (std::fabs(dot) >= 1e-9) throw std::invalid_argument("Edges are not orthogonal");  // source.cpp:134-143
+        for (int k = 0; k < 3; ++k) {
+          d.a[k] = (float)s->a[k];
+          d.b[k] = (float)s->b[k];
+          d.c[k] = (float)s->c[k];
+        }
+        break;
+      }
+      case MDX_DIR_DISC_ISOTROPIC: {
+        // source.cpp:117-131: u = normal x x_hat (x_hat -> y_hat if normal _|_ x_hat), v = normal x u
+        const double* n = s->b;
+        double u[3], v[3];
+        if (std::fabs(n[0]) < 1e-9) {
+          u[0] = n[1] * 0 - n[2] * 1; u[1] = n[2] * 0 - n[0] * 0; u[2] = n[0] * 1 - n[1] * 0;
+        } else {
+          u[0] = 0; u[1] = n[2]; u[2] = -n[1];
+        }
+        double lu = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        need(lu > 0, "disc normal must be non-zero");
+        for (double& x : u) x /= lu;
+        v[0] = n[1] * u[2] - n[2] * u[1];
+        v[1] = n[2] * u[0] - n[0] * u[2];
+        v[2] = n[0] * u[1] - n[1] * u[0];
+        double lv = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        for (double& x : v) x /= lv;
+        for (int k = 0; k < 3; ++k) {
+          d.a[k] = (float)s->a[k];
+          d.b[k] = (float)u[k];
+          d.c[k] = (float)v[k];
+        }
+        d.radius = (float)s->radius;
+        break;
+      }
+      default:
+        throw std::invalid_argument("unknown directionality");
+    }
+    set_device(c);
+    if (s->spectrum == MDX_SPEC_MONO) {
+      need(s->energy > 0, "energy must be positive");
+      d.energy = (float)s->energy;
+    } else if (s->spectrum == MDX_SPEC_POLY) {
+      need(s->n_bins > 0 && s->bin_energy && s->bin_weight, "polyenergetic spectrum needs bins");
+      // probability_dist.cpp:11-60: normalise, cumulative sum
+      double sum = 0;
+      for (int i = 0; i < s->n_bins; ++i) sum += s->bin_weight[i];
+      need(sum > 0, "spectrum weights sum to zero");
+      std::vector<float> cdf(s->n_bins), en(s->n_bins);
+      double acc = 0;
+      for (int i = 0; i < s->n_bins; ++i) {
+        acc += sum != 1.0 ? s->bin_weight[i] / sum : s->bin_weight[i];
+        cdf[i] = (float)acc;
+        en[i] = (float)s->bin_energy[i];
+      }
+      c->d_cdf.upload(cdf.data(), cdf.size(), c->stream);
+      c->d_energies.upload(en.data(), en.size(), c->stream);
+      d.n_bins = s->n_bins;
+      d.cdf = c->d_cdf.as<float>();
+      d.energies = c->d_energies.as<float>();
+    } else {
+      throw std::invalid_argument("unknown spectrum kind");
+    }
+    c->src = d;
+    c->d_source.upload(&d, 1, c->stream);
+    hip_check(hipStreamSynchronize(c->stream), "upload source");
+    c->source_set = true;
+  });
+}
+
+int mdx_set_tallies(mdx_ctx* c, const mdx_tally* t, int n) {
+  return guard([&] {
+    need(c && (t || n == 0), "NULL argument");
+    need(n >= 0 && n <= MDX_MAX_TALLIES, "too many tallies");
+    std::vector<std::unique_ptr<TallyHost>> v;
+    set_device(c);
+    for (int i = 0; i < n; ++i) {
+      auto h = std::make_unique<TallyHost>();
+      h->t = t[i];
+      mdx_tally& q = h->t;
+      if (q.kind == MDX_TALLY_RECT || q.kind == MDX_TALLY_IMAGE) {
+        double dot = q.p1[0] * q.p2[0] + q.p1[1] * q.p2[1] + q.p1[2] * q.p2[2];
+        if (std::fabs(dot) >= 1e-9) throw std::invalid_argument("Edges are not orthogonal");  // surface_tally.cpp:80-84
+        double nx = q.p1[1] * q.p2[2] - q.p1[2] * q.p2[1];
+        double ny = q.p1[2] * q.p2[0] - q.p1[0] * q.p2[2];
+        double nz = q.p1[0] * q.p2[1] - q.p1[1] * q.p2[0];
+        double l = std::sqrt(nx * nx + ny * ny + nz * nz);
+        need(l > 0, "degenerate rectangle");
+        q.normal[0] = nx / l;
+        q.normal[1] = ny / l;
+        q.normal[2] = nz / l;
+        if (q.kind == MDX_TALLY_IMAGE) need(q.nx > 0 && q.ny > 0, "image needs pixels");
+      } else if (q.kind == MDX_TALLY_DISC) {
+        need(q.radius > 0, "disc radius must be positive");
+      } else if (q.kind != MDX_TALLY_AACUBOID) {
+        throw std::invalid_argument("unknown tally kind");
+      }
+      if (q.kind != MDX_TALLY_AACUBOID && q.hist_bins > 0) need(q.hist_width > 0, "histogram width must be positive");
+      h->acc.alloc(MDX_TALLY_ACC_SIZE * sizeof(double));
+      if (q.kind != MDX_TALLY_AACUBOID && q.hist_bins > 0) h->hist.alloc((size_t)4 * q.hist_bins * sizeof(double));
+      if (q.kind == MDX_TALLY_IMAGE) h->image.alloc((size_t)3 * q.nx * q.ny * sizeof(double));
+      v.push_back(std::move(h));
+    }
+    c->tallies = std::move(v);
+    upload_tallies(c);
+    zero_tallies(c);
+  });
+}
+
+int mdx_enable_voxel_dose(mdx_ctx* c, int enable) {
+  return guard([&] {
+    need(c, "NULL context");
+    c->voxel_dose = enable != 0;
+    set_device(c);
+    if (c->domain_set) upload_domain(c);
+  });
+}
+
+int mdx_reset_tallies(mdx_ctx* c) {
+  return guard([&] {
+    need(c, "NULL context");
+    set_device(c);
+    zero_tallies(c);
+  });
+}
+
+int mdx_run(mdx_ctx* c, uint64_t n, uint64_t seed, uint64_t first, int kernel, mdx_stats* stats) {
+  return guard([&] {
+    need(c, "NULL context");
+    check_materials(c);
+    need(c->source_set, "source not set (mdx_set_source)");
+    need(kernel == MDX_KERNEL_PLAIN, "unknown kernel");
+    set_device(c);
+    if (!c->d_tallies.p) upload_tallies(c);
+    unsigned long long* cnt = c->d_counters.as<unsigned long long>();
+    hip_check(mdx_launch::zero_u64(cnt, MDX_NCOUNTER, c->stream), "zero counters");
+    hip_check(hipEventRecord(c->ev0, c->stream), "hipEventRecord");
+    if (n > 0) {
+      hip_check(mdx_launch::transport_plain(c->d_tables.as<mdx_tables_dev>(), c->d_domain.as<mdx_domain_dev>(),
+                                            c->d_source.as<mdx_source_dev>(), c->d_tallies.as<mdx_tallies_dev>(),
+                                            first, n, seed, 0u, cnt, c->num_cu * 8, c->stream),
+                "transport launch");
+    }
+    hip_check(hipEventRecord(c->ev1, c->stream), "hipEventRecord");
+    hip_check(hipStreamSynchronize(c->stream), "transport");
+    if (stats) {
+      std::memset(stats, 0, sizeof *stats);
+      unsigned long long h[MDX_NCOUNTER];
+      hip_check(hipMemcpy(h, cnt, sizeof h, hipMemcpyDeviceToHost), "counters");
+      for (int i = 0; i < MDX_NCOUNTER && i < MDX_STAT_NCOUNTERS; ++i) stats->counters[i] = h[i];
+      float ms = 0;
+      hip_check(hipEventElapsedTime(&ms, c->ev0, c->ev1), "hipEventElapsedTime");
+      stats->kernel_ms = ms;
+    }
+  });
+}
+
+int mdx_get_tally(mdx_ctx* c, int i, double* acc, double* hist, double* image) {
+  return guard([&] {
+    need(c, "NULL context");
+    if (i < 0 || i >= (int)c->tallies.size()) throw std::out_of_range("tally index out of range");
+    set_device(c);
+    TallyHost& h = *c->tallies[i];
+    if (acc) hip_check(hipMemcpy(acc, h.acc.p, h.acc.bytes, hipMemcpyDeviceToHost), "read tally");
+    if (hist && h.hist.p) hip_check(hipMemcpy(hist, h.hist.p, h.hist.bytes, hipMemcpyDeviceToHost), "read hist");
+    if (image && h.image.p) hip_check(hipMemcpy(image, h.image.p, h.image.bytes, hipMemcpyDeviceToHost), "read image");
+  });
+}
+
+int mdx_get_voxel_dose(mdx_ctx* c, int grid, double* ss, uint32_t* count) {
+  return guard([&] {
+    need(c, "NULL context");
+    if (grid < 0 || grid >= (int)c->grids.size()) throw std::out_of_range("grid index out of range");
+    need(c->voxel_dose, "voxel dose is not enabled");
+    set_device(c);
+    GridHost& g = *c->grids[grid];
+    if (ss) hip_check(hipMemcpy(ss, g.d_dose.p, g.d_dose.bytes, hipMemcpyDeviceToHost), "read dose");
+    if (count) hip_check(hipMemcpy(count, g.d_dose_n.p, g.d_dose_n.bytes, hipMemcpyDeviceToHost), "read dose n");
+  });
+}
+
+int mdx_get_background_dose(mdx_ctx* c, double out[3]) {
+  return guard([&] {
+    need(c && out, "NULL argument");
+    set_device(c);
+    hip_check(hipMemcpy(out, c->d_bg.p, 3 * sizeof(double), hipMemcpyDeviceToHost), "read bg");
+  });
+}
+
+int mdx_interpolate(mdx_ctx* c, int id, int which, int n, const float* x, float* out) {
+  return guard([&] {
+    need(c && x && out && n >= 0, "bad argument");
+    need(c->tables_ready, "physics tables not built");
+    need(which >= 0 && which <= MDX_SF_INCOHERENT, "unknown table selector");
+    int slot = -1;
+    if (id >= 0) {
+      slot = c->slot_of(id);
+      if (slot == 255) throw std::out_of_range("material has no tables");
+    }
+    if (n == 0) return;
+    set_device(c);
+    DevArray<float> dx(x, n, c->stream), dout(nullptr, n, c->stream);
+    hip_check(mdx_launch::eval_curve(c->d_tables.as<mdx_tables_dev>(), slot, which, n, dx.p(), dout.p(), c->stream),
+              "eval");
+    hip_check(hipMemcpyAsync(out, dout.p(), n * sizeof(float), hipMemcpyDeviceToHost, c->stream), "read");
+    hip_check(hipStreamSynchronize(c->stream), "eval");
+  });
+}
+
+int mdx_interpolate_host(mdx_ctx* c, int id, int which, int n, const double* x, double* out) {
+  return guard([&] {
+    need(c && x && out && n >= 0, "bad argument");
+    if (id < 0) {
+      need(c->tables_ready, "physics tables not built");
+      mdx::LogLogLinear f(c->majorant.x, c->majorant.y);
+      for (int i = 0; i < n; ++i) out[i] = f(x[i]);
+      return;
+    }
+    const mdx::MaterialTables& m = host_material(c, id);
+    for (int i = 0; i < n; ++i) {
+      switch (which) {
+        case MDX_XS_TOTAL: out[i] = m.total_i(x[i]); break;
+        case MDX_XS_COHERENT: out[i] = m.coh_i(x[i]); break;
+        case MDX_XS_INCOHERENT: out[i] = m.incoh_i(x[i]); break;
+        case MDX_XS_PHOTOELECTRIC: out[i] = m.photo_i(x[i]); break;
+        case MDX_SF_INCOHERENT: out[i] = m.sf_i(x[i]); break;
+        case MDX_FF_COHERENT: out[i] = m.ff_i(x[i]); break;
+        default: throw std::invalid_argument("unknown table selector");
+      }
+    }
+  });
+}
+
+int mdx_get_table(mdx_ctx* c, int id, int which, int* n, double* x, double* y) {
+  return guard([&] {
+    need(c && n, "NULL argument");
+    const mdx::Table* t = nullptr;
+    if (id < 0) {
+      need(c->tables_ready, "physics tables not built");
+      t = &c->majorant;
+    } else {
+      const mdx::MaterialTables& m = host_material(c, id);
+      switch (which) {
+        case MDX_XS_TOTAL: t = &m.total; break;
+        case MDX_XS_COHERENT: t = &m.coh; break;
+        case MDX_XS_INCOHERENT: t = &m.incoh; break;
+        case MDX_XS_PHOTOELECTRIC: t = &m.photo; break;
+        case MDX_SF_INCOHERENT: t = &m.sf; break;
+        case MDX_FF_COHERENT: t = &m.ff; break;
+        default: throw std::invalid_argument("unknown table selector");
+      }
+    }
+    if (!x) {
+      *n = (int)t->x.size();
+      return;
+    }
+    int m = std::min(*n, (int)t->x.size());
+    std::memcpy(x, t->x.data(), m * sizeof(double));
+    if (y) std::memcpy(y, t->y.data(), m * sizeof(double));
+    *n = m;
+  });
+}
+
+int mdx_sample_rayleigh(mdx_ctx* c, int id, int n, const float* E, const float* y, float* mu) {
+  return guard([&] {
+    need(c && E && y && mu && n >= 0, "bad argument");
+    need(c->tables_ready, "physics tables not built");
+    int slot = c->slot_of(id);
+    if (slot == 255) throw std::out_of_range("material has no tables");
+    if (n == 0) return;
+    set_device(c);
+    DevArray<float> dE(E, n, c->stream), dy(y, n, c->stream), dmu(nullptr, n, c->stream);
+    hip_check(mdx_launch::rita(c->d_tables.as<mdx_tables_dev>(), slot, n, dE.p(), dy.p(), dmu.p(), c->stream), "rita");
+    hip_check(hipMemcpyAsync(mu, dmu.p(), n * sizeof(float), hipMemcpyDeviceToHost, c->stream), "read");
+    hip_check(hipStreamSynchronize(c->stream), "rita");
+  });
+}
+
+int mdx_sample_rayleigh_host(mdx_ctx* c, int id, int n, const double* E, const double* y, double* mu) {
+  return guard([&] {
+    need(c && E && y && mu && n >= 0, "bad argument");
+    const mdx::MaterialTables& m = host_material(c, id);
+    for (int i = 0; i < n; ++i) mu[i] = m.rita.sample(E[i], y[i]);
+  });
+}
+
+int mdx_get_rita_row(mdx_ctx* c, int id, int ie, int* rows, double* energy, double* x, double* y, double* a,
+                     double* b) {
+  return guard([&] {
+    need(c && rows, "NULL argument");
+    const mdx::MaterialTables& m = host_material(c, id);
+    if (ie < 0 || ie >= (int)m.rita.energies.size()) throw std::out_of_range("energy index out of range");
+    if (energy) *energy = m.rita.energies[ie];
+    if (!x) {
+      *rows = (int)m.rita.x[ie].size();
+      return;
+    }
+    int r = std::min(*rows, (int)m.rita.x[ie].size());
+    for (int i = 0; i < r; ++i) {
+      x[i] = m.rita.x[ie][i];
+      if (y) y[i] = m.rita.y[ie][i];
+      if (a) a[i] = i < (int)m.rita.a[ie].size() ? m.rita.a[ie][i] : 0.0;
+      if (b) b[i] = i < (int)m.rita.b[ie].size() ? m.rita.b[ie][i] : 0.0;
+    }
+    *rows = r;
+  });
+}
+
+int mdx_sample_compton(mdx_ctx* c, int id, int n, const float* E, uint64_t seed, float* mu, float* Ep) {
+  return guard([&] {
+    need(c && E && mu && Ep && n >= 0, "bad argument");
+    need(c->tables_ready, "physics tables not built");
+    int slot = c->slot_of(id);
+    if (slot == 255) throw std::out_of_range("material has no tables");
+    if (n == 0) return;
+    set_device(c);
+    DevArray<float> dE(E, n, c->stream), dmu(nullptr, n, c->stream), dEp(nullptr, n, c->stream);
+    hip_check(mdx_launch::kahn(c->d_tables.as<mdx_tables_dev>(), slot, n, dE.p(), seed, dmu.p(), dEp.p(), c->stream),
+              "kahn");
+    hip_check(hipMemcpyAsync(mu, dmu.p(), n * sizeof(float), hipMemcpyDeviceToHost, c->stream), "read");
+    hip_check(hipMemcpyAsync(Ep, dEp.p(), n * sizeof(float), hipMemcpyDeviceToHost, c->stream), "read");
+    hip_check(hipStreamSynchronize(c->stream), "kahn");
+  });
+}
+
+int mdx_voxel_lookup(mdx_ctx* c, int n, const float* xyz, int* grid, long long* lin, int* mat) {
+  return guard([&] {
+    need(c && xyz && grid && lin && mat && n >= 0, "bad argument");
+    check_materials(c);
+    if (n == 0) return;
+    set_device(c);
+    DevArray<float> dp(xyz, 3 * (size_t)n, c->stream);
+    DevArray<int> dg(nullptr, n, c->stream), dm(nullptr, n, c->stream);
+    DevArray<long long> dl(nullptr, n, c->stream);
+    hip_check(mdx_launch::voxel(c->d_tables.as<mdx_tables_dev>(), c->d_domain.as<mdx_domain_dev>(), n, dp.p(), dg.p(),
+                                dl.p(), dm.p(), c->stream),
+              "voxel");
+    hip_check(hipMemcpyAsync(grid, dg.p(), n * sizeof(int), hipMemcpyDeviceToHost, c->stream), "read");
+    hip_check(hipMemcpyAsync(lin, dl.p(), n * sizeof(long long), hipMemcpyDeviceToHost, c->stream), "read");
+    hip_check(hipMemcpyAsync(mat, dm.p(), n * sizeof(int), hipMemcpyDeviceToHost, c->stream), "read");
+    hip_check(hipStreamSynchronize(c->stream), "voxel");
+  });
+}
+
+int mdx_select_interaction(mdx_ctx* c, int n, const float* sc, const float* si, const float* st, const float* u,
+                           int* type) {
+  return guard([&] {
+    need(c && sc && si && st && u && type && n >= 0, "bad argument");
+    if (n == 0) return;
+    set_device(c);
+    DevArray<float> a(sc, n, c->stream), b(si, n, c->stream), d(st, n, c->stream), e(u, n, c->stream);
+    DevArray<int> o(nullptr, n, c->stream);
+    hip_check(mdx_launch::select(n, a.p(), b.p(), d.p(), e.p(), o.p(), c->stream), "select");
+    hip_check(hipMemcpyAsync(type, o.p(), n * sizeof(int), hipMemcpyDeviceToHost, c->stream), "read");
+    hip_check(hipStreamSynchronize(c->stream), "select");
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,218 @@
+// Device-side building blocks of the photon-transport path (FP32 transport,
+// FP64 tallies).  Each function cites the reference code whose behaviour it
+// reproduces.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mdx_types.h"
+
+namespace mdxd {
+
+// include/MIDSX/Core/constants.h:13-23, evaluated in FP64 and rounded once.
+constexpr float kPiF = 3.14159265358979323846f;
+constexpr float kMc2 = 510998.95f;
+constexpr float kInvMc2 = (float)(1.0 / 510998.95);
+// ALPHA = mc^2 / (sqrt(2) h c 1e8) in 1/Angstrom
+constexpr double kAlphaD = 0.51099895000e6 / (1.41421356237309504880 * 4.135667662e-15 * 299792458e2 * 1e8);
+constexpr float kAlpha = (float)kAlphaD;
+constexpr float kAlphaSqrt2 = (float)(kAlphaD * 1.41421356237309504880);
+constexpr float kEps = 1e-9f;  // constants.h:14
+
+// ------------------------------------------------------------------ RNG
+// Philox-4x32-10 (Salmon et al., SC'11), counter-based: one stream per
+// history, keyed by (seed, job) with the history index in the counter.
+struct Philox {
+  uint32_t k0, k1;
+  uint32_t c0, c1, c2, c3;
+  uint32_t buf[4];
+  int left;
+
+  __device__ void init(uint64_t seed, uint32_t job, uint64_t history) {
+    k0 = (uint32_t)seed;
+    k1 = (uint32_t)(seed >> 32) ^ (job * 0x9E3779B9u);
+    c0 = 0;
+    c1 = (uint32_t)history;
+    c2 = (uint32_t)(history >> 32);
+    c3 = 0x4D445831u;
+    left = 0;
+  }
+  __device__ void refill() {
+    uint32_t x0 = c0, x1 = c1, x2 = c2, x3 = c3, a = k0, b = k1;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+      uint32_t h0 = __umulhi(0xD2511F53u, x0), l0 = 0xD2511F53u * x0;
+      uint32_t h1 = __umulhi(0xCD9E8D57u, x2), l1 = 0xCD9E8D57u * x2;
+      x0 = h1 ^ x1 ^ a;
+      x1 = l1;
+      x2 = h0 ^ x3 ^ b;
+      x3 = l0;
+      a += 0x9E3779B9u;
+      b += 0xBB67AE85u;
+    }
+    buf[0] = x0;
+    buf[1] = x1;
+    buf[2] = x2;
+    buf[3] = x3;
+    left = 4;
+    ++c0;
+  }
+  // Uniform in (0, 1] with 24 random bits (never 0, so -log(u) is finite).
+  __device__ float uniform() {
+    if (left == 0) refill();
+    uint32_t x = buf[4 - left];
+    --left;
+    return (float)((x >> 8) + 1u) * (1.0f / 16777216.0f);
+  }
+};
+
+// ------------------------------------------------------------------ curves
+// ln(X / xk) for X, xk > 0, with the rounding error of the quotient folded
+// back in (the quotient's exact residual comes from one fma).
+__device__ __forceinline__ float log_ratio(float X, float xk) {
+  float r = X / xk;
+  float e = __builtin_fmaf(-r, xk, X);
+  return logf(r) + e / (r * xk);
+}
+
+__device__ __forceinline__ int curve_bracket(const mdx_tables_dev* T, const mdx_curve& c, float X) {
+  const float* xs = T->xs + c.off;
+  int b = (int)((logf(X) - c.lnx0) * c.inv_dlnx);
+  b = b < 0 ? 0 : (b >= c.nh ? c.nh - 1 : b);
+  int k = T->hash[c.hoff + b];
+  while (k > 0 && xs[k] > X) --k;
+  while (k + 1 < c.n && xs[k + 1] <= X) ++k;
+  return k;
+}
+
+// Evaluates a packed curve (see mdx_types.h).  Linear curves clamp outside
+// their node range (interpolators.cpp:48-54); spline curves extrapolate with
+// their end spans (Eigen Spline::Span).
+__device__ __forceinline__ float curve_eval(const mdx_tables_dev* T, const mdx_curve& c, float X) {
+  const float* xs = T->xs + c.off;
+  const mdx_node* nd = T->nodes + c.off;
+  int k;
+  float t;
+  if (c.kind == 0) {
+    if (X <= xs[0]) return nd[0].a_hi + nd[0].a_lo;
+    if (X >= xs[c.n - 1]) return nd[c.n - 1].a_hi + nd[c.n - 1].a_lo;
+    k = curve_bracket(T, c, X);
+    t = log_ratio(X, xs[k]);
+  } else {
+    k = curve_bracket(T, c, X);
+    k = k > c.n - 2 ? c.n - 2 : k;
+    t = X >= xs[k] ? log_ratio(X, xs[k]) : -log_ratio(xs[k], X);
+  }
+  const mdx_node r = nd[k];
+  float a = t * (r.c1 + t * (r.c2 + t * r.c3));
+  float em1 = expm1f(a);
+  return __builtin_fmaf(r.a_hi, em1, __builtin_fmaf(r.a_lo, em1, r.a_lo)) + r.a_hi;
+}
+
+// ------------------------------------------------------------------ geometry
+// computational_domain.cpp:16-21 (inclusive bounds)
+__device__ __forceinline__ bool in_domain(const mdx_domain_dev* D, float x, float y, float z) {
+  return x >= 0.0f && x <= D->dim_space[0] && y >= 0.0f && y <= D->dim_space[1] && z >= 0.0f &&
+         z <= D->dim_space[2];
+}
+
+// computational_domain.cpp:23-31, voxel_grid.cpp:55-59,119-128: first grid
+// containing the point wins; index = truncation of the IEEE quotient
+// (position - origin) / spacing.  A point exactly on a grid's far face maps to
+// the last voxel (the reference's at() throws there).  Returns the grid, or -1
+// for the background voxel.
+__device__ __forceinline__ int voxel_lookup(const mdx_domain_dev* D, float x, float y, float z, uint64_t& lin) {
+  for (int g = 0; g < D->n_grids; ++g) {
+    const mdx_grid_dev& G = D->grids[g];
+    float qx = x - G.origin[0], qy = y - G.origin[1], qz = z - G.origin[2];
+    if (qx >= 0.0f && qy >= 0.0f && qz >= 0.0f && qx <= G.dim[0] && qy <= G.dim[1] && qz <= G.dim[2]) {
+      int i = (int)(qx / G.spacing[0]);
+      int j = (int)(qy / G.spacing[1]);
+      int k = (int)(qz / G.spacing[2]);
+      i = i >= G.n[0] ? G.n[0] - 1 : i;
+      j = j >= G.n[1] ? G.n[1] - 1 : j;
+      k = k >= G.n[2] ? G.n[2] - 1 : k;
+      lin = (uint64_t)i + (uint64_t)G.n[0] * ((uint64_t)j + (uint64_t)G.n[1] * (uint64_t)k);
+      return g;
+    }
+  }
+  return -1;
+}
+
+// physics_engine.cpp:82-109: u < p_coh -> Rayleigh (1); u < p_coh + p_inc ->
+// Compton (2); otherwise photoelectric (0).  p = sigma / sigma_tot with IEEE
+// division, compared in the reference's order.
+__device__ __forceinline__ int select_type(float s_coh, float s_inc, float s_tot, float u) {
+  float pc = s_coh / s_tot;
+  float pi = s_inc / s_tot;
+  if (u < pc) return 1;
+  if (u < pc + pi) return 2;
+  return 0;
+}
+
+// particle.cpp:7-45: d' = R(phi about d) R(theta about n) d with
+// n = normalize(e_min x d), e_min the axis of the smallest |d_i|.  The
+// quaternion product reduces to d' = cos(t) d + sin(t) (cos(p) n x d + sin(p) n).
+__device__ __forceinline__ void rotate(float& dx, float& dy, float& dz, float mu, float phi) {
+  float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
+  float nx, ny, nz;
+  if (ax <= ay && ax <= az) {  // e_x x d
+    nx = 0.0f; ny = -dz; nz = dy;
+  } else if (ay <= az) {       // e_y x d
+    nx = dz; ny = 0.0f; nz = -dx;
+  } else {                     // e_z x d
+    nx = -dy; ny = dx; nz = 0.0f;
+  }
+  float inv = rsqrtf(nx * nx + ny * ny + nz * nz);
+  nx *= inv; ny *= inv; nz *= inv;
+  float cx = ny * dz - nz * dy, cy = nz * dx - nx * dz, cz = nx * dy - ny * dx;  // n x d
+  float st = sqrtf(fmaxf(0.0f, 1.0f - mu * mu));
+  float sp, cp;
+  sincosf(phi, &sp, &cp);
+  float ox = mu * dx + st * (cp * cx + sp * nx);
+  float oy = mu * dy + st * (cp * cy + sp * ny);
+  float oz = mu * dz + st * (cp * cz + sp * nz);
+  float r = rsqrtf(ox * ox + oy * oy + oz * oz);
+  dx = ox * r; dy = oy * r; dz = oz * r;
+}
+
+// ------------------------------------------------------------------ RITA
+// probability_dist.cpp:90-119 on one energy row, in w = 1 + x.
+__device__ __forceinline__ float rita_row(const mdx_tables_dev* T, int row0, int row1, float y) {
+  const mdx_f4* R = T->rita_rows;
+  int m = row1 - row0;
+  // lower_bound over the row's CDF values, minus one
+  int lo = 0, hi = m;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (R[row0 + mid].y < y) lo = mid + 1; else hi = mid;
+  }
+  int i = lo - 1;
+  if (i == m - 1) return 2.0f;
+  if (i < 0) return 0.0f;
+  mdx_f4 a = R[row0 + i], b = R[row0 + i + 1];
+  float nu = y - a.y;
+  float d = b.y - a.y;
+  return a.x + (1.0f + a.z + a.w) * d * nu / (d * d + a.z * d * nu + a.w * nu * nu) * (b.x - a.x);
+}
+
+// probability_dist.cpp:73-88: energy bracket by lower_bound - 1 on the
+// incoherent-CS grid, x at both energies from the same y, then log-log
+// interpolation of (x + 1) in E.
+__device__ __forceinline__ float rita_sample(const mdx_tables_dev* T, const mdx_material_dev& M, float E, float y) {
+  const float* Eg = T->rita_E + M.rita_eoff;
+  int lo = 0, hi = M.rita_ne;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (Eg[mid] < E) lo = mid + 1; else hi = mid;
+  }
+  int il = lo - 1;
+  il = il < 0 ? 0 : (il > M.rita_ne - 2 ? M.rita_ne - 2 : il);
+  const int* ro = T->rita_row_off + M.rita_roff;
+  float wl = rita_row(T, ro[il], ro[il + 1], y);
+  float wu = rita_row(T, ro[il + 1], ro[il + 2], y);
+  float f = log_ratio(E, Eg[il]) * T->rita_inv_dlnE[M.rita_eoff + il];
+  float w = (wl > 0.0f && wu > 0.0f) ? wl * expf(f * log_ratio(wu, wl)) : wl;
+  return w - 1.0f;
+}
+
+}  // namespace mdxd

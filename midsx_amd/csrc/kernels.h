@@ -1,0 +1,26 @@
+// Launchers of the HIP kernels (transport.hip), called by the C-ABI layer.
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include "mdx_types.h"
+
+namespace mdx_launch {
+
+hipError_t transport_plain(const mdx_tables_dev* T, const mdx_domain_dev* D, const mdx_source_dev* S,
+                           const mdx_tallies_dev* TL, uint64_t first, uint64_t n, uint64_t seed, uint32_t job,
+                           unsigned long long* counters, int grid_blocks, hipStream_t st);
+hipError_t eval_curve(const mdx_tables_dev* T, int slot, int which, int n, const float* x, float* out,
+                      hipStream_t st);
+hipError_t rita(const mdx_tables_dev* T, int slot, int n, const float* E, const float* y, float* mu,
+                hipStream_t st);
+hipError_t voxel(const mdx_tables_dev* T, const mdx_domain_dev* D, int n, const float* pos, int* grid,
+                 long long* lin, int* mat, hipStream_t st);
+hipError_t select(int n, const float* sc, const float* si, const float* stt, const float* u, int* type,
+                  hipStream_t st);
+hipError_t kahn(const mdx_tables_dev* T, int slot, int n, const float* E, uint64_t seed, float* mu, float* Ep,
+                hipStream_t st);
+hipError_t zero(double* p, size_t n, hipStream_t st);
+hipError_t zero_u32(uint32_t* p, size_t n, hipStream_t st);
+hipError_t zero_u64(unsigned long long* p, size_t n, hipStream_t st);
+
+}  // namespace mdx_launch

@@ -1,0 +1,155 @@
+// Data layout shared by the host table builder, the C-ABI layer and the HIP
+// kernels.  Everything the kernels read lives in one of these plain structs;
+// host code fills them and copies them to device memory once per upload.
+#pragma once
+#include <stdint.h>
+
+#define MDX_MAX_MATERIALS 32
+#define MDX_MAX_GRIDS 8
+#define MDX_MAX_TALLIES 16
+
+// Tally quantity / bucket layout (mirrors include/MIDSX/Core/volume_quantity.h:13-28,
+// surface_quantity.h:13-28 and the four buckets of *_quantity.cpp measure()).
+#define MDX_NBUCKET 4  // 0 primary, 1 single incoherent, 2 single coherent, 3 multiple
+#define MDX_NFIELD 3   // n, sum, sum of squares (per recorded event)
+#define MDX_VQ_ENERGY_DEPOSITION 0
+#define MDX_VQ_INCIDENT_ENERGY 1
+#define MDX_VQ_NUMBER_OF_PHOTONS 2
+#define MDX_VQ_NUMBER_OF_INTERACTIONS 3
+#define MDX_SQ_INCIDENT_ENERGY 0
+#define MDX_SQ_ENTRANCE_COSINE 1
+#define MDX_SQ_NUMBER_OF_PHOTONS 2
+#define MDX_NQ 4
+
+// One interpolated curve f(X) on FP32 nodes.  For X in [x_k, x_{k+1}):
+//   f(X) = (a_hi + a_lo) * exp(c1 t + c2 t^2 + c3 t^3),  t = ln(X / x_k)
+// kind 0: log-log linear, clamped to the end values outside [x_0, x_{n-1}]
+//         (interpolators.cpp:37-76);
+// kind 1: log-log cubic B-spline span polynomials, extrapolated with the end
+//         spans (interpolators.cpp:3-31).
+struct alignas(32) mdx_node {
+  float a_hi, a_lo, c1, c2;
+  float c3, x, pad0, pad1;
+};
+
+struct mdx_curve {
+  int32_t n;       // nodes
+  int32_t kind;    // 0 linear, 1 spline
+  int32_t off;     // first node in the node arrays
+  int32_t hoff;    // first entry in the hash array
+  int32_t nh;      // hash bins
+  float lnx0;      // ln of the hash range start
+  float inv_dlnx;  // hash bins per unit ln X
+  float pad;
+};
+
+struct mdx_material_dev {
+  mdx_curve total, coh, incoh, photo, sf;
+  int32_t rita_ne;     // energies in the RITA grid
+  int32_t rita_eoff;   // offset into rita_E / rita_inv_dlnE
+  int32_t rita_roff;   // offset into rita_row_off (rita_ne + 1 entries)
+  int32_t material_id;
+};
+
+struct alignas(16) mdx_f4 {
+  float x, y, z, w;
+};
+
+// RITA row (PENELOPE inverse-CDF interpolation) stored as w = 1 + x so the
+// backward-scatter end keeps its precision: (w_i, y_i, a_i, b_i).
+struct mdx_tables_dev {
+  const float* xs;          // node abscissae (all curves)
+  const mdx_node* nodes;    // node records (all curves)
+  const int32_t* hash;      // bracket hash (all curves)
+  const float* rita_E;      // RITA energies (all materials)
+  const float* rita_inv_dlnE;   // 1 / ln(E_{i+1}/E_i)
+  const int32_t* rita_row_off;  // per energy: first row; one extra entry per material
+  const mdx_f4* rita_rows;
+  mdx_curve majorant;
+  int32_t n_mat;
+  int32_t pad[3];
+  mdx_material_dev mat[MDX_MAX_MATERIALS];
+  uint8_t id_to_slot[256];  // material ID -> slot, 255 = unknown
+};
+
+struct mdx_grid_dev {
+  float origin[3];
+  float spacing[3];
+  float dim[3];
+  int32_t n[3];
+  const uint8_t* ids;  // x-fastest: i + j*nx + k*nx*ny (voxel_grid.cpp:119-121)
+  double* dose;        // per voxel: sum, sum^2 interleaved (nullptr = off)
+  uint32_t* dose_n;    // per voxel deposit count (nullptr = off)
+};
+
+struct mdx_domain_dev {
+  float dim_space[3];
+  int32_t bg_slot;
+  int32_t n_grids;
+  int32_t pad[3];
+  mdx_grid_dev grids[MDX_MAX_GRIDS];
+  double* bg_dose;  // background voxel: sum, sum^2, n
+};
+
+#define MDX_DIR_BEAM 0
+#define MDX_DIR_ISOTROPIC 1
+#define MDX_DIR_RECT_ISOTROPIC 2
+#define MDX_DIR_DISC_ISOTROPIC 3
+#define MDX_SPEC_MONO 0
+#define MDX_SPEC_POLY 1
+
+struct mdx_source_dev {
+  int32_t dir_kind;
+  int32_t spec_kind;
+  int32_t n_bins;
+  int32_t pad;
+  float pos[3];
+  float a[3];  // beam: pass-through point; rect: corner; disc: centre
+  float b[3];  // rect: edge1; disc: u
+  float c[3];  // rect: edge2; disc: v
+  float radius;
+  float energy;
+  const float* cdf;       // poly: cumulative probabilities
+  const float* energies;  // poly: bin energies (eV)
+};
+
+#define MDX_TALLY_AACUBOID 0
+#define MDX_TALLY_DISC 1
+#define MDX_TALLY_RECT 2
+#define MDX_TALLY_IMAGE 3
+
+struct mdx_tally_dev {
+  int32_t kind;
+  int32_t hist_bins;  // surface tallies: IncidentEnergy histogram bins (0 = off)
+  int32_t nx, ny;     // image pixels
+  float p0[3];        // cuboid min / disc centre / rect and image corner
+  float p1[3];        // cuboid max / rect and image edge1
+  float p2[3];        // rect and image edge2
+  float nrm[3];       // surface normal
+  float radius;
+  float e1e1, e2e2;
+  float hist_lo, hist_inv_w;
+  double* acc;        // [MDX_NQ][MDX_NBUCKET][MDX_NFIELD]
+  double* hist;       // [MDX_NBUCKET][hist_bins] photon counts
+  double* image;      // [ny*nx][3]: sum of incident energy, sum^2, count
+};
+
+struct mdx_tallies_dev {
+  int32_t n;
+  int32_t pad[3];
+  mdx_tally_dev t[MDX_MAX_TALLIES];
+};
+
+// Event counters, one int64 each; indexes into the counter array.
+#define MDX_C_HIST 0
+#define MDX_C_STEP 1
+#define MDX_C_REAL 2
+#define MDX_C_RAYLEIGH 3
+#define MDX_C_COMPTON 4
+#define MDX_C_PHOTO 5
+#define MDX_C_KAHN_TRIAL 6
+#define MDX_C_KAHN_S 7
+#define MDX_C_DEPOSIT 8
+#define MDX_C_SURFHIT 9
+#define MDX_C_ESCAPE 10
+#define MDX_NCOUNTER 12

@@ -1,0 +1,467 @@
+// Photon-transport kernels for gfx950.
+//
+// k_transport_plain: one history per thread (grid-stride), the Woodcock
+// delta-tracking loop of src/MIDSX/Core/physics_engine.cpp:17-79 in FP32 with
+// each step's tallies processed as the step is taken (the reference records
+// steps and replays them after the history, physics_engine.cpp:192-210; the
+// per-step measurements are independent, so the order does not matter).
+// Scalar tallies are pre-reduced in LDS and flushed once per block; voxel
+// dose, images and spectra go straight to FP64 global atomics.
+#include <hip/hip_runtime.h>
+
+#include "device.h"
+#include "kernels.h"
+
+using namespace mdxd;
+
+namespace {
+
+constexpr int kBlock = 256;
+// Bound on Woodcock steps per history: far above any physical history (the
+// configurations here take tens of steps), it guarantees every lane exits.
+constexpr uint32_t kMaxSteps = 1u << 20;
+constexpr int kAccPerTally = MDX_NQ * MDX_NBUCKET * MDX_NFIELD;
+
+__device__ __forceinline__ int bucket_of(int nc, int ni) {
+  // *_quantity.cpp measure(): total > 1 -> multiple; coherent == 1; incoherent == 1; else primary
+  return (nc + ni) > 1 ? 3 : (nc == 1 ? 2 : (ni == 1 ? 1 : 0));
+}
+
+__device__ __forceinline__ void lds_event(double* acc, int t, int q, int b, double v) {
+  double* p = acc + ((t * MDX_NQ + q) * MDX_NBUCKET + b) * MDX_NFIELD;
+  atomicAdd(p, 1.0);
+  atomicAdd(p + 1, v);
+  atomicAdd(p + 2, v * v);
+}
+
+struct Step {
+  float px, py, pz;  // initial position
+  float dx, dy, dz;  // initial direction
+  float s;           // free path
+  float E;           // initial energy
+  int nc0, ni0;      // scatter counts of the initial photon
+  int ncf, nif;      // scatter counts of the final photon (0 unless interacted)
+  float dep;
+  bool interact;
+};
+
+// surface_tally.cpp:16-34 (+ DiscSurfaceTally :63-69, RectangularSurfaceTally
+// :100-112) and volume_tally.cpp:5-91 for one recorded step.
+__device__ void process_tallies(const mdx_tallies_dev* TL, double* acc, const Step& r, uint32_t& surf_hits) {
+  for (int t = 0; t < TL->n; ++t) {
+    const mdx_tally_dev& T = TL->t[t];
+    if (T.kind == MDX_TALLY_AACUBOID) {
+      float pos[3] = {r.px, r.py, r.pz}, dir[3] = {r.dx, r.dy, r.dz};
+      float enter = -INFINITY, exit = INFINITY;
+      for (int i = 0; i < 3; ++i) {
+        if (fabsf(dir[i]) < kEps) continue;
+        float a = (T.p0[i] - pos[i]) / dir[i];
+        float b = (T.p1[i] - pos[i]) / dir[i];
+        if (a > b) { float tmp = a; a = b; b = tmp; }
+        enter = fmaxf(enter, a);
+        exit = fminf(exit, b);
+      }
+      if (enter > exit || exit < 0.0f || enter > r.s) continue;  // MISSES
+      bool counted = false;
+      float dep = r.dep;
+      if (enter <= 0.0f) {
+        counted = true;                  // STARTS_INSIDE_*
+        if (!(exit >= r.s)) dep = 0.0f;  // ..._EXITS
+      } else if (exit < r.s) {
+        dep = 0.0f;                      // PASSES_THROUGH
+      }                                  // else LANDS_INSIDE
+      int b = bucket_of(r.ncf, r.nif);   // volume quantities use the final photon
+      if (dep != 0.0f) lds_event(acc, t, MDX_VQ_ENERGY_DEPOSITION, b, (double)dep);
+      if (!counted) {
+        if (r.E != 0.0f) lds_event(acc, t, MDX_VQ_INCIDENT_ENERGY, b, (double)r.E);
+        lds_event(acc, t, MDX_VQ_NUMBER_OF_PHOTONS, b, 1.0);
+      }
+      if (r.interact) lds_event(acc, t, MDX_VQ_NUMBER_OF_INTERACTIONS, b, 1.0);
+    } else {
+      float cosv = r.dx * T.nrm[0] + r.dy * T.nrm[1] + r.dz * T.nrm[2];
+      if (fabsf(cosv) < kEps || cosv < 0.0f) continue;
+      float tt = ((T.p0[0] - r.px) * T.nrm[0] + (T.p0[1] - r.py) * T.nrm[1] + (T.p0[2] - r.pz) * T.nrm[2]) / cosv;
+      if (tt > r.s || tt < 0.0f) continue;
+      float ix = r.px + tt * r.dx - T.p0[0];
+      float iy = r.py + tt * r.dy - T.p0[1];
+      float iz = r.pz + tt * r.dz - T.p0[2];
+      float a = 0.0f, bb = 0.0f;
+      if (T.kind == MDX_TALLY_DISC) {
+        if (sqrtf(ix * ix + iy * iy + iz * iz) > T.radius) continue;
+      } else {
+        a = T.p1[0] * ix + T.p1[1] * iy + T.p1[2] * iz;
+        bb = T.p2[0] * ix + T.p2[1] * iy + T.p2[2] * iz;
+        if (!(a >= 0.0f && a <= T.e1e1 && bb >= 0.0f && bb <= T.e2e2)) continue;
+      }
+      ++surf_hits;
+      int b = bucket_of(r.nc0, r.ni0);  // surface quantities use the initial photon
+      if (T.kind == MDX_TALLY_IMAGE) {
+        int px = (int)(a / T.e1e1 * (float)T.nx);
+        int py = (int)(bb / T.e2e2 * (float)T.ny);
+        px = px >= T.nx ? T.nx - 1 : (px < 0 ? 0 : px);
+        py = py >= T.ny ? T.ny - 1 : (py < 0 ? 0 : py);
+        double* im = T.image + 3 * ((size_t)py * T.nx + px);
+        atomicAdd(im, (double)r.E);
+        atomicAdd(im + 1, (double)r.E * (double)r.E);
+        atomicAdd(im + 2, 1.0);
+      }
+      if (r.E != 0.0f) lds_event(acc, t, MDX_SQ_INCIDENT_ENERGY, b, (double)r.E);
+      lds_event(acc, t, MDX_SQ_ENTRANCE_COSINE, b, (double)cosv);
+      lds_event(acc, t, MDX_SQ_NUMBER_OF_PHOTONS, b, 1.0);
+      if (T.hist_bins > 0) {
+        float fb = (r.E - T.hist_lo) * T.hist_inv_w;
+        if (fb >= 0.0f && fb < (float)T.hist_bins) atomicAdd(T.hist + b * T.hist_bins + (int)fb, 1.0);
+      }
+    }
+  }
+}
+
+// source.cpp:150-155 and the directionality / spectrum samplers it calls.
+// RNG order: position (point: none), direction, energy.
+__device__ void generate(const mdx_source_dev* S, Philox& rng, float& px, float& py, float& pz, float& dx,
+                         float& dy, float& dz, float& E) {
+  px = S->pos[0];
+  py = S->pos[1];
+  pz = S->pos[2];
+  float vx, vy, vz;
+  switch (S->dir_kind) {
+    case MDX_DIR_ISOTROPIC: {  // source.cpp:64-70 (theta ~ U[0, pi])
+      float th = kPiF * rng.uniform();
+      float ph = 2.0f * kPiF * rng.uniform();
+      float st, ct, sp, cp;
+      sincosf(th, &st, &ct);
+      sincosf(ph, &sp, &cp);
+      vx = st * cp; vy = st * sp; vz = ct;
+      break;
+    }
+    case MDX_DIR_RECT_ISOTROPIC: {  // source.cpp:85-90
+      float a1 = rng.uniform(), a2 = rng.uniform();
+      vx = S->a[0] + a1 * S->b[0] + a2 * S->c[0] - px;
+      vy = S->a[1] + a1 * S->b[1] + a2 * S->c[1] - py;
+      vz = S->a[2] + a1 * S->b[2] + a2 * S->c[2] - pz;
+      break;
+    }
+    case MDX_DIR_DISC_ISOTROPIC: {  // source.cpp:108-115 (r ~ U[0, R])
+      float th = 2.0f * kPiF * rng.uniform();
+      float rr = S->radius * rng.uniform();
+      float st, ct;
+      sincosf(th, &st, &ct);
+      vx = S->a[0] + rr * (S->b[0] * ct + S->c[0] * st) - px;
+      vy = S->a[1] + rr * (S->b[1] * ct + S->c[1] * st) - py;
+      vz = S->a[2] + rr * (S->b[2] * ct + S->c[2] * st) - pz;
+      break;
+    }
+    default:  // beam, source.cpp:74-77
+      vx = S->a[0] - px; vy = S->a[1] - py; vz = S->a[2] - pz;
+  }
+  float inv = rsqrtf(vx * vx + vy * vy + vz * vz);
+  dx = vx * inv; dy = vy * inv; dz = vz * inv;
+  if (S->spec_kind == MDX_SPEC_POLY) {  // probability_dist.cpp:18-28
+    float u = rng.uniform();
+    int lo = 0, hi = S->n_bins;
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (S->cdf[mid] < u) lo = mid + 1; else hi = mid;
+    }
+    lo = lo >= S->n_bins ? S->n_bins - 1 : lo;
+    E = S->energies[lo];
+  } else {
+    E = S->energy;
+  }
+}
+
+// photon_interactions.cpp:62-141 (Kahn's method + S(x) rejection).  Returns
+// the scattered energy and sets mu.  S(x_max) depends only on E, so it is
+// evaluated once per event instead of once per trial.  The loop is bounded:
+// after kMaxKahnTrials rejections (probability far below 1e-100 for any
+// acceptance rate above 0.1) the last candidate is taken, so a corrupt table
+// cannot hang the kernel.
+constexpr uint32_t kMaxKahnTrials = 4096;
+__device__ float compton(const mdx_tables_dev* T, const mdx_material_dev& M, Philox& rng, float E, float& mu,
+                         uint32_t& trials, uint32_t& s_tests) {
+  const float k = E * kInvMc2;
+  const float r1p = (1.0f + 2.0f * k) / (9.0f + 2.0f * k);
+  const float s_max = curve_eval(T, M.sf, kAlphaSqrt2 * k);
+  float chi = 1.0f;
+  for (uint32_t it = 0;; ++it) {
+    ++trials;
+    float r1 = rng.uniform(), r2 = rng.uniform(), r3 = rng.uniform();
+    if (r1 <= r1p) {
+      chi = 1.0f + 2.0f * k * r2;
+      if (!(r3 <= 4.0f * (1.0f / chi - 1.0f / (chi * chi))) && it < kMaxKahnTrials) continue;
+    } else {
+      chi = (1.0f + 2.0f * k) / (1.0f + 2.0f * k * r2);
+      float q = 1.0f / k - chi / k + 1.0f;
+      if (!(2.0f * r3 <= q * q + 1.0f / chi) && it < kMaxKahnTrials) continue;
+    }
+    ++s_tests;
+    float one_minus_mu = (chi - 1.0f) / k;  // 1 - mu, mu = 1 + 1/k - 1/k'
+    float x = kAlpha * k * sqrtf(one_minus_mu);
+    float sx = curve_eval(T, M.sf, x);
+    if (rng.uniform() <= sx / s_max || it >= kMaxKahnTrials) {
+      mu = 1.0f - one_minus_mu;
+      return E / chi;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_transport_plain(const mdx_tables_dev* __restrict__ T, const mdx_domain_dev* __restrict__ D,
+                  const mdx_source_dev* __restrict__ S, const mdx_tallies_dev* __restrict__ TL, uint64_t first,
+                  uint64_t n, uint64_t seed, uint32_t job, unsigned long long* __restrict__ counters) {
+  __shared__ double acc[MDX_MAX_TALLIES * kAccPerTally];
+  __shared__ unsigned long long cnt[MDX_NCOUNTER];
+  const int ntl = TL->n;
+  for (int i = threadIdx.x; i < ntl * kAccPerTally; i += blockDim.x) acc[i] = 0.0;
+  if (threadIdx.x < MDX_NCOUNTER) cnt[threadIdx.x] = 0;
+  __syncthreads();
+
+  uint32_t c_hist = 0, c_step = 0, c_real = 0, c_ray = 0, c_comp = 0, c_photo = 0, c_trial = 0, c_stest = 0,
+           c_dep = 0, c_surf = 0, c_esc = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; h < n; h += stride) {
+    Philox rng;
+    rng.init(seed, job, first + h);
+    float px, py, pz, dx, dy, dz, E;
+    generate(S, rng, px, py, pz, dx, dy, dz, E);
+    ++c_hist;
+    int nc = 0, ni = 0;
+    float smax = curve_eval(T, T->majorant, E);
+    int cslot = -1;
+    float stot = 0.0f;
+    for (uint32_t it = 0; it < kMaxSteps; ++it) {
+      ++c_step;
+      Step r;
+      r.px = px; r.py = py; r.pz = pz;
+      r.dx = dx; r.dy = dy; r.dz = dz;
+      r.E = E;
+      r.nc0 = nc; r.ni0 = ni;
+      r.ncf = 0; r.nif = 0;
+      r.dep = 0.0f;
+      r.interact = false;
+      // physics_engine.cpp:38-44: s = -ln(u) / sigma_max, move
+      float s = -logf(rng.uniform()) / smax;
+      r.s = s;
+      px = px + s * dx;
+      py = py + s * dy;
+      pz = pz + s * dz;
+      if (!in_domain(D, px, py, pz)) {  // :47-52
+        ++c_esc;
+        process_tallies(TL, acc, r, c_surf);
+        break;
+      }
+      uint64_t lin = 0;
+      int g = voxel_lookup(D, px, py, pz, lin);
+      int slot = g < 0 ? D->bg_slot : (int)T->id_to_slot[D->grids[g].ids[lin]];
+      const mdx_material_dev& M = T->mat[slot];
+      if (slot != cslot) {
+        stot = curve_eval(T, M.total, E);
+        cslot = slot;
+      }
+      bool alive = true;
+      // :180-183 delta scatter if u < 1 - sigma_tot / sigma_max
+      if (!(rng.uniform() < 1.0f - stot / smax)) {
+        ++c_real;
+        r.interact = true;
+        float sc = curve_eval(T, M.coh, E);
+        float si = curve_eval(T, M.incoh, E);
+        int type = select_type(sc, si, stot, rng.uniform());
+        float dep = 0.0f;
+        if (type == 1) {  // photon_interactions.cpp:50-60
+          ++c_ray;
+          ++nc;
+          float mu = rita_sample(T, M, E, rng.uniform());
+          rotate(dx, dy, dz, mu, 2.0f * kPiF * rng.uniform());
+        } else if (type == 2) {  // :62-88
+          ++c_comp;
+          ++ni;
+          float mu;
+          float Ep = compton(T, M, rng, E, mu, c_trial, c_stest);
+          rotate(dx, dy, dz, mu, 2.0f * kPiF * rng.uniform());
+          dep = E - Ep;
+          E = Ep;
+          smax = curve_eval(T, T->majorant, E);
+          cslot = -1;
+        } else {  // :43-48
+          ++c_photo;
+          dep = E;
+          alive = false;
+        }
+        r.ncf = nc;
+        r.nif = ni;
+        r.dep = dep;
+        if (dep != 0.0f) {  // physics_engine.cpp:70-75, 205-208
+          ++c_dep;
+          double* dd;
+          uint32_t* dn;
+          if (g >= 0) {
+            dd = D->grids[g].dose ? D->grids[g].dose + 2 * lin : nullptr;
+            dn = D->grids[g].dose_n ? D->grids[g].dose_n + lin : nullptr;
+          } else {
+            dd = D->bg_dose;
+            dn = nullptr;
+          }
+          if (dd) {
+            atomicAdd(dd, (double)dep);
+            atomicAdd(dd + 1, (double)dep * (double)dep);
+            if (g < 0) atomicAdd(dd + 2, 1.0);
+          }
+          if (dn) atomicAdd(dn, 1u);
+        }
+      }
+      process_tallies(TL, acc, r, c_surf);
+      if (!alive) break;
+    }
+  }
+  atomicAdd(&cnt[MDX_C_HIST], (unsigned long long)c_hist);
+  atomicAdd(&cnt[MDX_C_STEP], (unsigned long long)c_step);
+  atomicAdd(&cnt[MDX_C_REAL], (unsigned long long)c_real);
+  atomicAdd(&cnt[MDX_C_RAYLEIGH], (unsigned long long)c_ray);
+  atomicAdd(&cnt[MDX_C_COMPTON], (unsigned long long)c_comp);
+  atomicAdd(&cnt[MDX_C_PHOTO], (unsigned long long)c_photo);
+  atomicAdd(&cnt[MDX_C_KAHN_TRIAL], (unsigned long long)c_trial);
+  atomicAdd(&cnt[MDX_C_KAHN_S], (unsigned long long)c_stest);
+  atomicAdd(&cnt[MDX_C_DEPOSIT], (unsigned long long)c_dep);
+  atomicAdd(&cnt[MDX_C_SURFHIT], (unsigned long long)c_surf);
+  atomicAdd(&cnt[MDX_C_ESCAPE], (unsigned long long)c_esc);
+  __syncthreads();
+  for (int i = threadIdx.x; i < ntl * kAccPerTally; i += blockDim.x) {
+    double v = acc[i];
+    if (v != 0.0) {
+      int t = i / kAccPerTally;
+      atomicAdd(TL->t[t].acc + (i - t * kAccPerTally), v);
+    }
+  }
+  if (threadIdx.x < MDX_NCOUNTER && cnt[threadIdx.x]) atomicAdd(counters + threadIdx.x, cnt[threadIdx.x]);
+}
+
+// ------------------------------------------------------------------ inspection kernels
+__global__ void k_eval_curve(const mdx_tables_dev* T, int slot, int which, int n, const float* x, float* out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const mdx_curve* c;
+  if (slot < 0) c = &T->majorant;
+  else {
+    const mdx_material_dev& M = T->mat[slot];
+    c = which == 0 ? &M.total : which == 1 ? &M.coh : which == 2 ? &M.incoh : which == 3 ? &M.photo : &M.sf;
+  }
+  out[i] = curve_eval(T, *c, x[i]);
+}
+
+__global__ void k_rita(const mdx_tables_dev* T, int slot, int n, const float* E, const float* y, float* mu) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  mu[i] = rita_sample(T, T->mat[slot], E[i], y[i]);
+}
+
+__global__ void k_voxel(const mdx_tables_dev* T, const mdx_domain_dev* D, int n, const float* pos, int* grid,
+                        long long* lin, int* mat) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t l = 0;
+  int g = voxel_lookup(D, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], l);
+  grid[i] = g;
+  lin[i] = g < 0 ? -1 : (long long)l;
+  int slot = g < 0 ? D->bg_slot : (int)T->id_to_slot[D->grids[g].ids[l]];
+  mat[i] = T->mat[slot].material_id;
+}
+
+__global__ void k_select(int n, const float* sc, const float* si, const float* st, const float* u, int* type) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  type[i] = select_type(sc[i], si[i], st[i], u[i]);
+}
+
+__global__ void k_kahn(const mdx_tables_dev* T, int slot, int n, const float* E, uint64_t seed, float* mu,
+                       float* Ep) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Philox rng;
+  rng.init(seed, 0x4B41u, (uint64_t)i);
+  uint32_t a = 0, b = 0;
+  float m;
+  Ep[i] = compton(T, T->mat[slot], rng, E[i], m, a, b);
+  mu[i] = m;
+}
+
+__global__ void k_zero(double* p, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+__global__ void k_zero_u32(uint32_t* p, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+__global__ void k_zero_u64(unsigned long long* p, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+inline unsigned blocks_for(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+namespace mdx_launch {
+
+hipError_t transport_plain(const mdx_tables_dev* T, const mdx_domain_dev* D, const mdx_source_dev* S,
+                           const mdx_tallies_dev* TL, uint64_t first, uint64_t n, uint64_t seed, uint32_t job,
+                           unsigned long long* counters, int grid_blocks, hipStream_t st) {
+  unsigned blocks = (unsigned)grid_blocks;
+  uint64_t need = (n + kBlock - 1) / kBlock;
+  if (need < blocks) blocks = (unsigned)(need ? need : 1);
+  hipLaunchKernelGGL(k_transport_plain, dim3(blocks), dim3(kBlock), 0, st, T, D, S, TL, first, n, seed, job,
+                     counters);
+  return hipGetLastError();
+}
+
+hipError_t eval_curve(const mdx_tables_dev* T, int slot, int which, int n, const float* x, float* out,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(k_eval_curve, dim3(blocks_for(n, 256)), dim3(256), 0, st, T, slot, which, n, x, out);
+  return hipGetLastError();
+}
+
+hipError_t rita(const mdx_tables_dev* T, int slot, int n, const float* E, const float* y, float* mu,
+                hipStream_t st) {
+  hipLaunchKernelGGL(k_rita, dim3(blocks_for(n, 256)), dim3(256), 0, st, T, slot, n, E, y, mu);
+  return hipGetLastError();
+}
+
+hipError_t voxel(const mdx_tables_dev* T, const mdx_domain_dev* D, int n, const float* pos, int* grid,
+                 long long* lin, int* mat, hipStream_t st) {
+  hipLaunchKernelGGL(k_voxel, dim3(blocks_for(n, 256)), dim3(256), 0, st, T, D, n, pos, grid, lin, mat);
+  return hipGetLastError();
+}
+
+hipError_t select(int n, const float* sc, const float* si, const float* stt, const float* u, int* type,
+                  hipStream_t st) {
+  hipLaunchKernelGGL(k_select, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, sc, si, stt, u, type);
+  return hipGetLastError();
+}
+
+hipError_t kahn(const mdx_tables_dev* T, int slot, int n, const float* E, uint64_t seed, float* mu, float* Ep,
+                hipStream_t st) {
+  hipLaunchKernelGGL(k_kahn, dim3(blocks_for(n, 256)), dim3(256), 0, st, T, slot, n, E, seed, mu, Ep);
+  return hipGetLastError();
+}
+
+hipError_t zero(double* p, size_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+hipError_t zero_u32(uint32_t* p, size_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+hipError_t zero_u64(unsigned long long* p, size_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_u64, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+}  // namespace mdx_launch
