@@ -38,7 +38,7 @@ def lib():
         for name in ["orc_destroy", "orc_set_majorant", "orc_add_material", "orc_set_domain", "orc_add_grid",
                      "orc_set_source", "orc_set_tallies", "orc_enable_voxel_dose", "orc_run", "orc_get_tally",
                      "orc_get_voxel_dose", "orc_get_background_dose", "orc_get_counters", "orc_reset",
-                     "orc_rita_sample", "orc_interp"]:
+                     "orc_rita_sample", "orc_interp", "orc_kahn_sample", "orc_rotate"]:
             getattr(L, name).argtypes = None
         _lib = L
     return _lib
@@ -207,6 +207,14 @@ class Simulation:
     def rita_sample(self, mid, E, y):
         return self.L.orc_rita_sample(self.h, ctypes.c_int(mid), ctypes.c_double(E), ctypes.c_double(y))
 
+    def kahn_sample(self, mid, E, n, seed=1):
+        mu = np.zeros(n)
+        ep = np.zeros(n)
+        self.L.orc_kahn_sample(self.h, ctypes.c_int(mid), ctypes.c_double(E), ctypes.c_int(n), ctypes.c_uint64(seed),
+                               mu.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                               ep.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return mu, ep
+
     def interp(self, mid, which, x):
         return self.L.orc_interp(self.h, ctypes.c_int(mid), ctypes.c_int(which), ctypes.c_double(x))
 
@@ -215,3 +223,10 @@ class Simulation:
             self.L.orc_destroy(self.h)
         except Exception:
             pass
+
+
+def rotate(d, theta, phi):
+    """The reference's quaternion rotation (particle.cpp:7-45), FP64."""
+    out = (ctypes.c_double * 3)()
+    lib().orc_rotate((ctypes.c_double * 3)(*d), ctypes.c_double(theta), ctypes.c_double(phi), out)
+    return np.array(out[:])

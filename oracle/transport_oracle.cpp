@@ -707,6 +707,25 @@ double orc_interp(void* h, int id, int which, double x) {
     default: return m.sf(x);
   }
 }
+// IncoherentScattering::interact alone: n events at energy E in material id;
+// writes mu and the scattered energy of each (photon_interactions.cpp:62-141).
+void orc_kahn_sample(void* h, int id, double E, int n, uint64_t seed, double* mu, double* Ep) {
+  Oracle& O = *static_cast<Oracle*>(h);
+  Engine eng(O);
+  Rng rng;
+  rng.gen.seed((uint32_t)seed);
+  uint64_t c[12] = {0};
+  const Material& m = O.mats.at(id);
+  for (int i = 0; i < n; ++i) {
+    Photon ph;
+    ph.dir = V3(0, 0, 1);
+    ph.E = E;
+    eng.compton(m, ph, rng, c);
+    mu[i] = ph.dir.z;
+    Ep[i] = ph.E;
+  }
+}
+
 void orc_rotate(const double* d, double theta, double phi, double* out) {
   V3 r = rotate(V3(d[0], d[1], d[2]), theta, phi);
   out[0] = r.x;

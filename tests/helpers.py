@@ -1,0 +1,49 @@
+"""Statistics helpers shared by the parity tests."""
+import math
+
+import numpy as np
+from scipy import stats
+
+
+def batch_mean_sigma(samples):
+    """Mean of per-batch values and the standard error of that mean."""
+    a = np.asarray(samples, dtype=np.float64)
+    k = a.shape[0]
+    return a.mean(axis=0), a.std(axis=0, ddof=1) / math.sqrt(k)
+
+
+def zscore(m1, s1, m2, s2):
+    s = np.sqrt(np.asarray(s1) ** 2 + np.asarray(s2) ** 2)
+    return np.where(s > 0, (np.asarray(m1) - np.asarray(m2)) / np.where(s > 0, s, 1), 0.0)
+
+
+def chi2_pvalue(z):
+    z = np.asarray(z, dtype=np.float64).ravel()
+    return float(stats.chi2.sf(np.sum(z * z), df=len(z))), float(np.sum(z * z)), len(z)
+
+
+def count_chi2(c1, n1, c2, n2, min_count=10):
+    """Two-sample chi^2 of histograms of counts (Poisson), scaled to the
+    number of histories of each sample; returns (p, chi2, dof)."""
+    c1 = np.asarray(c1, dtype=np.float64).ravel()
+    c2 = np.asarray(c2, dtype=np.float64).ravel()
+    keep = (c1 + c2) >= min_count
+    c1, c2 = c1[keep], c2[keep]
+    r = n1 / n2
+    z = (c1 - r * c2) / np.sqrt(c1 + r * r * c2)
+    return chi2_pvalue(z)
+
+
+def welch_normal_scores(m1, s1, k1, m2, s2, k2):
+    """Per-bin Welch t statistics of two batch means (k1, k2 batches), mapped
+    to standard-normal scores through their two-sided p-values so that a
+    chi^2 over bins is not inflated by the noise of the estimated sigmas."""
+    m1, s1, m2, s2 = (np.asarray(a, dtype=np.float64) for a in (m1, s1, m2, s2))
+    v1, v2 = s1 ** 2, s2 ** 2
+    se = np.sqrt(v1 + v2)
+    t = np.where(se > 0, (m1 - m2) / np.where(se > 0, se, 1), 0.0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        df = (v1 + v2) ** 2 / (v1 ** 2 / (k1 - 1) + v2 ** 2 / (k2 - 1))
+    df = np.where(np.isfinite(df) & (df > 0), df, k1 + k2 - 2)
+    p = 2 * stats.t.sf(np.abs(t), df)
+    return np.sign(t) * stats.norm.isf(np.clip(p, 1e-300, 1) / 2)

@@ -1,0 +1,171 @@
+"""Device building blocks against FP64 and FP32 references, through the C-ABI.
+
+* voxel indexing, material lookup and interaction-type selection: bit-exact
+  against the reference formulas evaluated in IEEE FP32 (numpy) on the same
+  inputs;
+* cross-section / scattering-function interpolation: within 2 FP32 ulp of the
+  FP64 evaluation of the same interpolant;
+* RITA inverse-CDF sampling: against the FP64 sampler on the same uniforms;
+* Kahn + S(x) Compton sampling: chi^2 against Klein-Nishina x S(x).
+"""
+import numpy as np
+import pytest
+from scipy import integrate, stats
+
+import midsx_amd as M
+import oracle
+from oracle import tables as T
+
+pytestmark = pytest.mark.gpu
+MATS = (3, 48, 169, 168)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = M.Context(0)
+    c.build_tables(list(MATS))
+    yield c
+    c.close()
+
+
+def ulp_err(dev, ref):
+    ref = np.asarray(ref, dtype=np.float64)
+    spacing = np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64)
+    return np.abs(np.asarray(dev, dtype=np.float64) - ref) / spacing
+
+
+def energies(ctx, mid, which, lo=1.5e3, hi=1.5e5):
+    E = list(np.geomspace(lo, hi, 20011))
+    x, _ = ctx.table(mid, which) if mid >= 0 else ctx.table(-1, M.XS_TOTAL)
+    x = x[(x > lo) & (x < hi)]
+    E += list(x) + list(np.nextafter(x.astype(np.float32), np.float32(0))) + list(0.5 * (x[1:] + x[:-1]))
+    return np.unique(np.asarray(E, dtype=np.float32))
+
+
+@pytest.mark.parametrize("mid", MATS)
+@pytest.mark.parametrize("which", [M.XS_TOTAL, M.XS_COHERENT, M.XS_INCOHERENT, M.XS_PHOTOELECTRIC])
+def test_cross_sections_within_2_ulp(ctx, mid, which):
+    E = energies(ctx, mid, which)
+    dev = ctx.interpolate(mid, which, E)
+    ref = ctx.interpolate_host(mid, which, E.astype(np.float64))
+    u = ulp_err(dev, ref)
+    assert u.max() <= 2.0, (u.max(), E[np.argmax(u)])
+
+
+def test_majorant_within_2_ulp(ctx):
+    E = energies(ctx, -1, M.XS_TOTAL)
+    u = ulp_err(ctx.interpolate(-1, M.XS_TOTAL, E), ctx.interpolate_host(-1, M.XS_TOTAL, E.astype(np.float64)))
+    assert u.max() <= 2.0
+
+
+@pytest.mark.parametrize("mid", MATS)
+def test_scattering_function_within_2_ulp(ctx, mid):
+    x = np.unique(np.geomspace(1e-4, 12.0, 20011).astype(np.float32))
+    u = ulp_err(ctx.interpolate(mid, M.SF_INCOHERENT, x),
+                ctx.interpolate_host(mid, M.SF_INCOHERENT, x.astype(np.float64)))
+    assert u.max() <= 2.0
+
+
+@pytest.mark.parametrize("mid", (48, 169))
+def test_rita_device_matches_fp64(ctx, mid):
+    rng = np.random.default_rng(4)
+    E = rng.uniform(15e3, 150e3, 200000).astype(np.float32)
+    y = rng.random(200000).astype(np.float32)
+    dev = ctx.sample_rayleigh(mid, E, y).astype(np.float64)
+    ref = ctx.sample_rayleigh_host(mid, E.astype(np.float64), y.astype(np.float64))
+    err = np.abs(dev - ref)
+    # error in units of the FP32 spacing of (1 + mu), the variable RITA interpolates
+    u = err / np.spacing(np.float32(1.0) + np.abs(ref).astype(np.float32)).astype(np.float64)
+    assert np.percentile(u, 99.9) <= 8 and err.max() < 1e-4, (np.percentile(u, 99.9), err.max())
+    assert np.all(dev >= -1.0) and np.all(dev <= 1.0)
+
+
+def test_voxel_lookup_bit_exact(ctx):
+    """computational_domain.cpp:23-31, voxel_grid.cpp:55-59,119-128 on FP32
+    inputs: truncation of the IEEE quotient, first grid wins, background
+    outside; the far face maps to the last voxel."""
+    ctx.set_domain((10.0, 10.0, 10.0), 3)
+    rng = np.random.default_rng(5)
+    g0 = rng.choice(np.array([48, 169, 168], np.uint8), size=(13, 7, 11))
+    g1 = rng.choice(np.array([48, 169], np.uint8), size=(5, 9, 4))
+    grids = [(g0, (0.3, 0.7, 0.11), (1.0, 2.0, 3.0)), (g1, (0.5, 0.25, 0.9), (2.0, 1.0, 2.5))]
+    for ids, sp, org in grids:
+        ctx.add_grid(ids, sp, org)
+    ctx.build_tables(list(MATS))
+    pts = [rng.uniform(0, 10, size=(50000, 3))]
+    for ids, sp, org in grids:  # faces, corners and neighbours of every face
+        dims = np.array(ids.shape, dtype=np.float64) * np.array(sp)
+        corner = np.array(org) + dims
+        for base in (np.array(org), corner):
+            q = np.tile(base, (2000, 1)) + rng.uniform(-1, 1, size=(2000, 3)) * 0.5
+            ax = rng.integers(0, 3, 2000)
+            q[np.arange(2000), ax] = base[ax]
+            pts.append(q)
+            pts.append(np.nextafter(q.astype(np.float32), np.float32(np.inf)))
+            pts.append(np.nextafter(q.astype(np.float32), np.float32(-np.inf)))
+        k = np.arange(ids.shape[0] + 1)[:, None] * np.array([sp[0], 0, 0]) + np.array(org)
+        pts.append(k + np.array([0, sp[1] * 0.5, sp[2] * 0.5]))
+    P = np.concatenate(pts).astype(np.float32)
+    g, lin, mat = ctx.voxel_lookup(P)
+    # FP32 restatement of the same formula
+    eg = np.full(len(P), -1)
+    elin = np.full(len(P), -1, dtype=np.int64)
+    emat = np.full(len(P), 3)
+    for gi in reversed(range(len(grids))):
+        ids, sp, org = grids[gi]
+        n = np.array(ids.shape)
+        dimf = (n * np.array(sp)).astype(np.float32)
+        q = P - np.array(org, dtype=np.float32)
+        inside = np.all((q >= 0) & (q <= dimf), axis=1)
+        idx = np.trunc(q / np.array(sp, dtype=np.float32)).astype(np.int64)
+        idx = np.minimum(idx, n - 1)
+        l = idx[:, 0] + n[0] * (idx[:, 1] + n[1] * idx[:, 2])
+        eg[inside] = gi
+        elin[inside] = l[inside]
+        flat = np.ascontiguousarray(ids.transpose(2, 1, 0)).ravel()
+        emat[inside] = flat[np.clip(l[inside], 0, flat.size - 1)]
+    assert np.array_equal(g, eg)
+    assert np.array_equal(lin, elin)
+    assert np.array_equal(mat, emat)
+
+
+def test_interaction_type_bit_exact(ctx):
+    """physics_engine.cpp:82-109 on FP32 inputs, including u equal to the
+    thresholds."""
+    rng = np.random.default_rng(6)
+    n = 400000
+    st = rng.uniform(0.01, 10.0, n).astype(np.float32)
+    sc = (st * rng.uniform(0, 0.3, n)).astype(np.float32)
+    si = (st * rng.uniform(0, 0.7, n)).astype(np.float32)
+    u = rng.random(n).astype(np.float32)
+    pc = sc / st
+    pi = si / st
+    u[: n // 4] = pc[: n // 4]
+    u[n // 4: n // 2] = (pc + pi)[n // 4: n // 2]
+    u[n // 2: n // 2 + 1000] = np.nextafter(pc[n // 2: n // 2 + 1000], np.float32(0))
+    expect = np.where(u < pc, 1, np.where(u < pc + pi, 2, 0))
+    got = ctx.select_interaction(sc, si, st, u)
+    assert np.array_equal(got, expect)
+
+
+def kn_s(m, E, mu):
+    k = E / T.ELECTRON_REST_MASS
+    r = 1.0 / (1 + k * (1 - mu))
+    x = T.ALPHA * k * np.sqrt(np.maximum(1 - mu, 0))
+    return r * r * (r + 1 / r - (1 - mu * mu)) * m.sf_i(np.maximum(x, 1e-12))
+
+
+@pytest.mark.parametrize("E", [20e3, 60e3, 120e3])
+def test_compton_sampling_chi2(ctx, E):
+    m = oracle.material(48)
+    n = 400000
+    mu, ep = ctx.sample_compton(48, np.full(n, E, np.float32), seed=int(E))
+    k = E / T.ELECTRON_REST_MASS
+    assert np.allclose(ep, E / (1 + k * (1 - mu.astype(np.float64))), rtol=2e-6)
+    edges = np.linspace(-1, 1, 51)
+    counts, _ = np.histogram(mu, edges)
+    f = lambda q: kn_s(m, E, np.atleast_1d(q))[0]
+    norm = integrate.quad(f, -1, 1, limit=400)[0]
+    expect = np.array([integrate.quad(f, edges[i], edges[i + 1])[0] for i in range(50)]) / norm * n
+    chi2 = np.sum((counts - expect) ** 2 / expect)
+    assert stats.chi2.sf(chi2, 49) > 0.01, chi2
