@@ -48,6 +48,8 @@ int main(int argc, char** argv) {
   std::printf("total_photons %d\n", photons.getTotalValues().getCount());
   std::printf("primary_energy_sum %.6f\n", energies.getPrimaryValues().getSum());
   std::printf("slab_energy_deposit %.6f %.6f\n", edep.getTotalValues().getSum(), edep.getTotalValues().getSumSTD());
-  std::printf("grid_energy_deposit %.6f\n", comp_domain.getVoxelGridN(0).getTotalEnergyDeposited());
+  double grid_dep = 0;
+  for (int g = 0; g < comp_domain.getNumVoxelGrids(); ++g) grid_dep += comp_domain.getVoxelGridN(g).getTotalEnergyDeposited();
+  std::printf("grid_energy_deposit %.6f\n", grid_dep);
   return 0;
 }

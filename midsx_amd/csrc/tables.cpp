@@ -528,7 +528,12 @@ mdx_curve pack_linear(Packed& p, const Table& t, double hash_lo, double hash_hi,
   const size_t n = t.x.size();
   for (size_t k = 0; k < n; ++k) {
     mdx_node r{};
+    // Abscissae are rounded up, so that for every FP32 input X the test
+    // X >= xs[k] is the same as X >= x_k in FP64 and the device brackets X
+    // exactly as the FP64 upper_bound does (interpolators.cpp:56-59); this
+    // matters on the near-vertical ramps across absorption edges.
     float xk = (float)t.x[k];
+    if ((double)xk < t.x[k]) xk = std::nextafter(xk, INFINITY);
     double slope = 0, a = t.y[k];
     if (k + 1 < n) {
       slope = std::log(t.y[k + 1] / t.y[k]) / std::log(t.x[k + 1] / t.x[k]);

@@ -47,3 +47,26 @@ def welch_normal_scores(m1, s1, k1, m2, s2, k2):
     df = np.where(np.isfinite(df) & (df > 0), df, k1 + k2 - 2)
     p = 2 * stats.t.sf(np.abs(t), df)
     return np.sign(t) * stats.norm.isf(np.clip(p, 1e-300, 1) / 2)
+
+
+def hotelling(a, b, rel_eig=1e-9):
+    """Two-sample Hotelling T^2 test of equal mean vectors for per-batch
+    samples a (k1 x d) and b (k2 x d) with a pooled covariance; correlated
+    and redundant components are handled by whitening on the non-null
+    eigenvectors.  Returns (p, T2, rank)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    k1, k2 = len(a), len(b)
+    scale = np.maximum(np.abs(np.concatenate([a, b])).mean(axis=0), 1e-300)
+    a, b = a / scale, b / scale
+    diff = a.mean(axis=0) - b.mean(axis=0)
+    S = ((k1 - 1) * np.cov(a, rowvar=False) + (k2 - 1) * np.cov(b, rowvar=False)) / (k1 + k2 - 2)
+    S = np.atleast_2d(S) * (1.0 / k1 + 1.0 / k2)
+    w, V = np.linalg.eigh(S)
+    keep = w > rel_eig * w.max()
+    p_dim = int(keep.sum())
+    proj = V[:, keep].T @ diff
+    t2 = float(np.sum(proj * proj / w[keep]))
+    dof2 = k1 + k2 - p_dim - 1
+    f = dof2 / (p_dim * (k1 + k2 - 2)) * t2
+    return float(stats.f.sf(f, p_dim, dof2)), t2, p_dim

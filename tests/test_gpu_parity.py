@@ -11,7 +11,7 @@ import pytest
 import midsx_amd as M
 import oracle
 from midsx_amd import scenes
-from helpers import batch_mean_sigma, chi2_pvalue, count_chi2, welch_normal_scores
+from helpers import batch_mean_sigma, chi2_pvalue, count_chi2, hotelling, welch_normal_scores
 
 pytestmark = pytest.mark.gpu
 
@@ -29,7 +29,7 @@ def orc_batches(sim, n, k, seed, extract):
     out = []
     for b in range(k):
         sim.reset()
-        sim.run(n, seed=seed + 7919 * b, threads=0)
+        sim.run(n, seed=seed + 7919 * b, threads=8)
         out.append(extract(sim, sim.counters(), n))
     return out
 
@@ -58,18 +58,22 @@ def slab():
 
 
 def test_slab_tallies_match_oracle(slab):
+    """Volume- and surface-tally quantities per bucket, the depth-dose
+    profile and the event counts: every bin within 3.5 combined standard
+    errors (Welch) and the whole vector by a two-sample Hotelling T^2 test on
+    equal-size batches (the bins are correlated; several are proportional)."""
     ctx, sim = slab
-    k = 16
-    g = gpu_batches(ctx, 200000, k, 11, slab_extract)
-    o = orc_batches(sim, 60000, k, 101, slab_extract)
+    k, n = 48, 50000
+    g = np.array(gpu_batches(ctx, n, k, 11, slab_extract))
+    o = np.array(orc_batches(sim, n, k, 101, slab_extract))
     gm, gs = batch_mean_sigma(g)
     om, os_ = batch_mean_sigma(o)
     keep = (gs > 0) & (os_ > 0)
     z = welch_normal_scores(gm, gs, k, om, os_, k)[keep]
-    assert keep.sum() > 30
-    assert np.all(np.abs(z) < 4.5), (np.max(np.abs(z)), np.nonzero(keep)[0][np.argmax(np.abs(z))])
-    p, chi2, dof = chi2_pvalue(z)
-    assert p > 0.01, (p, chi2, dof)
+    assert keep.sum() > 25
+    assert np.all(np.abs(z) < 3.5), (np.max(np.abs(z)), np.nonzero(keep)[0][np.argmax(np.abs(z))])
+    p, t2, rank = hotelling(g[:, keep], o[:, keep])
+    assert p > 0.01, (p, t2, rank)
 
 
 def test_slab_spectrum_chi2(slab):
@@ -83,7 +87,7 @@ def test_slab_spectrum_chi2(slab):
     sim.run(no, seed=23, threads=0)
     oh = sim.tally(1)["hist"]
     p, chi2, dof = count_chi2(gh, ng, oh, no)
-    assert dof >= 20 and p > 0.01, (p, chi2, dof)
+    assert dof >= 5 and p > 0.01, (p, chi2, dof)
     # primary photons all sit in the 60 keV bin
     assert gh[0, 60] == gh[0].sum() and oh[0, 60] == oh[0].sum()
 
@@ -154,30 +158,31 @@ def test_rng_streams_are_per_history(slab):
 
 def radiography_extract(src, st, n):
     img = src.tally(0)["image"]
-    coarse = img[:, :, 0].reshape(8, 4, 8, 4).sum(axis=(1, 3)).ravel()  # 8 x 8 blocks of energy fluence
-    dose = src.voxel_dose(0)[0].reshape(8, 8, 8, 8, 8, 8).sum(axis=(1, 3, 5)).ravel()  # 8^3 blocks
+    coarse = img[:, :, 0].reshape(4, 8, 4, 8).sum(axis=(1, 3)).ravel()  # 4 x 4 blocks of energy fluence
+    dose = src.voxel_dose(0)[0].reshape(4, 16, 4, 16, 4, 16).sum(axis=(1, 3, 5)).ravel()  # 4^3 blocks
     return np.concatenate([coarse / n, dose / n, [src.background_dose()[0] / n],
                            [st["steps"] / n, st["real"] / n, st["compton"] / n, st["photo"] / n]])
 
 
 def test_radiography_small_matches_oracle():
     """Polyenergetic Kramers source, cone beam, heterogeneous phantom
-    (water / bone / adipose), image and per-voxel dose tallies."""
+    (water / bone / adipose), image and per-voxel dose tallies: each coarse
+    bin within 3.5 combined standard errors and Hotelling T^2 on the whole."""
     kw = dict(shape=(64, 64, 64), voxel_cm=0.5, seed=3, image=(32, 32))
     ctx = M.Context(0)
     ctx.build_tables(list(scenes.C2_MATERIALS))
     scenes.setup_radiography(ctx, **kw)
     sim = oracle.Simulation(scenes.C2_MATERIALS)
     scenes.setup_radiography(sim, **kw)
-    k = 16
-    g = gpu_batches(ctx, 100000, k, 51, radiography_extract)
-    o = orc_batches(sim, 20000, k, 151, radiography_extract)
+    k, n = 160, 10000
+    g = np.array(gpu_batches(ctx, n, k, 51, radiography_extract))
+    o = np.array(orc_batches(sim, n, k, 151, radiography_extract))
     gm, gs = batch_mean_sigma(g)
     om, os_ = batch_mean_sigma(o)
     keep = (gs > 0) & (os_ > 0) & (np.abs(om) > 3 * os_)
     z = welch_normal_scores(gm, gs, k, om, os_, k)[keep]
-    assert keep.sum() > 100
-    assert np.all(np.abs(z) < 5.0), np.max(np.abs(z))
-    p, chi2, dof = chi2_pvalue(z)
-    assert p > 0.01, (p, chi2, dof)
+    assert keep.sum() > 40
+    assert np.all(np.abs(z) < 3.5), np.max(np.abs(z))
+    p, t2, rank = hotelling(g[:, keep], o[:, keep])
+    assert p > 0.01, (p, t2, rank)
     ctx.close()
