@@ -87,6 +87,7 @@ extern "C" {
 
 /* Transport kernels. */
 #define MDX_KERNEL_PLAIN 0 /* one history per thread, one simulation per launch */
+#define MDX_KERNEL_QUEUE 1 /* persistent lanes refilled from a work counter */
 
 typedef struct mdx_ctx mdx_ctx;
 
@@ -177,6 +178,24 @@ int mdx_reset_tallies(mdx_ctx* ctx);
  * calls until mdx_reset_tallies (physics_engine.cpp:111-122). */
 int mdx_run(mdx_ctx* ctx, uint64_t n_histories, uint64_t seed, uint64_t first_history, int kernel,
             mdx_stats* stats);
+
+/* Jobs.  A context holds several scenes ("jobs"), each with its own domain,
+ * grids, source and tallies, all sharing the context's tables; the per-scene
+ * calls above act on the selected job (job 0 exists from mdx_create).  Used
+ * to batch many simulations into one launch (no reference counterpart: the
+ * reference runs one simulation per runSimulation call). */
+int mdx_job_create(mdx_ctx* ctx, int* job);   /* creates and selects */
+int mdx_job_select(mdx_ctx* ctx, int job);
+int mdx_job_release(mdx_ctx* ctx, int job);   /* frees its device memory */
+/* Device memory held by one job (grids, dose, tallies) and by the tables. */
+int mdx_job_device_bytes(mdx_ctx* ctx, int job, uint64_t* bytes);
+int mdx_tables_device_bytes(mdx_ctx* ctx, uint64_t* bytes);
+/* One launch of the queue kernel over several jobs' history slices
+ * [first[i], first[i] + n[i]) with RNG streams keyed by (seed[i], job,
+ * history): a job's results do not depend on which other jobs share the
+ * launch.  Blocking. */
+int mdx_run_jobs(mdx_ctx* ctx, int n_jobs, const int* jobs, const uint64_t* first, const uint64_t* n,
+                 const uint64_t* seed, mdx_stats* stats);
 
 /* PhysicsEngine::getVolumeQuantityContainers / getSurfaceQuantityContainers
  * (physics_engine.cpp:124-158).  acc: MDX_TALLY_ACC_SIZE doubles; hist:

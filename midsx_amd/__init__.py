@@ -22,7 +22,7 @@ DIR_BEAM, DIR_ISOTROPIC, DIR_RECT_ISOTROPIC, DIR_DISC_ISOTROPIC = 0, 1, 2, 3
 SPEC_MONO, SPEC_POLY = 0, 1
 TALLY_AACUBOID, TALLY_DISC, TALLY_RECT, TALLY_IMAGE = 0, 1, 2, 3
 XS_TOTAL, XS_COHERENT, XS_INCOHERENT, XS_PHOTOELECTRIC, SF_INCOHERENT, FF_COHERENT = 0, 1, 2, 3, 4, 5
-KERNEL_PLAIN = 0
+KERNEL_PLAIN, KERNEL_QUEUE = 0, 1
 NQ, NBUCKET, NFIELD = 4, 4, 3
 STAT_NAMES = ["histories", "steps", "real", "rayleigh", "compton", "photo", "kahn_trials", "kahn_s_tests",
               "deposits", "surface_hits", "escapes", "reserved"]
@@ -37,7 +37,8 @@ EXPORTS = [
     "mdx_enable_voxel_dose", "mdx_reset_tallies", "mdx_run", "mdx_get_tally", "mdx_get_voxel_dose",
     "mdx_get_background_dose", "mdx_interpolate", "mdx_interpolate_host", "mdx_get_table", "mdx_sample_rayleigh",
     "mdx_sample_rayleigh_host", "mdx_get_rita_row", "mdx_sample_compton", "mdx_voxel_lookup",
-    "mdx_select_interaction",
+    "mdx_select_interaction", "mdx_job_create", "mdx_job_select", "mdx_job_release", "mdx_job_device_bytes",
+    "mdx_tables_device_bytes", "mdx_run_jobs",
 ]
 
 
@@ -110,7 +111,13 @@ class Context:
         self.device = device
         if data_dir:
             self._call(self.lib.mdx_set_data_dir, self._h, data_dir.encode())
-        self.tallies = []
+        self._job = 0
+        self._job_tallies = {0: []}
+
+    @property
+    def tallies(self):
+        """Tally descriptions of the selected job."""
+        return self._job_tallies.setdefault(self._job, [])
 
     def _call(self, fn, *args):
         st = fn(*args)
@@ -290,7 +297,7 @@ class Context:
             q.hist_lo = t.get("hist_lo", 0.0)
             q.hist_width = t.get("hist_width", 0.0)
         self._call(self.lib.mdx_set_tallies, self._h, arr, ctypes.c_int(len(tallies)))
-        self.tallies = [dict(t) for t in tallies]
+        self._job_tallies[self._job] = [dict(t) for t in tallies]
 
     def enable_voxel_dose(self, on=True):
         self._call(self.lib.mdx_enable_voxel_dose, self._h, ctypes.c_int(1 if on else 0))
@@ -302,6 +309,49 @@ class Context:
         st = _Stats()
         self._call(self.lib.mdx_run, self._h, ctypes.c_uint64(n), ctypes.c_uint64(seed), ctypes.c_uint64(first),
                    ctypes.c_int(kernel), ctypes.byref(st))
+        out = {name: int(st.counters[i]) for i, name in enumerate(STAT_NAMES)}
+        out["kernel_ms"] = st.kernel_ms
+        return out
+
+    # ---------------------------------------------------------- jobs
+    def job_create(self):
+        j = ctypes.c_int()
+        self._call(self.lib.mdx_job_create, self._h, ctypes.byref(j))
+        self._job = j.value
+        self._job_tallies[j.value] = []
+        return j.value
+
+    def job_select(self, j):
+        self._call(self.lib.mdx_job_select, self._h, ctypes.c_int(j))
+        self._job = j
+
+    def job_release(self, j):
+        self._call(self.lib.mdx_job_release, self._h, ctypes.c_int(j))
+        self._job_tallies.pop(j, None)
+        if self._job == j:
+            self._job = 0
+
+    def job_device_bytes(self, j):
+        b = ctypes.c_uint64()
+        self._call(self.lib.mdx_job_device_bytes, self._h, ctypes.c_int(j), ctypes.byref(b))
+        return b.value
+
+    def tables_device_bytes(self):
+        b = ctypes.c_uint64()
+        self._call(self.lib.mdx_tables_device_bytes, self._h, ctypes.byref(b))
+        return b.value
+
+    def run_jobs(self, jobs, n, first=None, seeds=None):
+        """One queue-kernel launch over several jobs; n[i] histories of job
+        jobs[i] starting at history first[i] with seed seeds[i]."""
+        k = len(jobs)
+        ids = np.ascontiguousarray(jobs, dtype=np.int32)
+        nn = np.ascontiguousarray(n, dtype=np.uint64)
+        ff = np.ascontiguousarray(first if first is not None else np.zeros(k), dtype=np.uint64)
+        ss = np.ascontiguousarray(seeds if seeds is not None else np.ones(k), dtype=np.uint64)
+        st = _Stats()
+        self._call(self.lib.mdx_run_jobs, self._h, ctypes.c_int(k), _ptr(ids, ctypes.c_int),
+                   _ptr(ff, ctypes.c_uint64), _ptr(nn, ctypes.c_uint64), _ptr(ss, ctypes.c_uint64), ctypes.byref(st))
         out = {name: int(st.counters[i]) for i, name in enumerate(STAT_NAMES)}
         out["kernel_ms"] = st.kernel_ms
         return out

@@ -92,6 +92,31 @@ struct TallyHost {
 
 }  // namespace
 
+// One scene: domain and grids, source, tallies and their device copies.
+// A context holds several (one per served job); the per-scene calls of the
+// C-ABI act on the selected one.
+struct Job {
+  bool domain_set = false;
+  double dim_space[3] = {0, 0, 0};
+  int background_id = -1;
+  std::vector<std::unique_ptr<GridHost>> grids;
+  bool voxel_dose = false;
+  DBuf d_bg, d_domain;
+  bool source_set = false;
+  mdx_source_dev src{};
+  DBuf d_cdf, d_energies, d_source;
+  std::vector<std::unique_ptr<TallyHost>> tallies;
+  DBuf d_tallies;
+  DBuf d_counters;  // per-job event counters
+  size_t device_bytes() const {
+    size_t b = d_bg.bytes + d_domain.bytes + d_cdf.bytes + d_energies.bytes + d_source.bytes + d_tallies.bytes +
+               d_counters.bytes;
+    for (auto& g : grids) b += g->d_ids.bytes + g->d_dose.bytes + g->d_dose_n.bytes;
+    for (auto& t : tallies) b += t->acc.bytes + t->hist.bytes + t->image.bytes;
+    return b;
+  }
+};
+
 struct mdx_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -101,30 +126,22 @@ struct mdx_ctx {
   bool registry_loaded = false;
   std::map<int, mdx::MaterialDef> registry;
 
-  // tables
+  // tables (shared by every job)
   std::vector<std::unique_ptr<mdx::MaterialTables>> mats;
   mdx::Table majorant;
   mdx::Packed packed;
   DBuf d_xs, d_nodes, d_hash, d_rE, d_rinv, d_roff, d_rrows, d_tables;
   bool tables_ready = false;
 
-  // domain
-  bool domain_set = false;
-  double dim_space[3] = {0, 0, 0};
-  int background_id = -1;
-  std::vector<std::unique_ptr<GridHost>> grids;
-  bool voxel_dose = false;
-  DBuf d_bg, d_domain;
+  // scenes
+  std::vector<std::unique_ptr<Job>> jobs;
+  int cur = 0;
+  Job& J() { return *jobs[cur]; }
 
-  // source
-  bool source_set = false;
-  mdx_source_dev src{};
-  DBuf d_cdf, d_energies, d_source;
-
-  // tallies
-  std::vector<std::unique_ptr<TallyHost>> tallies;
-  DBuf d_tallies;
   DBuf d_counters;
+  int queue_blocks_per_cu = 8;
+  DBuf d_work;  // queue kernel: work counter
+  DBuf d_jobs, d_prefix;
 
   void ensure_registry() {
     if (registry_loaded) return;
@@ -177,21 +194,22 @@ void set_device(mdx_ctx* c) {
 
 void upload_domain(mdx_ctx* c) {
   mdx_domain_dev D{};
-  for (int k = 0; k < 3; ++k) D.dim_space[k] = (float)c->dim_space[k];
-  D.bg_slot = c->tables_ready ? c->slot_of(c->background_id) : 0;
-  D.n_grids = (int)c->grids.size();
-  for (size_t g = 0; g < c->grids.size(); ++g) {
-    GridHost& G = *c->grids[g];
+  for (int k = 0; k < 3; ++k) D.dim_space[k] = (float)c->J().dim_space[k];
+  D.bg_slot = c->tables_ready ? c->slot_of(c->J().background_id) : 0;
+  D.n_grids = (int)c->J().grids.size();
+  for (size_t g = 0; g < c->J().grids.size(); ++g) {
+    GridHost& G = *c->J().grids[g];
     mdx_grid_dev& d = D.grids[g];
     size_t nv = (size_t)G.dims[0] * G.dims[1] * G.dims[2];
     for (int k = 0; k < 3; ++k) {
       d.origin[k] = (float)G.origin[k];
       d.spacing[k] = (float)G.spacing[k];
+      d.inv_spacing[k] = 1.0f / d.spacing[k];
       d.dim[k] = (float)(G.dims[k] * G.spacing[k]);  // voxel_grid.cpp:175-177
       d.n[k] = G.dims[k];
     }
     d.ids = G.d_ids.as<uint8_t>();
-    if (c->voxel_dose) {
+    if (c->J().voxel_dose) {
       if (G.d_dose.bytes != nv * 16) {
         G.d_dose.alloc(nv * 16);
         G.d_dose_n.alloc(nv * 4);
@@ -205,21 +223,21 @@ void upload_domain(mdx_ctx* c) {
       G.d_dose_n.release();
     }
   }
-  if (!c->d_bg.p) {
-    c->d_bg.alloc(3 * sizeof(double));
-    hip_check(mdx_launch::zero(c->d_bg.as<double>(), 3, c->stream), "zero bg");
+  if (!c->J().d_bg.p) {
+    c->J().d_bg.alloc(3 * sizeof(double));
+    hip_check(mdx_launch::zero(c->J().d_bg.as<double>(), 3, c->stream), "zero bg");
   }
-  D.bg_dose = c->d_bg.as<double>();
-  c->d_domain.upload(&D, 1, c->stream);
+  D.bg_dose = c->J().d_bg.as<double>();
+  c->J().d_domain.upload(&D, 1, c->stream);
   hip_check(hipStreamSynchronize(c->stream), "upload domain");
 }
 
 void check_materials(mdx_ctx* c) {
   need(c->tables_ready, "physics tables not built (mdx_build_tables)");
-  need(c->domain_set, "domain not set (mdx_set_domain / mdx_load_scene)");
-  if (c->slot_of(c->background_id) == 255)
-    throw std::out_of_range("background material " + std::to_string(c->background_id) + " has no tables");
-  for (auto& g : c->grids) {
+  need(c->J().domain_set, "domain not set (mdx_set_domain / mdx_load_scene)");
+  if (c->slot_of(c->J().background_id) == 255)
+    throw std::out_of_range("background material " + std::to_string(c->J().background_id) + " has no tables");
+  for (auto& g : c->J().grids) {
     bool seen[256] = {false};
     for (uint8_t v : g->ids) seen[v] = true;
     for (int v = 0; v < 256; ++v)
@@ -277,14 +295,19 @@ void build_and_upload(mdx_ctx* c, const std::vector<int>& ids) {
   c->d_tables.upload(&T, 1, c->stream);
   hip_check(hipStreamSynchronize(c->stream), "upload tables");
   c->tables_ready = true;
-  if (c->domain_set) upload_domain(c);
+  const int keep = c->cur;
+  for (size_t j = 0; j < c->jobs.size(); ++j) {
+    c->cur = (int)j;
+    if (c->J().domain_set) upload_domain(c);
+  }
+  c->cur = keep;
 }
 
 void upload_tallies(mdx_ctx* c) {
   mdx_tallies_dev TL{};
-  TL.n = (int)c->tallies.size();
-  for (size_t i = 0; i < c->tallies.size(); ++i) {
-    TallyHost& H = *c->tallies[i];
+  TL.n = (int)c->J().tallies.size();
+  for (size_t i = 0; i < c->J().tallies.size(); ++i) {
+    TallyHost& H = *c->J().tallies[i];
     const mdx_tally& t = H.t;
     mdx_tally_dev& d = TL.t[i];
     d.kind = t.kind;
@@ -306,21 +329,21 @@ void upload_tallies(mdx_ctx* c) {
     d.hist = H.hist.as<double>();
     d.image = H.image.as<double>();
   }
-  c->d_tallies.upload(&TL, 1, c->stream);
+  c->J().d_tallies.upload(&TL, 1, c->stream);
   hip_check(hipStreamSynchronize(c->stream), "upload tallies");
 }
 
 void zero_tallies(mdx_ctx* c) {
-  for (auto& h : c->tallies) {
+  for (auto& h : c->J().tallies) {
     hip_check(mdx_launch::zero(h->acc.as<double>(), h->acc.bytes / 8, c->stream), "zero acc");
     hip_check(mdx_launch::zero(h->hist.as<double>(), h->hist.bytes / 8, c->stream), "zero hist");
     hip_check(mdx_launch::zero(h->image.as<double>(), h->image.bytes / 8, c->stream), "zero image");
   }
-  for (auto& g : c->grids) {
+  for (auto& g : c->J().grids) {
     hip_check(mdx_launch::zero(g->d_dose.as<double>(), g->d_dose.bytes / 8, c->stream), "zero dose");
     hip_check(mdx_launch::zero_u32(g->d_dose_n.as<uint32_t>(), g->d_dose_n.bytes / 4, c->stream), "zero n");
   }
-  if (c->d_bg.p) hip_check(mdx_launch::zero(c->d_bg.as<double>(), 3, c->stream), "zero bg");
+  if (c->J().d_bg.p) hip_check(mdx_launch::zero(c->J().d_bg.as<double>(), 3, c->stream), "zero bg");
   hip_check(hipStreamSynchronize(c->stream), "reset tallies");
 }
 
@@ -341,6 +364,60 @@ struct DevArray {
   T* p() { return b.as<T>(); }
 };
 
+
+void read_stats(mdx_ctx* c, mdx_stats* stats) {
+  if (!stats) return;
+  std::memset(stats, 0, sizeof *stats);
+  unsigned long long h[MDX_NCOUNTER];
+  hip_check(hipMemcpy(h, c->d_counters.p, sizeof h, hipMemcpyDeviceToHost), "counters");
+  for (int i = 0; i < MDX_NCOUNTER && i < MDX_STAT_NCOUNTERS; ++i) stats->counters[i] = h[i];
+  float ms = 0;
+  hip_check(hipEventElapsedTime(&ms, c->ev0, c->ev1), "hipEventElapsedTime");
+  stats->kernel_ms = ms;
+}
+
+// One launch of the queue kernel over the listed jobs' history slices.
+void run_jobs(mdx_ctx* c, int n_jobs, const int* ids, const uint64_t* first, const uint64_t* n,
+              const uint64_t* seed, mdx_stats* stats) {
+  need(c->tables_ready, "physics tables not built (mdx_build_tables)");
+  set_device(c);
+  std::vector<mdx_job_dev> jd(n_jobs);
+  std::vector<uint64_t> prefix(n_jobs + 1, 0);
+  const int keep = c->cur;
+  for (int i = 0; i < n_jobs; ++i) {
+    if (ids[i] < 0 || ids[i] >= (int)c->jobs.size() || !c->jobs[ids[i]]) throw std::out_of_range("no such job");
+    c->cur = ids[i];
+    check_materials(c);
+    need(c->J().source_set, "source not set (mdx_set_source)");
+    if (!c->J().d_tallies.p) upload_tallies(c);
+    jd[i].D = c->J().d_domain.as<mdx_domain_dev>();
+    jd[i].S = c->J().d_source.as<mdx_source_dev>();
+    jd[i].TL = c->J().d_tallies.as<mdx_tallies_dev>();
+    jd[i].first = first[i];
+    jd[i].n = n[i];
+    jd[i].seed = seed[i];
+    jd[i].job_id = (uint32_t)ids[i];
+    prefix[i + 1] = prefix[i] + n[i];
+  }
+  c->cur = keep;
+  const uint64_t total = prefix[n_jobs];
+  c->d_jobs.upload(jd.data(), jd.size(), c->stream);
+  c->d_prefix.upload(prefix.data(), prefix.size(), c->stream);
+  unsigned long long* cnt = c->d_counters.as<unsigned long long>();
+  hip_check(mdx_launch::zero_u64(cnt, MDX_NCOUNTER, c->stream), "zero counters");
+  hip_check(mdx_launch::zero_u64(c->d_work.as<unsigned long long>(), 1, c->stream), "zero work");
+  hip_check(hipEventRecord(c->ev0, c->stream), "hipEventRecord");
+  if (total > 0)
+    hip_check(mdx_launch::transport_queue(c->d_tables.as<mdx_tables_dev>(), c->d_jobs.as<mdx_job_dev>(),
+                                          c->d_prefix.as<uint64_t>(), n_jobs, total,
+                                          c->d_work.as<unsigned long long>(), cnt, c->num_cu * c->queue_blocks_per_cu,
+                                          c->stream),
+              "queue launch");
+  hip_check(hipEventRecord(c->ev1, c->stream), "hipEventRecord");
+  hip_check(hipStreamSynchronize(c->stream), "transport");
+  read_stats(c, stats);
+}
+
 }  // namespace
 
 extern "C" {
@@ -355,6 +432,7 @@ int mdx_create(int device, mdx_ctx** out) {
       auto c = std::make_unique<mdx_ctx>();
       c->device = -1;
       c->data_dir = default_data_dir();
+      c->jobs.push_back(std::make_unique<Job>());
       *out = c.release();
       return;
     }
@@ -364,6 +442,7 @@ int mdx_create(int device, mdx_ctx** out) {
     auto c = std::make_unique<mdx_ctx>();
     c->device = device;
     c->data_dir = default_data_dir();
+    c->jobs.push_back(std::make_unique<Job>());
     set_device(c.get());
     hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
     hip_check(hipEventCreate(&c->ev0), "hipEventCreate");
@@ -372,6 +451,7 @@ int mdx_create(int device, mdx_ctx** out) {
     hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
     c->num_cu = prop.multiProcessorCount;
     c->d_counters.alloc(MDX_NCOUNTER * sizeof(unsigned long long));
+    c->d_work.alloc(sizeof(unsigned long long));
     *out = c.release();
   });
 }
@@ -435,9 +515,9 @@ int mdx_build_tables(mdx_ctx* c, const int* ids, int n) {
 
 int mdx_build_tables_for_domain(mdx_ctx* c) {
   return guard([&] {
-    need(c && c->domain_set, "domain not set");
-    std::vector<int> ids{c->background_id};
-    for (auto& g : c->grids) {
+    need(c && c->J().domain_set, "domain not set");
+    std::vector<int> ids{c->J().background_id};
+    for (auto& g : c->J().grids) {
       bool seen[256] = {false};
       for (uint8_t v : g->ids) seen[v] = true;
       for (int v = 0; v < 256; ++v)
@@ -453,11 +533,11 @@ int mdx_set_domain(mdx_ctx* c, const double dim_space[3], int background_id) {
     need(background_id >= 0 && background_id < 256, "background material ID must be in [0, 255]");
     for (int k = 0; k < 3; ++k) {
       need(dim_space[k] > 0, "dim_space must be positive");
-      c->dim_space[k] = dim_space[k];
+      c->J().dim_space[k] = dim_space[k];
     }
-    c->background_id = background_id;
-    c->grids.clear();
-    c->domain_set = true;
+    c->J().background_id = background_id;
+    c->J().grids.clear();
+    c->J().domain_set = true;
     if (c->device >= 0) {
       set_device(c);
       upload_domain(c);
@@ -469,8 +549,8 @@ int mdx_add_grid(mdx_ctx* c, const uint8_t* ids, const int dims[3], const double
                  const double origin[3]) {
   return guard([&] {
     need(c && ids && dims && spacing && origin, "NULL argument");
-    need(c->domain_set, "mdx_set_domain must come first");
-    need((int)c->grids.size() < MDX_MAX_GRIDS, "too many grids");
+    need(c->J().domain_set, "mdx_set_domain must come first");
+    need((int)c->J().grids.size() < MDX_MAX_GRIDS, "too many grids");
     auto g = std::make_unique<GridHost>();
     size_t nv = 1;
     for (int k = 0; k < 3; ++k) {
@@ -485,7 +565,7 @@ int mdx_add_grid(mdx_ctx* c, const uint8_t* ids, const int dims[3], const double
       set_device(c);
       g->d_ids.upload(g->ids.data(), nv, c->stream);
     }
-    c->grids.push_back(std::move(g));
+    c->J().grids.push_back(std::move(g));
     if (c->device >= 0) upload_domain(c);
   });
 }
@@ -503,12 +583,12 @@ int mdx_load_scene(mdx_ctx* c, const char* json_path) {
     for (auto& g : sc.grids) grids.push_back(mdx::read_nifti(g.path));
     double ds[3] = {sc.dim_space[0], sc.dim_space[1], sc.dim_space[2]};
     for (int k = 0; k < 3; ++k) need(ds[k] > 0, "dim_space must be positive");
-    c->dim_space[0] = ds[0];
-    c->dim_space[1] = ds[1];
-    c->dim_space[2] = ds[2];
-    c->background_id = bg;
-    c->grids.clear();
-    c->domain_set = true;
+    c->J().dim_space[0] = ds[0];
+    c->J().dim_space[1] = ds[1];
+    c->J().dim_space[2] = ds[2];
+    c->J().background_id = bg;
+    c->J().grids.clear();
+    c->J().domain_set = true;
     for (size_t i = 0; i < grids.size(); ++i) {
       need(i < MDX_MAX_GRIDS, "too many grids");
       auto g = std::make_unique<GridHost>();
@@ -522,7 +602,7 @@ int mdx_load_scene(mdx_ctx* c, const char* json_path) {
         set_device(c);
         g->d_ids.upload(g->ids.data(), g->ids.size(), c->stream);
       }
-      c->grids.push_back(std::move(g));
+      c->J().grids.push_back(std::move(g));
     }
     if (c->device >= 0) upload_domain(c);
   });
@@ -531,18 +611,18 @@ int mdx_load_scene(mdx_ctx* c, const char* json_path) {
 int mdx_num_grids(mdx_ctx* c, int* n) {
   return guard([&] {
     need(c && n, "NULL argument");
-    *n = (int)c->grids.size();
+    *n = (int)c->J().grids.size();
   });
 }
 
 int mdx_grid_info(mdx_ctx* c, int grid, int dims[3], double spacing[3], double origin[3]) {
   return guard([&] {
     need(c, "NULL context");
-    if (grid < 0 || grid >= (int)c->grids.size()) throw std::out_of_range("grid index out of range");
+    if (grid < 0 || grid >= (int)c->J().grids.size()) throw std::out_of_range("grid index out of range");
     for (int k = 0; k < 3; ++k) {
-      if (dims) dims[k] = c->grids[grid]->dims[k];
-      if (spacing) spacing[k] = c->grids[grid]->spacing[k];
-      if (origin) origin[k] = c->grids[grid]->origin[k];
+      if (dims) dims[k] = c->J().grids[grid]->dims[k];
+      if (spacing) spacing[k] = c->J().grids[grid]->spacing[k];
+      if (origin) origin[k] = c->J().grids[grid]->origin[k];
     }
   });
 }
@@ -550,8 +630,8 @@ int mdx_grid_info(mdx_ctx* c, int grid, int dims[3], double spacing[3], double o
 int mdx_get_grid_ids(mdx_ctx* c, int grid, uint8_t* out) {
   return guard([&] {
     need(c && out, "NULL argument");
-    if (grid < 0 || grid >= (int)c->grids.size()) throw std::out_of_range("grid index out of range");
-    std::memcpy(out, c->grids[grid]->ids.data(), c->grids[grid]->ids.size());
+    if (grid < 0 || grid >= (int)c->J().grids.size()) throw std::out_of_range("grid index out of range");
+    std::memcpy(out, c->J().grids[grid]->ids.data(), c->J().grids[grid]->ids.size());
   });
 }
 
@@ -623,18 +703,18 @@ int mdx_set_source(mdx_ctx* c, const mdx_source* s) {
         cdf[i] = (float)acc;
         en[i] = (float)s->bin_energy[i];
       }
-      c->d_cdf.upload(cdf.data(), cdf.size(), c->stream);
-      c->d_energies.upload(en.data(), en.size(), c->stream);
+      c->J().d_cdf.upload(cdf.data(), cdf.size(), c->stream);
+      c->J().d_energies.upload(en.data(), en.size(), c->stream);
       d.n_bins = s->n_bins;
-      d.cdf = c->d_cdf.as<float>();
-      d.energies = c->d_energies.as<float>();
+      d.cdf = c->J().d_cdf.as<float>();
+      d.energies = c->J().d_energies.as<float>();
     } else {
       throw std::invalid_argument("unknown spectrum kind");
     }
-    c->src = d;
-    c->d_source.upload(&d, 1, c->stream);
+    c->J().src = d;
+    c->J().d_source.upload(&d, 1, c->stream);
     hip_check(hipStreamSynchronize(c->stream), "upload source");
-    c->source_set = true;
+    c->J().source_set = true;
   });
 }
 
@@ -671,7 +751,7 @@ int mdx_set_tallies(mdx_ctx* c, const mdx_tally* t, int n) {
       if (q.kind == MDX_TALLY_IMAGE) h->image.alloc((size_t)3 * q.nx * q.ny * sizeof(double));
       v.push_back(std::move(h));
     }
-    c->tallies = std::move(v);
+    c->J().tallies = std::move(v);
     upload_tallies(c);
     zero_tallies(c);
   });
@@ -680,9 +760,9 @@ int mdx_set_tallies(mdx_ctx* c, const mdx_tally* t, int n) {
 int mdx_enable_voxel_dose(mdx_ctx* c, int enable) {
   return guard([&] {
     need(c, "NULL context");
-    c->voxel_dose = enable != 0;
+    c->J().voxel_dose = enable != 0;
     set_device(c);
-    if (c->domain_set) upload_domain(c);
+    if (c->J().domain_set) upload_domain(c);
   });
 }
 
@@ -697,40 +777,99 @@ int mdx_reset_tallies(mdx_ctx* c) {
 int mdx_run(mdx_ctx* c, uint64_t n, uint64_t seed, uint64_t first, int kernel, mdx_stats* stats) {
   return guard([&] {
     need(c, "NULL context");
+    need(kernel == MDX_KERNEL_PLAIN || kernel == MDX_KERNEL_QUEUE, "unknown kernel");
+    if (kernel == MDX_KERNEL_QUEUE) {
+      int job = c->cur;
+      run_jobs(c, 1, &job, &first, &n, &seed, stats);
+      return;
+    }
     check_materials(c);
-    need(c->source_set, "source not set (mdx_set_source)");
-    need(kernel == MDX_KERNEL_PLAIN, "unknown kernel");
+    need(c->J().source_set, "source not set (mdx_set_source)");
     set_device(c);
-    if (!c->d_tallies.p) upload_tallies(c);
+    if (!c->J().d_tallies.p) upload_tallies(c);
     unsigned long long* cnt = c->d_counters.as<unsigned long long>();
     hip_check(mdx_launch::zero_u64(cnt, MDX_NCOUNTER, c->stream), "zero counters");
     hip_check(hipEventRecord(c->ev0, c->stream), "hipEventRecord");
     if (n > 0) {
-      hip_check(mdx_launch::transport_plain(c->d_tables.as<mdx_tables_dev>(), c->d_domain.as<mdx_domain_dev>(),
-                                            c->d_source.as<mdx_source_dev>(), c->d_tallies.as<mdx_tallies_dev>(),
+      hip_check(mdx_launch::transport_plain(c->d_tables.as<mdx_tables_dev>(), c->J().d_domain.as<mdx_domain_dev>(),
+                                            c->J().d_source.as<mdx_source_dev>(), c->J().d_tallies.as<mdx_tallies_dev>(),
                                             first, n, seed, 0u, cnt, c->num_cu * 8, c->stream),
                 "transport launch");
     }
     hip_check(hipEventRecord(c->ev1, c->stream), "hipEventRecord");
     hip_check(hipStreamSynchronize(c->stream), "transport");
-    if (stats) {
-      std::memset(stats, 0, sizeof *stats);
-      unsigned long long h[MDX_NCOUNTER];
-      hip_check(hipMemcpy(h, cnt, sizeof h, hipMemcpyDeviceToHost), "counters");
-      for (int i = 0; i < MDX_NCOUNTER && i < MDX_STAT_NCOUNTERS; ++i) stats->counters[i] = h[i];
-      float ms = 0;
-      hip_check(hipEventElapsedTime(&ms, c->ev0, c->ev1), "hipEventElapsedTime");
-      stats->kernel_ms = ms;
+    read_stats(c, stats);
+  });
+}
+
+int mdx_job_create(mdx_ctx* c, int* job) {
+  return guard([&] {
+    need(c && job, "NULL argument");
+    int slot = -1;
+    for (size_t i = 0; i < c->jobs.size(); ++i)
+      if (!c->jobs[i]) slot = (int)i;
+    if (slot < 0) {
+      slot = (int)c->jobs.size();
+      c->jobs.emplace_back();
     }
+    c->jobs[slot] = std::make_unique<Job>();
+    c->cur = slot;
+    *job = slot;
+  });
+}
+
+int mdx_job_select(mdx_ctx* c, int job) {
+  return guard([&] {
+    need(c, "NULL context");
+    if (job < 0 || job >= (int)c->jobs.size() || !c->jobs[job]) throw std::out_of_range("no such job");
+    c->cur = job;
+  });
+}
+
+int mdx_job_release(mdx_ctx* c, int job) {
+  return guard([&] {
+    need(c, "NULL context");
+    if (job < 0 || job >= (int)c->jobs.size() || !c->jobs[job]) throw std::out_of_range("no such job");
+    if (c->device >= 0) {
+      set_device(c);
+      hip_check(hipStreamSynchronize(c->stream), "release");
+    }
+    if (job == 0) c->jobs[0] = std::make_unique<Job>();
+    else c->jobs[job].reset();
+    if (c->cur == job) c->cur = 0;
+  });
+}
+
+int mdx_job_device_bytes(mdx_ctx* c, int job, uint64_t* bytes) {
+  return guard([&] {
+    need(c && bytes, "NULL argument");
+    if (job < 0 || job >= (int)c->jobs.size() || !c->jobs[job]) throw std::out_of_range("no such job");
+    *bytes = c->jobs[job]->device_bytes();
+  });
+}
+
+int mdx_tables_device_bytes(mdx_ctx* c, uint64_t* bytes) {
+  return guard([&] {
+    need(c && bytes, "NULL argument");
+    *bytes = c->d_xs.bytes + c->d_nodes.bytes + c->d_hash.bytes + c->d_rE.bytes + c->d_rinv.bytes + c->d_roff.bytes +
+             c->d_rrows.bytes + c->d_tables.bytes;
+  });
+}
+
+int mdx_run_jobs(mdx_ctx* c, int n_jobs, const int* jobs, const uint64_t* first, const uint64_t* n,
+                 const uint64_t* seed, mdx_stats* stats) {
+  return guard([&] {
+    need(c && n_jobs > 0 && jobs && first && n && seed, "bad argument");
+    run_jobs(c, n_jobs, jobs, first, n, seed, stats);
   });
 }
 
 int mdx_get_tally(mdx_ctx* c, int i, double* acc, double* hist, double* image) {
   return guard([&] {
     need(c, "NULL context");
-    if (i < 0 || i >= (int)c->tallies.size()) throw std::out_of_range("tally index out of range");
+    if (i < 0 || i >= (int)c->J().tallies.size()) throw std::out_of_range("tally index out of range");
     set_device(c);
-    TallyHost& h = *c->tallies[i];
+    TallyHost& h = *c->J().tallies[i];
     if (acc) hip_check(hipMemcpy(acc, h.acc.p, h.acc.bytes, hipMemcpyDeviceToHost), "read tally");
     if (hist && h.hist.p) hip_check(hipMemcpy(hist, h.hist.p, h.hist.bytes, hipMemcpyDeviceToHost), "read hist");
     if (image && h.image.p) hip_check(hipMemcpy(image, h.image.p, h.image.bytes, hipMemcpyDeviceToHost), "read image");
@@ -740,10 +879,10 @@ int mdx_get_tally(mdx_ctx* c, int i, double* acc, double* hist, double* image) {
 int mdx_get_voxel_dose(mdx_ctx* c, int grid, double* ss, uint32_t* count) {
   return guard([&] {
     need(c, "NULL context");
-    if (grid < 0 || grid >= (int)c->grids.size()) throw std::out_of_range("grid index out of range");
-    need(c->voxel_dose, "voxel dose is not enabled");
+    if (grid < 0 || grid >= (int)c->J().grids.size()) throw std::out_of_range("grid index out of range");
+    need(c->J().voxel_dose, "voxel dose is not enabled");
     set_device(c);
-    GridHost& g = *c->grids[grid];
+    GridHost& g = *c->J().grids[grid];
     if (ss) hip_check(hipMemcpy(ss, g.d_dose.p, g.d_dose.bytes, hipMemcpyDeviceToHost), "read dose");
     if (count) hip_check(hipMemcpy(count, g.d_dose_n.p, g.d_dose_n.bytes, hipMemcpyDeviceToHost), "read dose n");
   });
@@ -753,7 +892,7 @@ int mdx_get_background_dose(mdx_ctx* c, double out[3]) {
   return guard([&] {
     need(c && out, "NULL argument");
     set_device(c);
-    hip_check(hipMemcpy(out, c->d_bg.p, 3 * sizeof(double), hipMemcpyDeviceToHost), "read bg");
+    hip_check(hipMemcpy(out, c->J().d_bg.p, 3 * sizeof(double), hipMemcpyDeviceToHost), "read bg");
   });
 }
 
@@ -902,7 +1041,7 @@ int mdx_voxel_lookup(mdx_ctx* c, int n, const float* xyz, int* grid, long long* 
     DevArray<float> dp(xyz, 3 * (size_t)n, c->stream);
     DevArray<int> dg(nullptr, n, c->stream), dm(nullptr, n, c->stream);
     DevArray<long long> dl(nullptr, n, c->stream);
-    hip_check(mdx_launch::voxel(c->d_tables.as<mdx_tables_dev>(), c->d_domain.as<mdx_domain_dev>(), n, dp.p(), dg.p(),
+    hip_check(mdx_launch::voxel(c->d_tables.as<mdx_tables_dev>(), c->J().d_domain.as<mdx_domain_dev>(), n, dp.p(), dg.p(),
                                 dl.p(), dm.p(), c->stream),
               "voxel");
     hip_check(hipMemcpyAsync(grid, dg.p(), n * sizeof(int), hipMemcpyDeviceToHost, c->stream), "read");

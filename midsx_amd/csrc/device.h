@@ -115,6 +115,20 @@ __device__ __forceinline__ bool in_domain(const mdx_domain_dev* D, float x, floa
          z <= D->dim_space[2];
 }
 
+// trunc(fl(q / s)) for q >= 0, s > 0, as the reference's
+// (position / spacing).cast<int>() (voxel_grid.cpp:55-59), without a
+// division in the common case: r = q * fl(1/s) is within 3 * 2^-24 relative of
+// fl(q / s), so when r is farther than that from an integer both truncate to
+// the same value; otherwise the IEEE quotient is taken.
+__device__ __forceinline__ int trunc_quotient(float q, float s, float inv_s) {
+  float r = q * inv_s;
+  float f = floorf(r);
+  float frac = r - f;  // exact: f and r share the binade or f = 0
+  float tol = r * 3.0e-7f;
+  if (frac > tol && frac < 1.0f - tol) return (int)f;
+  return (int)(q / s);
+}
+
 // computational_domain.cpp:23-31, voxel_grid.cpp:55-59,119-128: first grid
 // containing the point wins; index = truncation of the IEEE quotient
 // (position - origin) / spacing.  A point exactly on a grid's far face maps to
@@ -125,9 +139,9 @@ __device__ __forceinline__ int voxel_lookup(const mdx_domain_dev* D, float x, fl
     const mdx_grid_dev& G = D->grids[g];
     float qx = x - G.origin[0], qy = y - G.origin[1], qz = z - G.origin[2];
     if (qx >= 0.0f && qy >= 0.0f && qz >= 0.0f && qx <= G.dim[0] && qy <= G.dim[1] && qz <= G.dim[2]) {
-      int i = (int)(qx / G.spacing[0]);
-      int j = (int)(qy / G.spacing[1]);
-      int k = (int)(qz / G.spacing[2]);
+      int i = trunc_quotient(qx, G.spacing[0], G.inv_spacing[0]);
+      int j = trunc_quotient(qy, G.spacing[1], G.inv_spacing[1]);
+      int k = trunc_quotient(qz, G.spacing[2], G.inv_spacing[2]);
       i = i >= G.n[0] ? G.n[0] - 1 : i;
       j = j >= G.n[1] ? G.n[1] - 1 : j;
       k = k >= G.n[2] ? G.n[2] - 1 : k;

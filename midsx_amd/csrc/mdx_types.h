@@ -75,6 +75,7 @@ struct mdx_tables_dev {
 struct mdx_grid_dev {
   float origin[3];
   float spacing[3];
+  float inv_spacing[3];  // fl(1 / spacing), for the division-free voxel index
   float dim[3];
   int32_t n[3];
   const uint8_t* ids;  // x-fastest: i + j*nx + k*nx*ny (voxel_grid.cpp:119-121)
@@ -138,6 +139,19 @@ struct mdx_tallies_dev {
   int32_t n;
   int32_t pad[3];
   mdx_tally_dev t[MDX_MAX_TALLIES];
+};
+
+// One job of a queue-kernel launch: its scene and its slice of histories
+// [first, first + n) with RNG streams keyed by (seed, job_id, history).
+struct mdx_job_dev {
+  const mdx_domain_dev* D;
+  const mdx_source_dev* S;
+  const mdx_tallies_dev* TL;
+  uint64_t first;
+  uint64_t n;
+  uint64_t seed;
+  uint32_t job_id;
+  uint32_t pad;
 };
 
 // Event counters, one int64 each; indexes into the counter array.

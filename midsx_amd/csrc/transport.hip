@@ -335,6 +335,273 @@ k_transport_plain(const mdx_tables_dev* __restrict__ T, const mdx_domain_dev* __
   if (threadIdx.x < MDX_NCOUNTER && cnt[threadIdx.x]) atomicAdd(counters + threadIdx.x, cnt[threadIdx.x]);
 }
 
+// ------------------------------------------------------------------ queue kernel
+// Persistent transport with per-lane refill: every lane carries one photon;
+// when its history ends it takes the next history index from a global work
+// counter (one wave-aggregated atomic per refill), so a wave never waits for
+// its slowest history and the only tail is the launch's last histories.
+// MULTI: the work space is the concatenation of several jobs' history slices
+// (prefix[] holds their offsets); each job has its own scene and tallies and
+// shares the tables.  Scalar tallies are pre-reduced in LDS for one job and
+// go to global FP64 atomics for several.
+__device__ __forceinline__ void event_global(double* acc, int q, int b, double v) {
+  double* p = acc + (q * MDX_NBUCKET + b) * MDX_NFIELD;
+  atomicAdd(p, 1.0);
+  atomicAdd(p + 1, v);
+  atomicAdd(p + 2, v * v);
+}
+
+template <bool MULTI>
+__device__ void tallies_step(const mdx_tallies_dev* TL, double* acc, const Step& r, uint32_t& surf_hits) {
+  if (!MULTI) {
+    process_tallies(TL, acc, r, surf_hits);
+    return;
+  }
+  for (int t = 0; t < TL->n; ++t) {
+    const mdx_tally_dev& T = TL->t[t];
+    if (T.kind == MDX_TALLY_AACUBOID) {
+      float pos[3] = {r.px, r.py, r.pz}, dir[3] = {r.dx, r.dy, r.dz};
+      float enter = -INFINITY, exit = INFINITY;
+      for (int i = 0; i < 3; ++i) {
+        if (fabsf(dir[i]) < kEps) continue;
+        float a = (T.p0[i] - pos[i]) / dir[i];
+        float b = (T.p1[i] - pos[i]) / dir[i];
+        if (a > b) { float tmp = a; a = b; b = tmp; }
+        enter = fmaxf(enter, a);
+        exit = fminf(exit, b);
+      }
+      if (enter > exit || exit < 0.0f || enter > r.s) continue;
+      bool counted = false;
+      float dep = r.dep;
+      if (enter <= 0.0f) {
+        counted = true;
+        if (!(exit >= r.s)) dep = 0.0f;
+      } else if (exit < r.s) {
+        dep = 0.0f;
+      }
+      int b = bucket_of(r.ncf, r.nif);
+      if (dep != 0.0f) event_global(T.acc, MDX_VQ_ENERGY_DEPOSITION, b, (double)dep);
+      if (!counted) {
+        if (r.E != 0.0f) event_global(T.acc, MDX_VQ_INCIDENT_ENERGY, b, (double)r.E);
+        event_global(T.acc, MDX_VQ_NUMBER_OF_PHOTONS, b, 1.0);
+      }
+      if (r.interact) event_global(T.acc, MDX_VQ_NUMBER_OF_INTERACTIONS, b, 1.0);
+    } else {
+      float cosv = r.dx * T.nrm[0] + r.dy * T.nrm[1] + r.dz * T.nrm[2];
+      if (fabsf(cosv) < kEps || cosv < 0.0f) continue;
+      float tt = ((T.p0[0] - r.px) * T.nrm[0] + (T.p0[1] - r.py) * T.nrm[1] + (T.p0[2] - r.pz) * T.nrm[2]) / cosv;
+      if (tt > r.s || tt < 0.0f) continue;
+      float ix = r.px + tt * r.dx - T.p0[0];
+      float iy = r.py + tt * r.dy - T.p0[1];
+      float iz = r.pz + tt * r.dz - T.p0[2];
+      float a = 0.0f, bb = 0.0f;
+      if (T.kind == MDX_TALLY_DISC) {
+        if (sqrtf(ix * ix + iy * iy + iz * iz) > T.radius) continue;
+      } else {
+        a = T.p1[0] * ix + T.p1[1] * iy + T.p1[2] * iz;
+        bb = T.p2[0] * ix + T.p2[1] * iy + T.p2[2] * iz;
+        if (!(a >= 0.0f && a <= T.e1e1 && bb >= 0.0f && bb <= T.e2e2)) continue;
+      }
+      ++surf_hits;
+      int b = bucket_of(r.nc0, r.ni0);
+      if (T.kind == MDX_TALLY_IMAGE) {
+        int px = (int)(a / T.e1e1 * (float)T.nx);
+        int py = (int)(bb / T.e2e2 * (float)T.ny);
+        px = px >= T.nx ? T.nx - 1 : (px < 0 ? 0 : px);
+        py = py >= T.ny ? T.ny - 1 : (py < 0 ? 0 : py);
+        double* im = T.image + 3 * ((size_t)py * T.nx + px);
+        atomicAdd(im, (double)r.E);
+        atomicAdd(im + 1, (double)r.E * (double)r.E);
+        atomicAdd(im + 2, 1.0);
+      }
+      if (r.E != 0.0f) event_global(T.acc, MDX_SQ_INCIDENT_ENERGY, b, (double)r.E);
+      event_global(T.acc, MDX_SQ_ENTRANCE_COSINE, b, (double)cosv);
+      event_global(T.acc, MDX_SQ_NUMBER_OF_PHOTONS, b, 1.0);
+      if (T.hist_bins > 0) {
+        float fb = (r.E - T.hist_lo) * T.hist_inv_w;
+        if (fb >= 0.0f && fb < (float)T.hist_bins) atomicAdd(T.hist + b * T.hist_bins + (int)fb, 1.0);
+      }
+    }
+  }
+}
+
+template <bool MULTI>
+__global__ void __launch_bounds__(kBlock)
+k_transport_queue(const mdx_tables_dev* __restrict__ T, const mdx_job_dev* __restrict__ jobs,
+                  const uint64_t* __restrict__ prefix, int n_jobs, uint64_t total,
+                  unsigned long long* __restrict__ work, unsigned long long* __restrict__ counters) {
+  __shared__ double acc[MULTI ? 1 : MDX_MAX_TALLIES * kAccPerTally];
+  __shared__ unsigned long long cnt[MDX_NCOUNTER];
+  if (!MULTI) {
+    const int ntl = jobs[0].TL->n;
+    for (int i = threadIdx.x; i < ntl * kAccPerTally; i += blockDim.x) acc[i] = 0.0;
+  }
+  if (threadIdx.x < MDX_NCOUNTER) cnt[threadIdx.x] = 0;
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63;
+  uint32_t c_hist = 0, c_step = 0, c_real = 0, c_ray = 0, c_comp = 0, c_photo = 0, c_trial = 0, c_stest = 0,
+           c_dep = 0, c_surf = 0, c_esc = 0;
+  bool alive = false, exhausted = false;
+  const mdx_domain_dev* D = jobs[0].D;
+  const mdx_source_dev* S = jobs[0].S;
+  const mdx_tallies_dev* TL = jobs[0].TL;
+  Philox rng;
+  float px = 0, py = 0, pz = 0, dx = 0, dy = 0, dz = 1, E = 1;
+  int nc = 0, ni = 0;
+  float smax = 1, step_scale = 1;  // step_scale = ln 2 / sigma_max
+  int cslot = -1;
+  float stot = 0, thr = 0;         // thr = 1 - sigma_tot / sigma_max
+  uint32_t nstep = 0;
+  for (;;) {
+    // refill finished lanes with the next histories
+    const bool need = !alive && !exhausted;
+    const unsigned long long mask = __ballot(need);
+    if (mask) {
+      const int leader = __ffsll((long long)mask) - 1;
+      unsigned long long base = 0;
+      if (lane == leader) base = atomicAdd(work, (unsigned long long)__popcll(mask));
+      base = __shfl(base, leader);
+      if (need) {
+        const uint64_t g = base + __popcll(mask & ((1ull << lane) - 1ull));
+        if (g < total) {
+          uint64_t local = g;
+          int j = 0;
+          if (MULTI) {
+            int lo = 0, hi = n_jobs - 1;  // last job with prefix[j] <= g
+            while (lo < hi) {
+              int mid = (lo + hi + 1) >> 1;
+              if (prefix[mid] <= g) lo = mid; else hi = mid - 1;
+            }
+            j = lo;
+            local = g - prefix[j];
+            D = jobs[j].D;
+            S = jobs[j].S;
+            TL = jobs[j].TL;
+          }
+          rng.init(jobs[j].seed, jobs[j].job_id, jobs[j].first + local);
+          generate(S, rng, px, py, pz, dx, dy, dz, E);
+          ++c_hist;
+          nc = ni = 0;
+          smax = curve_eval(T, T->majorant, E);
+          step_scale = 0.69314718055994531f / smax;
+          cslot = -1;
+          nstep = 0;
+          alive = true;
+        } else {
+          exhausted = true;
+        }
+      }
+    }
+    if (__ballot(alive) == 0ull) break;
+    if (!alive) continue;
+
+    // one Woodcock step (physics_engine.cpp:28-79)
+    ++c_step;
+    ++nstep;
+    Step r;
+    r.px = px; r.py = py; r.pz = pz;
+    r.dx = dx; r.dy = dy; r.dz = dz;
+    r.E = E;
+    r.nc0 = nc; r.ni0 = ni;
+    r.ncf = 0; r.nif = 0;
+    r.dep = 0.0f;
+    r.interact = false;
+    const float s = -__log2f(rng.uniform()) * step_scale;
+    r.s = s;
+    px = px + s * dx;
+    py = py + s * dy;
+    pz = pz + s * dz;
+    if (!in_domain(D, px, py, pz)) {
+      ++c_esc;
+      tallies_step<MULTI>(TL, acc, r, c_surf);
+      alive = false;
+      continue;
+    }
+    uint64_t lin = 0;
+    const int g = voxel_lookup(D, px, py, pz, lin);
+    const int slot = g < 0 ? D->bg_slot : (int)T->id_to_slot[D->grids[g].ids[lin]];
+    const mdx_material_dev& M = T->mat[slot];
+    if (slot != cslot) {
+      stot = curve_eval(T, M.total, E);
+      thr = 1.0f - stot / smax;
+      cslot = slot;
+    }
+    if (!(rng.uniform() < thr)) {
+      ++c_real;
+      r.interact = true;
+      const float sc = curve_eval(T, M.coh, E);
+      const float si = curve_eval(T, M.incoh, E);
+      const int type = select_type(sc, si, stot, rng.uniform());
+      float dep = 0.0f;
+      if (type == 1) {
+        ++c_ray;
+        ++nc;
+        const float mu = rita_sample(T, M, E, rng.uniform());
+        rotate(dx, dy, dz, mu, 2.0f * kPiF * rng.uniform());
+      } else if (type == 2) {
+        ++c_comp;
+        ++ni;
+        float mu;
+        const float Ep = compton(T, M, rng, E, mu, c_trial, c_stest);
+        rotate(dx, dy, dz, mu, 2.0f * kPiF * rng.uniform());
+        dep = E - Ep;
+        E = Ep;
+        smax = curve_eval(T, T->majorant, E);
+        step_scale = 0.69314718055994531f / smax;
+        cslot = -1;
+      } else {
+        ++c_photo;
+        dep = E;
+        alive = false;
+      }
+      r.ncf = nc;
+      r.nif = ni;
+      r.dep = dep;
+      if (dep != 0.0f) {
+        ++c_dep;
+        if (g >= 0) {
+          const mdx_grid_dev& G = D->grids[g];
+          if (G.dose) {
+            atomicAdd(G.dose + 2 * lin, (double)dep);
+            atomicAdd(G.dose + 2 * lin + 1, (double)dep * (double)dep);
+          }
+          if (G.dose_n) atomicAdd(G.dose_n + lin, 1u);
+        } else if (D->bg_dose) {
+          atomicAdd(D->bg_dose, (double)dep);
+          atomicAdd(D->bg_dose + 1, (double)dep * (double)dep);
+          atomicAdd(D->bg_dose + 2, 1.0);
+        }
+      }
+    }
+    tallies_step<MULTI>(TL, acc, r, c_surf);
+    if (nstep >= kMaxSteps) alive = false;
+  }
+  atomicAdd(&cnt[MDX_C_HIST], (unsigned long long)c_hist);
+  atomicAdd(&cnt[MDX_C_STEP], (unsigned long long)c_step);
+  atomicAdd(&cnt[MDX_C_REAL], (unsigned long long)c_real);
+  atomicAdd(&cnt[MDX_C_RAYLEIGH], (unsigned long long)c_ray);
+  atomicAdd(&cnt[MDX_C_COMPTON], (unsigned long long)c_comp);
+  atomicAdd(&cnt[MDX_C_PHOTO], (unsigned long long)c_photo);
+  atomicAdd(&cnt[MDX_C_KAHN_TRIAL], (unsigned long long)c_trial);
+  atomicAdd(&cnt[MDX_C_KAHN_S], (unsigned long long)c_stest);
+  atomicAdd(&cnt[MDX_C_DEPOSIT], (unsigned long long)c_dep);
+  atomicAdd(&cnt[MDX_C_SURFHIT], (unsigned long long)c_surf);
+  atomicAdd(&cnt[MDX_C_ESCAPE], (unsigned long long)c_esc);
+  __syncthreads();
+  if (!MULTI) {
+    const int ntl = jobs[0].TL->n;
+    for (int i = threadIdx.x; i < ntl * kAccPerTally; i += blockDim.x) {
+      double v = acc[i];
+      if (v != 0.0) {
+        int t = i / kAccPerTally;
+        atomicAdd(jobs[0].TL->t[t].acc + (i - t * kAccPerTally), v);
+      }
+    }
+  }
+  if (threadIdx.x < MDX_NCOUNTER && cnt[threadIdx.x]) atomicAdd(counters + threadIdx.x, cnt[threadIdx.x]);
+}
+
 // ------------------------------------------------------------------ inspection kernels
 __global__ void k_eval_curve(const mdx_tables_dev* T, int slot, int which, int n, const float* x, float* out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -413,6 +680,21 @@ hipError_t transport_plain(const mdx_tables_dev* T, const mdx_domain_dev* D, con
   if (need < blocks) blocks = (unsigned)(need ? need : 1);
   hipLaunchKernelGGL(k_transport_plain, dim3(blocks), dim3(kBlock), 0, st, T, D, S, TL, first, n, seed, job,
                      counters);
+  return hipGetLastError();
+}
+
+hipError_t transport_queue(const mdx_tables_dev* T, const mdx_job_dev* jobs, const uint64_t* prefix, int n_jobs,
+                           uint64_t total, unsigned long long* work, unsigned long long* counters, int grid_blocks,
+                           hipStream_t st) {
+  unsigned blocks = (unsigned)grid_blocks;
+  uint64_t need = (total + kBlock - 1) / kBlock;
+  if (need < blocks) blocks = (unsigned)(need ? need : 1);
+  if (n_jobs == 1)
+    hipLaunchKernelGGL(k_transport_queue<false>, dim3(blocks), dim3(kBlock), 0, st, T, jobs, prefix, n_jobs, total,
+                       work, counters);
+  else
+    hipLaunchKernelGGL(k_transport_queue<true>, dim3(blocks), dim3(kBlock), 0, st, T, jobs, prefix, n_jobs, total,
+                       work, counters);
   return hipGetLastError();
 }
 

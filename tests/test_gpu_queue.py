@@ -1,0 +1,127 @@
+"""The persistent queue kernel (per-lane refill) and multi-job launches.
+
+* parity of the queue kernel against the oracle on the slab workload (same
+  Hotelling T^2 test as the plain kernel);
+* a job's results are the same alone and batched with other jobs: its RNG
+  streams are keyed by (seed, job, history), so only the FP64 summation order
+  of its tallies may differ (checked to 1e-12 relative).
+"""
+import numpy as np
+import pytest
+
+import midsx_amd as M
+import oracle
+from midsx_amd import scenes
+from helpers import batch_mean_sigma, hotelling, welch_normal_scores
+from test_gpu_parity import orc_batches, slab_extract
+
+pytestmark = pytest.mark.gpu
+
+
+def test_queue_kernel_slab_matches_oracle():
+    ctx = M.Context(0)
+    ctx.build_tables(list(scenes.C1_MATERIALS))
+    scenes.setup_slab(ctx)
+    k, n = 48, 50000
+    g = []
+    for b in range(k):
+        ctx.reset_tallies()
+        st = ctx.run(n, seed=13, first=b * n, kernel=M.KERNEL_QUEUE)
+        assert st["histories"] == n
+        g.append(slab_extract(ctx, st, n))
+    g = np.array(g)
+    sim = oracle.Simulation(scenes.C1_MATERIALS)
+    scenes.setup_slab(sim)
+    o = np.array(orc_batches(sim, n, k, 303, slab_extract))
+    gm, gs = batch_mean_sigma(g)
+    om, os_ = batch_mean_sigma(o)
+    keep = (gs > 0) & (os_ > 0)
+    z = welch_normal_scores(gm, gs, k, om, os_, k)[keep]
+    assert np.all(np.abs(z) < 3.5), np.max(np.abs(z))
+    p, t2, rank = hotelling(g[:, keep], o[:, keep])
+    assert p > 0.01, (p, t2, rank)
+    ctx.close()
+
+
+def make_jobs(ctx):
+    """Three scenes of different shapes sharing one table set."""
+    jobs = []
+    # job A: slab (job 0)
+    scenes.setup_slab(ctx)
+    jobs.append(0)
+    # job B: small radiography with image + voxel dose
+    b = ctx.job_create()
+    scenes.setup_radiography(ctx, shape=(32, 32, 24), voxel_cm=0.5, seed=7, image=(16, 16))
+    jobs.append(b)
+    # job C: bone block, isotropic source, AABB tally
+    c = ctx.job_create()
+    ctx.set_domain((10.0, 10.0, 10.0), 3)
+    ids = np.full((20, 20, 20), scenes.BONE, np.uint8)
+    ids[5:15, 5:15, 5:15] = scenes.WATER
+    ctx.add_grid(ids, (0.25, 0.25, 0.25), (2.5, 2.5, 2.5))
+    ctx.set_source(direction=M.DIR_ISOTROPIC, position=(5, 5, 5), energy=40e3)
+    ctx.set_tallies([dict(kind=M.TALLY_AACUBOID, p0=(2.5, 2.5, 2.5), p1=(7.5, 7.5, 7.5))])
+    ctx.enable_voxel_dose(True)
+    jobs.append(c)
+    return jobs
+
+
+def snapshot(ctx, job):
+    ctx.job_select(job)
+    out = [ctx.tally(i)["acc"].ravel() for i in range(len(ctx.tallies))]
+    if ctx.num_grids():
+        s, s2, n = ctx.voxel_dose(0)
+        out += [s, s2, n.astype(np.float64)]
+    out.append(np.array(ctx.background_dose()))
+    return np.concatenate(out)
+
+
+def test_job_same_alone_and_in_mix():
+    ctx = M.Context(0)
+    ctx.build_tables(list(set(scenes.C1_MATERIALS) | set(scenes.C2_MATERIALS)))
+    jobs = make_jobs(ctx)
+    n = [300000, 200000, 100000]
+    seeds = [21, 22, 23]
+    alone = []
+    for j, nj, sj in zip(jobs, n, seeds):
+        for jj in jobs:
+            ctx.job_select(jj)
+            ctx.reset_tallies()
+        st = ctx.run_jobs([j], [nj], [0], [sj])
+        assert st["histories"] == nj
+        alone.append(snapshot(ctx, j))
+        if j == jobs[1]:
+            dose_b = ctx.voxel_dose(0)[0].copy()
+    for jj in jobs:
+        ctx.job_select(jj)
+        ctx.reset_tallies()
+    st = ctx.run_jobs(jobs, n, [0, 0, 0], seeds)
+    assert st["histories"] == sum(n)
+    for j, a in zip(jobs, alone):
+        m = snapshot(ctx, j)
+        assert a.shape == m.shape
+        assert np.count_nonzero(a) > 10
+        assert np.allclose(m, a, rtol=1e-12, atol=1e-9), (j, np.max(np.abs(m - a)))
+    # and a job split over two launches equals one launch
+    ctx.job_select(jobs[1])
+    ctx.reset_tallies()
+    ctx.run_jobs([jobs[1]], [120000], [0], [22])
+    ctx.run_jobs([jobs[1], jobs[2]], [80000, 1000], [120000, 0], [22, 99])
+    ctx.job_select(jobs[1])
+    s = ctx.voxel_dose(0)[0]
+    assert np.allclose(s, dose_b, rtol=1e-12, atol=1e-9)
+    ctx.close()
+
+
+def test_job_footprint_and_release():
+    ctx = M.Context(0)
+    ctx.build_tables(list(scenes.C2_MATERIALS))
+    j = ctx.job_create()
+    scenes.setup_radiography(ctx, shape=(64, 64, 32), voxel_cm=0.5, seed=1, image=(8, 8))
+    b = ctx.job_device_bytes(j)
+    assert b >= 64 * 64 * 32 * (1 + 16 + 4)
+    assert ctx.tables_device_bytes() > 0
+    ctx.job_release(j)
+    with pytest.raises(M.MdxError):
+        ctx.job_select(j)
+    ctx.close()
