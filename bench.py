@@ -56,6 +56,30 @@ def cpu_baseline(kw, seconds=12.0, threads=16):
                        "FP64 C++/OpenMP restatement, %.1f s wall" % (n, dt))
 
 
+def timed_steps(ctx, kernel, n, steps, warmup, first):
+    for _ in range(warmup):
+        ctx.run(n, seed=2, first=first, kernel=kernel)
+        first += n
+    ctx.reset_tallies()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        stats.append(ctx.run(n, seed=2, first=first, kernel=kernel))
+        first += n
+    wall = time.perf_counter() - t0
+    return stats, wall, first
+
+
+def copy_probe(mb):
+    """A plain device-to-device copy of known size, for scaling HBM counters."""
+    import torch
+    a = torch.empty(mb * 2 ** 20, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    return 2 * mb * 2 ** 20
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -63,8 +87,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--histories-per-step", type=int, default=10 ** 7)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--copy-probe-mb", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: sharding histories over several GPUs is not built in this "
@@ -79,27 +106,24 @@ def main():
     ctx.build_tables(list(scenes.C2_MATERIALS))
     scenes.setup_radiography(ctx, **kw)
     n = args.histories_per_step
-    first = 0
-    for _ in range(args.warmup):
-        ctx.run(n, seed=2, first=first)
-        first += n
-    ctx.reset_tallies()
-    stats = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        stats.append(ctx.run(n, seed=2, first=first))
-        first += n
-    t1 = time.perf_counter()
-    wall = t1 - t0
+    tuned = None if args.no_autotune else ctx.autotune(probe_histories=1000000)
+    stats, wall, first = timed_steps(ctx, M.KERNEL_QUEUE, n, args.steps, args.warmup, 0)
+    plain = None
+    if not args.no_plain:
+        pstats, pwall, _ = timed_steps(ctx, M.KERNEL_PLAIN, n, args.steps, args.warmup, first)
+        plain = {"value": n * args.steps / pwall, "ms_per_step": pwall / args.steps * 1e3,
+                 "kernel_ms_avg": float(np.mean([s["kernel_ms"] for s in pstats]))}
     total = n * args.steps
     kern_ms = np.array([s["kernel_ms"] for s in stats])
     bytes_per_launch = np.mean([algorithmic_bytes(s) for s in stats])
     achieved = bytes_per_launch / (kern_ms.mean() / 1e3) / 1e9
     agg = {k: sum(s[k] for s in stats) for k in M.STAT_NAMES}
+    value = total / wall
 
     out = {
-        "metric": "photon histories/sec (radiography workload, plain one-simulation-per-launch kernel)",
-        "value": total / wall,
+        "metric": "photon histories/sec (radiography workload; queue kernel vs the plain one-simulation-per-launch "
+                  "kernel at equal histories, hence equal tally sigma)",
+        "value": value,
         "unit": "histories/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -114,9 +138,12 @@ def main():
             "workload": "radiography: 256x256x160 uint8 phantom at 1.25 mm, 15-80 keV Kramers cone beam, "
                         "256x256 image + FP64 voxel dose, 1e8 histories",
             "histories_per_step": n,
-            "kernel": "plain",
+            "kernel": "queue",
+            "queue_params": dict(zip(("blocks_per_cu", "refill_min", "interact_min"), ctx.queue_params())),
             "steps_per_history": agg["steps"] / total,
         },
+        "plain": plain,
+        "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,
@@ -130,6 +157,10 @@ def main():
         "events_per_history": {k: agg[k] / total for k in ("steps", "real", "rayleigh", "compton", "photo",
                                                             "kahn_trials", "deposits", "surface_hits")},
     }
+    if tuned:
+        out["config"]["autotune_probe_ms"] = tuned["probe_ms"]
+    if args.copy_probe_mb:
+        out["copy_probe_bytes"] = copy_probe(args.copy_probe_mb)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(kw, seconds=args.cpu_seconds, threads=args.cpu_threads)
     print(json.dumps(out), flush=True)

@@ -179,6 +179,19 @@ int mdx_reset_tallies(mdx_ctx* ctx);
 int mdx_run(mdx_ctx* ctx, uint64_t n_histories, uint64_t seed, uint64_t first_history, int kernel,
             mdx_stats* stats);
 
+/* Launch parameters of the queue kernel: workgroups per compute unit, and
+ * the number of lanes of a wave that must be free (resp. parked at a real
+ * interaction) before the wave claims new histories (resp. runs the parked
+ * interactions).  They change speed, not results.  No reference counterpart
+ * (the reference's only parallel knob is the OpenMP thread count). */
+int mdx_set_queue_params(mdx_ctx* ctx, int blocks_per_cu, int refill_min, int interact_min);
+int mdx_get_queue_params(mdx_ctx* ctx, int* blocks_per_cu, int* refill_min, int* interact_min);
+/* Autotuner: times the queue kernel on the selected job with probe_histories
+ * per launch over a coordinate search of the three parameters, keeps the
+ * fastest, and resets that job's tallies.  Outputs may be NULL. */
+int mdx_autotune(mdx_ctx* ctx, uint64_t probe_histories, int* blocks_per_cu, int* refill_min, int* interact_min,
+                 double* best_ms);
+
 /* Jobs.  A context holds several scenes ("jobs"), each with its own domain,
  * grids, source and tallies, all sharing the context's tables; the per-scene
  * calls above act on the selected job (job 0 exists from mdx_create).  Used

@@ -38,7 +38,8 @@ EXPORTS = [
     "mdx_get_background_dose", "mdx_interpolate", "mdx_interpolate_host", "mdx_get_table", "mdx_sample_rayleigh",
     "mdx_sample_rayleigh_host", "mdx_get_rita_row", "mdx_sample_compton", "mdx_voxel_lookup",
     "mdx_select_interaction", "mdx_job_create", "mdx_job_select", "mdx_job_release", "mdx_job_device_bytes",
-    "mdx_tables_device_bytes", "mdx_run_jobs",
+    "mdx_tables_device_bytes", "mdx_run_jobs", "mdx_set_queue_params", "mdx_get_queue_params",
+    "mdx_autotune",
 ]
 
 
@@ -312,6 +313,23 @@ class Context:
         out = {name: int(st.counters[i]) for i, name in enumerate(STAT_NAMES)}
         out["kernel_ms"] = st.kernel_ms
         return out
+
+    def set_queue_params(self, blocks_per_cu=8, refill_min=8, interact_min=8):
+        self._call(self.lib.mdx_set_queue_params, self._h, ctypes.c_int(blocks_per_cu), ctypes.c_int(refill_min),
+                   ctypes.c_int(interact_min))
+
+    def autotune(self, probe_histories=1000000):
+        """Tunes the queue kernel on the selected job (resets its tallies)."""
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        ms = ctypes.c_double()
+        self._call(self.lib.mdx_autotune, self._h, ctypes.c_uint64(probe_histories), ctypes.byref(a),
+                   ctypes.byref(b), ctypes.byref(c), ctypes.byref(ms))
+        return dict(blocks_per_cu=a.value, refill_min=b.value, interact_min=c.value, probe_ms=ms.value)
+
+    def queue_params(self):
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._call(self.lib.mdx_get_queue_params, self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        return a.value, b.value, c.value
 
     # ---------------------------------------------------------- jobs
     def job_create(self):

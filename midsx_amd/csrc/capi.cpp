@@ -139,7 +139,10 @@ struct mdx_ctx {
   Job& J() { return *jobs[cur]; }
 
   DBuf d_counters;
+  // queue-kernel launch parameters (mdx_set_queue_params)
   int queue_blocks_per_cu = 8;
+  int refill_min = 8;
+  int interact_min = 16;
   DBuf d_work;  // queue kernel: work counter
   DBuf d_jobs, d_prefix;
 
@@ -411,7 +414,7 @@ void run_jobs(mdx_ctx* c, int n_jobs, const int* ids, const uint64_t* first, con
     hip_check(mdx_launch::transport_queue(c->d_tables.as<mdx_tables_dev>(), c->d_jobs.as<mdx_job_dev>(),
                                           c->d_prefix.as<uint64_t>(), n_jobs, total,
                                           c->d_work.as<unsigned long long>(), cnt, c->num_cu * c->queue_blocks_per_cu,
-                                          c->stream),
+                                          c->refill_min, c->interact_min, c->stream),
               "queue launch");
   hip_check(hipEventRecord(c->ev1, c->stream), "hipEventRecord");
   hip_check(hipStreamSynchronize(c->stream), "transport");
@@ -799,6 +802,74 @@ int mdx_run(mdx_ctx* c, uint64_t n, uint64_t seed, uint64_t first, int kernel, m
     hip_check(hipEventRecord(c->ev1, c->stream), "hipEventRecord");
     hip_check(hipStreamSynchronize(c->stream), "transport");
     read_stats(c, stats);
+  });
+}
+
+int mdx_set_queue_params(mdx_ctx* c, int blocks_per_cu, int refill_min, int interact_min) {
+  return guard([&] {
+    need(c, "NULL context");
+    need(blocks_per_cu >= 1 && blocks_per_cu <= 64, "blocks_per_cu must be in [1, 64]");
+    need(refill_min >= 1 && refill_min <= 64, "refill_min must be in [1, 64]");
+    need(interact_min >= 1 && interact_min <= 64, "interact_min must be in [1, 64]");
+    c->queue_blocks_per_cu = blocks_per_cu;
+    c->refill_min = refill_min;
+    c->interact_min = interact_min;
+  });
+}
+
+int mdx_get_queue_params(mdx_ctx* c, int* blocks_per_cu, int* refill_min, int* interact_min) {
+  return guard([&] {
+    need(c && blocks_per_cu && refill_min && interact_min, "NULL argument");
+    *blocks_per_cu = c->queue_blocks_per_cu;
+    *refill_min = c->refill_min;
+    *interact_min = c->interact_min;
+  });
+}
+
+int mdx_autotune(mdx_ctx* c, uint64_t probe_histories, int* blocks_per_cu, int* refill_min, int* interact_min,
+                 double* best_ms) {
+  return guard([&] {
+    need(c, "NULL context");
+    need(probe_histories > 0, "probe_histories must be positive");
+    const int job = c->cur;
+    const uint64_t seed = 0x7475u;
+    uint64_t first = 1ull << 40;  // a stream region no production run uses
+    auto probe = [&](int b, int r, int i) {
+      c->queue_blocks_per_cu = b;
+      c->refill_min = r;
+      c->interact_min = i;
+      double best = 1e300;
+      for (int rep = 0; rep < 2; ++rep) {
+        mdx_stats st;
+        run_jobs(c, 1, &job, &first, &probe_histories, &seed, &st);
+        first += probe_histories;
+        best = std::min(best, st.kernel_ms);
+      }
+      return best;
+    };
+    int b = c->queue_blocks_per_cu, r = c->refill_min, i = c->interact_min;
+    double cur = probe(b, r, i);
+    // coordinate search: interaction batch, refill batch, then occupancy
+    for (int v : {8, 12, 16, 24, 32}) {
+      double t = probe(b, r, v);
+      if (t < cur) { cur = t; i = v; }
+    }
+    for (int v : {2, 4, 8, 16}) {
+      double t = probe(b, v, i);
+      if (t < cur) { cur = t; r = v; }
+    }
+    for (int v : {4, 8, 12, 16}) {
+      double t = probe(v, r, i);
+      if (t < cur) { cur = t; b = v; }
+    }
+    c->queue_blocks_per_cu = b;
+    c->refill_min = r;
+    c->interact_min = i;
+    zero_tallies(c);
+    if (blocks_per_cu) *blocks_per_cu = b;
+    if (refill_min) *refill_min = r;
+    if (interact_min) *interact_min = i;
+    if (best_ms) *best_ms = cur;
   });
 }
 

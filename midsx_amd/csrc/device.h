@@ -24,7 +24,7 @@ constexpr float kEps = 1e-9f;  // constants.h:14
 struct Philox {
   uint32_t k0, k1;
   uint32_t c0, c1, c2, c3;
-  uint32_t buf[4];
+  uint32_t b0, b1, b2, b3;  // unused outputs of the last block, consumed b0 first
   int left;
 
   __device__ void init(uint64_t seed, uint32_t job, uint64_t history) {
@@ -49,19 +49,60 @@ struct Philox {
       a += 0x9E3779B9u;
       b += 0xBB67AE85u;
     }
-    buf[0] = x0;
-    buf[1] = x1;
-    buf[2] = x2;
-    buf[3] = x3;
+    b0 = x0;
+    b1 = x1;
+    b2 = x2;
+    b3 = x3;
     left = 4;
     ++c0;
   }
   // Uniform in (0, 1] with 24 random bits (never 0, so -log(u) is finite).
+  // The block's words are used in order b0, b1, b2, b3 (a register shift,
+  // no indexed access).
   __device__ float uniform() {
     if (left == 0) refill();
-    uint32_t x = buf[4 - left];
+    uint32_t x = b0;
+    b0 = b1;
+    b1 = b2;
+    b2 = b3;
     --left;
     return (float)((x >> 8) + 1u) * (1.0f / 16777216.0f);
+  }
+};
+
+// Philox-2x32-10 for the Woodcock step: one block per step gives the step's
+// two uniforms (free path, delta test), keyed by (seed, job, history high
+// word) with counter (history low word, step index).  Every lane draws the
+// same amount per step, so no lane waits on another's buffer refill.
+__device__ __forceinline__ uint32_t mix32(uint32_t a, uint32_t b) {
+  uint32_t h = a * 0x9E3779B1u ^ (b + 0x7F4A7C15u + (a << 6) + (a >> 2));
+  h ^= h >> 16;
+  h *= 0x85EBCA6Bu;
+  h ^= h >> 13;
+  h *= 0xC2B2AE35u;
+  h ^= h >> 16;
+  return h;
+}
+
+struct StepRng {
+  uint32_t key, hist, step;
+  __device__ void init(uint64_t seed, uint32_t job, uint64_t history) {
+    key = mix32(mix32((uint32_t)seed, (uint32_t)(seed >> 32)), mix32(job, (uint32_t)(history >> 32)));
+    hist = (uint32_t)history;
+    step = 0;
+  }
+  __device__ void next(float& u1, float& u2) {
+    uint32_t x0 = hist, x1 = step, k = key;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+      uint32_t hi = __umulhi(0xD256D193u, x0), lo = 0xD256D193u * x0;
+      x0 = hi ^ k ^ x1;
+      x1 = lo;
+      k += 0x9E3779B9u;
+    }
+    ++step;
+    u1 = (float)((x0 >> 8) + 1u) * (1.0f / 16777216.0f);
+    u2 = (float)((x1 >> 8) + 1u) * (1.0f / 16777216.0f);
   }
 };
 

@@ -45,9 +45,40 @@ struct Step {
   bool interact;
 };
 
+// LDS budget for block-level IncidentEnergy histograms (doubles); tallies whose
+// histograms do not fit go straight to global atomics.
+constexpr int kLdsHist = 2048;
+
+// Offsets of each tally's histogram in the block's LDS histogram (-1: global).
+__device__ __forceinline__ void hist_layout(const mdx_tallies_dev* TL, int* off) {
+  int used = 0;
+  for (int t = 0; t < TL->n; ++t) {
+    const mdx_tally_dev& T = TL->t[t];
+    int need = (T.kind != MDX_TALLY_AACUBOID && T.hist_bins > 0) ? MDX_NBUCKET * T.hist_bins : 0;
+    if (need > 0 && used + need <= kLdsHist) {
+      off[t] = used;
+      used += need;
+    } else {
+      off[t] = -1;
+    }
+  }
+}
+
+__device__ void flush_hist(const mdx_tallies_dev* TL, const int* off, double* lhist) {
+  for (int t = 0; t < TL->n; ++t) {
+    if (off[t] < 0) continue;
+    const mdx_tally_dev& T = TL->t[t];
+    for (int i = threadIdx.x; i < MDX_NBUCKET * T.hist_bins; i += blockDim.x) {
+      double v = lhist[off[t] + i];
+      if (v != 0.0) atomicAdd(T.hist + i, v);
+    }
+  }
+}
+
 // surface_tally.cpp:16-34 (+ DiscSurfaceTally :63-69, RectangularSurfaceTally
 // :100-112) and volume_tally.cpp:5-91 for one recorded step.
-__device__ void process_tallies(const mdx_tallies_dev* TL, double* acc, const Step& r, uint32_t& surf_hits) {
+__device__ void process_tallies(const mdx_tallies_dev* TL, double* acc, const Step& r, uint32_t& surf_hits,
+                                double* lhist, const int* hoff) {
   for (int t = 0; t < TL->n; ++t) {
     const mdx_tally_dev& T = TL->t[t];
     if (T.kind == MDX_TALLY_AACUBOID) {
@@ -110,7 +141,11 @@ __device__ void process_tallies(const mdx_tallies_dev* TL, double* acc, const St
       lds_event(acc, t, MDX_SQ_NUMBER_OF_PHOTONS, b, 1.0);
       if (T.hist_bins > 0) {
         float fb = (r.E - T.hist_lo) * T.hist_inv_w;
-        if (fb >= 0.0f && fb < (float)T.hist_bins) atomicAdd(T.hist + b * T.hist_bins + (int)fb, 1.0);
+        if (fb >= 0.0f && fb < (float)T.hist_bins) {
+          const int bin = b * T.hist_bins + (int)fb;
+          if (hoff[t] >= 0) atomicAdd(lhist + hoff[t] + bin, 1.0);
+          else atomicAdd(T.hist + bin, 1.0);
+        }
       }
     }
   }
@@ -210,9 +245,13 @@ k_transport_plain(const mdx_tables_dev* __restrict__ T, const mdx_domain_dev* __
                   const mdx_source_dev* __restrict__ S, const mdx_tallies_dev* __restrict__ TL, uint64_t first,
                   uint64_t n, uint64_t seed, uint32_t job, unsigned long long* __restrict__ counters) {
   __shared__ double acc[MDX_MAX_TALLIES * kAccPerTally];
+  __shared__ double lhist[kLdsHist];
   __shared__ unsigned long long cnt[MDX_NCOUNTER];
+  int hoff[MDX_MAX_TALLIES];
+  hist_layout(TL, hoff);
   const int ntl = TL->n;
   for (int i = threadIdx.x; i < ntl * kAccPerTally; i += blockDim.x) acc[i] = 0.0;
+  for (int i = threadIdx.x; i < kLdsHist; i += blockDim.x) lhist[i] = 0.0;
   if (threadIdx.x < MDX_NCOUNTER) cnt[threadIdx.x] = 0;
   __syncthreads();
 
@@ -247,7 +286,7 @@ k_transport_plain(const mdx_tables_dev* __restrict__ T, const mdx_domain_dev* __
       pz = pz + s * dz;
       if (!in_domain(D, px, py, pz)) {  // :47-52
         ++c_esc;
-        process_tallies(TL, acc, r, c_surf);
+        process_tallies(TL, acc, r, c_surf, lhist, hoff);
         break;
       }
       uint64_t lin = 0;
@@ -309,7 +348,7 @@ k_transport_plain(const mdx_tables_dev* __restrict__ T, const mdx_domain_dev* __
           if (dn) atomicAdd(dn, 1u);
         }
       }
-      process_tallies(TL, acc, r, c_surf);
+      process_tallies(TL, acc, r, c_surf, lhist, hoff);
       if (!alive) break;
     }
   }
@@ -332,6 +371,7 @@ k_transport_plain(const mdx_tables_dev* __restrict__ T, const mdx_domain_dev* __
       atomicAdd(TL->t[t].acc + (i - t * kAccPerTally), v);
     }
   }
+  flush_hist(TL, hoff, lhist);
   if (threadIdx.x < MDX_NCOUNTER && cnt[threadIdx.x]) atomicAdd(counters + threadIdx.x, cnt[threadIdx.x]);
 }
 
@@ -352,9 +392,10 @@ __device__ __forceinline__ void event_global(double* acc, int q, int b, double v
 }
 
 template <bool MULTI>
-__device__ void tallies_step(const mdx_tallies_dev* TL, double* acc, const Step& r, uint32_t& surf_hits) {
+__device__ void tallies_step(const mdx_tallies_dev* TL, double* acc, const Step& r, uint32_t& surf_hits,
+                             double* lhist, const int* hoff) {
   if (!MULTI) {
-    process_tallies(TL, acc, r, surf_hits);
+    process_tallies(TL, acc, r, surf_hits, lhist, hoff);
     return;
   }
   for (int t = 0; t < TL->n; ++t) {
@@ -425,16 +466,26 @@ __device__ void tallies_step(const mdx_tallies_dev* TL, double* acc, const Step&
   }
 }
 
+// Waves per SIMD the queue kernel is compiled for (register budget).
+#ifndef MDX_QUEUE_WAVES_PER_EU
+#define MDX_QUEUE_WAVES_PER_EU 4
+#endif
+
 template <bool MULTI>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MDX_QUEUE_WAVES_PER_EU)))
 k_transport_queue(const mdx_tables_dev* __restrict__ T, const mdx_job_dev* __restrict__ jobs,
                   const uint64_t* __restrict__ prefix, int n_jobs, uint64_t total,
-                  unsigned long long* __restrict__ work, unsigned long long* __restrict__ counters) {
+                  unsigned long long* __restrict__ work, unsigned long long* __restrict__ counters,
+                  int refill_min, int interact_min) {
   __shared__ double acc[MULTI ? 1 : MDX_MAX_TALLIES * kAccPerTally];
+  __shared__ double lhist[MULTI ? 1 : kLdsHist];
   __shared__ unsigned long long cnt[MDX_NCOUNTER];
+  int hoff[MDX_MAX_TALLIES];
   if (!MULTI) {
+    hist_layout(jobs[0].TL, hoff);
     const int ntl = jobs[0].TL->n;
     for (int i = threadIdx.x; i < ntl * kAccPerTally; i += blockDim.x) acc[i] = 0.0;
+    for (int i = threadIdx.x; i < kLdsHist; i += blockDim.x) lhist[i] = 0.0;
   }
   if (threadIdx.x < MDX_NCOUNTER) cnt[threadIdx.x] = 0;
   __syncthreads();
@@ -442,28 +493,36 @@ k_transport_queue(const mdx_tables_dev* __restrict__ T, const mdx_job_dev* __res
   const int lane = threadIdx.x & 63;
   uint32_t c_hist = 0, c_step = 0, c_real = 0, c_ray = 0, c_comp = 0, c_photo = 0, c_trial = 0, c_stest = 0,
            c_dep = 0, c_surf = 0, c_esc = 0;
-  bool alive = false, exhausted = false;
+  // lane state: alive = carries a photon; pend = its last step ended in a
+  // real interaction that waits for the wave's interaction phase
+  bool alive = false, pend = false, exhausted = false;
   const mdx_domain_dev* D = jobs[0].D;
   const mdx_source_dev* S = jobs[0].S;
   const mdx_tallies_dev* TL = jobs[0].TL;
-  Philox rng;
+  Philox rng;     // source and interaction sampling
+  StepRng srng;   // Woodcock steps
   float px = 0, py = 0, pz = 0, dx = 0, dy = 0, dz = 1, E = 1;
   int nc = 0, ni = 0;
   float smax = 1, step_scale = 1;  // step_scale = ln 2 / sigma_max
   int cslot = -1;
   float stot = 0, thr = 0;         // thr = 1 - sigma_tot / sigma_max
+  // the pending step: start point, length, voxel
+  float qx = 0, qy = 0, qz = 0, qs = 0;
+  int qg = -1;
+  uint64_t qlin = 0;
   uint32_t nstep = 0;
   for (;;) {
-    // refill finished lanes with the next histories
-    const bool need = !alive && !exhausted;
-    const unsigned long long mask = __ballot(need);
-    if (mask) {
-      const int leader = __ffsll((long long)mask) - 1;
+    // ---- refill: claim histories for free lanes, in batches
+    const bool free_lane = !alive && !exhausted;
+    const unsigned long long fmask = __ballot(free_lane);
+    const bool steppable = __ballot(alive && !pend) != 0ull;
+    if (fmask && (__popcll(fmask) >= refill_min || !steppable)) {
+      const int leader = __ffsll((long long)fmask) - 1;
       unsigned long long base = 0;
-      if (lane == leader) base = atomicAdd(work, (unsigned long long)__popcll(mask));
+      if (lane == leader) base = atomicAdd(work, (unsigned long long)__popcll(fmask));
       base = __shfl(base, leader);
-      if (need) {
-        const uint64_t g = base + __popcll(mask & ((1ull << lane) - 1ull));
+      if (free_lane) {
+        const uint64_t g = base + __popcll(fmask & ((1ull << lane) - 1ull));
         if (g < total) {
           uint64_t local = g;
           int j = 0;
@@ -480,6 +539,7 @@ k_transport_queue(const mdx_tables_dev* __restrict__ T, const mdx_job_dev* __res
             TL = jobs[j].TL;
           }
           rng.init(jobs[j].seed, jobs[j].job_id, jobs[j].first + local);
+          srng.init(jobs[j].seed, jobs[j].job_id, jobs[j].first + local);
           generate(S, rng, px, py, pz, dx, dy, dz, E);
           ++c_hist;
           nc = ni = 0;
@@ -494,88 +554,100 @@ k_transport_queue(const mdx_tables_dev* __restrict__ T, const mdx_job_dev* __res
       }
     }
     if (__ballot(alive) == 0ull) break;
-    if (!alive) continue;
 
-    // one Woodcock step (physics_engine.cpp:28-79)
-    ++c_step;
-    ++nstep;
-    Step r;
-    r.px = px; r.py = py; r.pz = pz;
-    r.dx = dx; r.dy = dy; r.dz = dz;
-    r.E = E;
-    r.nc0 = nc; r.ni0 = ni;
-    r.ncf = 0; r.nif = 0;
-    r.dep = 0.0f;
-    r.interact = false;
-    const float s = -__log2f(rng.uniform()) * step_scale;
-    r.s = s;
-    px = px + s * dx;
-    py = py + s * dy;
-    pz = pz + s * dz;
-    if (!in_domain(D, px, py, pz)) {
-      ++c_esc;
-      tallies_step<MULTI>(TL, acc, r, c_surf);
-      alive = false;
-      continue;
-    }
-    uint64_t lin = 0;
-    const int g = voxel_lookup(D, px, py, pz, lin);
-    const int slot = g < 0 ? D->bg_slot : (int)T->id_to_slot[D->grids[g].ids[lin]];
-    const mdx_material_dev& M = T->mat[slot];
-    if (slot != cslot) {
-      stot = curve_eval(T, M.total, E);
-      thr = 1.0f - stot / smax;
-      cslot = slot;
-    }
-    if (!(rng.uniform() < thr)) {
-      ++c_real;
-      r.interact = true;
-      const float sc = curve_eval(T, M.coh, E);
-      const float si = curve_eval(T, M.incoh, E);
-      const int type = select_type(sc, si, stot, rng.uniform());
-      float dep = 0.0f;
-      if (type == 1) {
-        ++c_ray;
-        ++nc;
-        const float mu = rita_sample(T, M, E, rng.uniform());
-        rotate(dx, dy, dz, mu, 2.0f * kPiF * rng.uniform());
-      } else if (type == 2) {
-        ++c_comp;
-        ++ni;
-        float mu;
-        const float Ep = compton(T, M, rng, E, mu, c_trial, c_stest);
-        rotate(dx, dy, dz, mu, 2.0f * kPiF * rng.uniform());
-        dep = E - Ep;
-        E = Ep;
-        smax = curve_eval(T, T->majorant, E);
-        step_scale = 0.69314718055994531f / smax;
-        cslot = -1;
-      } else {
-        ++c_photo;
-        dep = E;
+    // ---- one Woodcock step for every lane that can step (physics_engine.cpp:38-66)
+    if (alive && !pend) {
+      ++c_step;
+      ++nstep;
+      float u1, u2;
+      srng.next(u1, u2);
+      const float s = -__log2f(u1) * step_scale;
+      const float x0 = px, y0 = py, z0 = pz;
+      px = px + s * dx;
+      py = py + s * dy;
+      pz = pz + s * dz;
+      if (!in_domain(D, px, py, pz)) {
+        ++c_esc;
+        Step r{x0, y0, z0, dx, dy, dz, s, E, nc, ni, 0, 0, 0.0f, false};
+        tallies_step<MULTI>(TL, acc, r, c_surf, lhist, hoff);
         alive = false;
-      }
-      r.ncf = nc;
-      r.nif = ni;
-      r.dep = dep;
-      if (dep != 0.0f) {
-        ++c_dep;
-        if (g >= 0) {
-          const mdx_grid_dev& G = D->grids[g];
-          if (G.dose) {
-            atomicAdd(G.dose + 2 * lin, (double)dep);
-            atomicAdd(G.dose + 2 * lin + 1, (double)dep * (double)dep);
-          }
-          if (G.dose_n) atomicAdd(G.dose_n + lin, 1u);
-        } else if (D->bg_dose) {
-          atomicAdd(D->bg_dose, (double)dep);
-          atomicAdd(D->bg_dose + 1, (double)dep * (double)dep);
-          atomicAdd(D->bg_dose + 2, 1.0);
+      } else {
+        uint64_t lin = 0;
+        const int g = voxel_lookup(D, px, py, pz, lin);
+        const int slot = g < 0 ? D->bg_slot : (int)T->id_to_slot[D->grids[g].ids[lin]];
+        if (slot != cslot) {
+          stot = curve_eval(T, T->mat[slot].total, E);
+          thr = 1.0f - stot / smax;
+          cslot = slot;
+        }
+        if (!(u2 < thr)) {  // real interaction: park it for the interaction phase
+          pend = true;
+          qx = x0; qy = y0; qz = z0; qs = s;
+          qg = g;
+          qlin = lin;
+        } else {
+          Step r{x0, y0, z0, dx, dy, dz, s, E, nc, ni, 0, 0, 0.0f, false};
+          tallies_step<MULTI>(TL, acc, r, c_surf, lhist, hoff);
+          if (nstep >= kMaxSteps) alive = false;
         }
       }
     }
-    tallies_step<MULTI>(TL, acc, r, c_surf);
-    if (nstep >= kMaxSteps) alive = false;
+
+    // ---- interaction phase: the parked lanes together (physics_engine.cpp:62-78)
+    const unsigned long long pmask = __ballot(pend);
+    if (pmask && (__popcll(pmask) >= interact_min || __ballot(alive && !pend) == 0ull)) {
+      if (pend) {
+        pend = false;
+        ++c_real;
+        const mdx_material_dev& M = T->mat[cslot];
+        Step r{qx, qy, qz, dx, dy, dz, qs, E, nc, ni, 0, 0, 0.0f, true};
+        const float sc = curve_eval(T, M.coh, E);
+        const float si = curve_eval(T, M.incoh, E);
+        const int type = select_type(sc, si, stot, rng.uniform());
+        float dep = 0.0f;
+        if (type == 1) {
+          ++c_ray;
+          ++nc;
+          const float mu = rita_sample(T, M, E, rng.uniform());
+          rotate(dx, dy, dz, mu, 2.0f * kPiF * rng.uniform());
+        } else if (type == 2) {
+          ++c_comp;
+          ++ni;
+          float mu;
+          const float Ep = compton(T, M, rng, E, mu, c_trial, c_stest);
+          rotate(dx, dy, dz, mu, 2.0f * kPiF * rng.uniform());
+          dep = E - Ep;
+          E = Ep;
+          smax = curve_eval(T, T->majorant, E);
+          step_scale = 0.69314718055994531f / smax;
+          cslot = -1;
+        } else {
+          ++c_photo;
+          dep = E;
+          alive = false;
+        }
+        r.ncf = nc;
+        r.nif = ni;
+        r.dep = dep;
+        if (dep != 0.0f) {
+          ++c_dep;
+          if (qg >= 0) {
+            const mdx_grid_dev& G = D->grids[qg];
+            if (G.dose) {
+              atomicAdd(G.dose + 2 * qlin, (double)dep);
+              atomicAdd(G.dose + 2 * qlin + 1, (double)dep * (double)dep);
+            }
+            if (G.dose_n) atomicAdd(G.dose_n + qlin, 1u);
+          } else if (D->bg_dose) {
+            atomicAdd(D->bg_dose, (double)dep);
+            atomicAdd(D->bg_dose + 1, (double)dep * (double)dep);
+            atomicAdd(D->bg_dose + 2, 1.0);
+          }
+        }
+        tallies_step<MULTI>(TL, acc, r, c_surf, lhist, hoff);
+        if (nstep >= kMaxSteps) alive = false;
+      }
+    }
   }
   atomicAdd(&cnt[MDX_C_HIST], (unsigned long long)c_hist);
   atomicAdd(&cnt[MDX_C_STEP], (unsigned long long)c_step);
@@ -598,6 +670,7 @@ k_transport_queue(const mdx_tables_dev* __restrict__ T, const mdx_job_dev* __res
         atomicAdd(jobs[0].TL->t[t].acc + (i - t * kAccPerTally), v);
       }
     }
+    flush_hist(jobs[0].TL, hoff, lhist);
   }
   if (threadIdx.x < MDX_NCOUNTER && cnt[threadIdx.x]) atomicAdd(counters + threadIdx.x, cnt[threadIdx.x]);
 }
@@ -685,16 +758,18 @@ hipError_t transport_plain(const mdx_tables_dev* T, const mdx_domain_dev* D, con
 
 hipError_t transport_queue(const mdx_tables_dev* T, const mdx_job_dev* jobs, const uint64_t* prefix, int n_jobs,
                            uint64_t total, unsigned long long* work, unsigned long long* counters, int grid_blocks,
-                           hipStream_t st) {
+                           int refill_min, int interact_min, hipStream_t st) {
   unsigned blocks = (unsigned)grid_blocks;
   uint64_t need = (total + kBlock - 1) / kBlock;
   if (need < blocks) blocks = (unsigned)(need ? need : 1);
+  refill_min = refill_min < 1 ? 1 : (refill_min > 64 ? 64 : refill_min);
+  interact_min = interact_min < 1 ? 1 : (interact_min > 64 ? 64 : interact_min);
   if (n_jobs == 1)
     hipLaunchKernelGGL(k_transport_queue<false>, dim3(blocks), dim3(kBlock), 0, st, T, jobs, prefix, n_jobs, total,
-                       work, counters);
+                       work, counters, refill_min, interact_min);
   else
     hipLaunchKernelGGL(k_transport_queue<true>, dim3(blocks), dim3(kBlock), 0, st, T, jobs, prefix, n_jobs, total,
-                       work, counters);
+                       work, counters, refill_min, interact_min);
   return hipGetLastError();
 }
 
