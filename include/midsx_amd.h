@@ -200,15 +200,35 @@ int mdx_autotune(mdx_ctx* ctx, uint64_t probe_histories, int* blocks_per_cu, int
 int mdx_job_create(mdx_ctx* ctx, int* job);   /* creates and selects */
 int mdx_job_select(mdx_ctx* ctx, int job);
 int mdx_job_release(mdx_ctx* ctx, int job);   /* frees its device memory */
+/* The selected job uses another job's domain (grids and per-voxel dose)
+ * instead of its own, e.g. the views of one CT scan.  A job whose domain is
+ * in use by another cannot be released before that one. */
+int mdx_job_alias_domain(mdx_ctx* ctx, int src_job);
 /* Device memory held by one job (grids, dose, tallies) and by the tables. */
 int mdx_job_device_bytes(mdx_ctx* ctx, int job, uint64_t* bytes);
 int mdx_tables_device_bytes(mdx_ctx* ctx, uint64_t* bytes);
 /* One launch of the queue kernel over several jobs' history slices
- * [first[i], first[i] + n[i]) with RNG streams keyed by (seed[i], job,
- * history): a job's results do not depend on which other jobs share the
- * launch.  Blocking. */
+ * [first[i], first[i] + n[i]) with RNG streams keyed by (seed[i], history):
+ * a job's results do not depend on which other jobs share the launch nor on
+ * its job slot, so give different jobs different seeds.  Blocking. */
 int mdx_run_jobs(mdx_ctx* ctx, int n_jobs, const int* jobs, const uint64_t* first, const uint64_t* n,
                  const uint64_t* seed, mdx_stats* stats);
+/* Run mdx_run_jobs rounds as a replayed hipGraph (round buffer copy, counter
+ * zeroing, queue kernel), at most 1024 jobs per round. */
+int mdx_set_graphs(mdx_ctx* ctx, int enable);
+/* mdx_run_jobs split in two: the round is queued and the call returns; while
+ * it runs, mdx_get_tally, mdx_get_voxel_dose and mdx_get_background_dose copy
+ * on a second stream beside it (a job in the running round reads a partial
+ * sum), and every other call waits for the round first.  mdx_wait waits for
+ * the round and returns its counters.  One round in flight at a time. */
+int mdx_run_jobs_async(mdx_ctx* ctx, int n_jobs, const int* jobs, const uint64_t* first, const uint64_t* n,
+                       const uint64_t* seed);
+int mdx_wait(mdx_ctx* ctx, mdx_stats* stats);
+/* Free and total device memory (hipMemGetInfo), for admission by footprint. */
+int mdx_mem_info(mdx_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
+/* Pinned host memory for fast tally and dose reads (hipHostMalloc). */
+int mdx_host_alloc(mdx_ctx* ctx, uint64_t bytes, void** out);
+int mdx_host_free(void* p);
 
 /* PhysicsEngine::getVolumeQuantityContainers / getSurfaceQuantityContainers
  * (physics_engine.cpp:124-158).  acc: MDX_TALLY_ACC_SIZE doubles; hist:

@@ -8,6 +8,7 @@ physics: every number comes from the HIP library, and importing the module
 fails loudly when the library has not been built.
 """
 import ctypes
+import weakref
 import os
 
 import numpy as np
@@ -39,7 +40,8 @@ EXPORTS = [
     "mdx_sample_rayleigh_host", "mdx_get_rita_row", "mdx_sample_compton", "mdx_voxel_lookup",
     "mdx_select_interaction", "mdx_job_create", "mdx_job_select", "mdx_job_release", "mdx_job_device_bytes",
     "mdx_tables_device_bytes", "mdx_run_jobs", "mdx_set_queue_params", "mdx_get_queue_params",
-    "mdx_autotune",
+    "mdx_autotune", "mdx_job_alias_domain", "mdx_set_graphs",
+    "mdx_run_jobs_async", "mdx_wait", "mdx_mem_info", "mdx_host_alloc", "mdx_host_free",
 ]
 
 
@@ -349,6 +351,13 @@ class Context:
         if self._job == j:
             self._job = 0
 
+    def job_alias_domain(self, src):
+        """The selected job uses job src's domain (grids, voxel dose)."""
+        self._call(self.lib.mdx_job_alias_domain, self._h, ctypes.c_int(src))
+
+    def set_graphs(self, on=True):
+        self._call(self.lib.mdx_set_graphs, self._h, ctypes.c_int(1 if on else 0))
+
     def job_device_bytes(self, j):
         b = ctypes.c_uint64()
         self._call(self.lib.mdx_job_device_bytes, self._h, ctypes.c_int(j), ctypes.byref(b))
@@ -374,6 +383,24 @@ class Context:
         out["kernel_ms"] = st.kernel_ms
         return out
 
+    def run_jobs_async(self, jobs, n, first=None, seeds=None):
+        """Queues one round (as run_jobs) and returns; wait() ends it.  Tally
+        and dose reads overlap the round; every other call waits for it."""
+        k = len(jobs)
+        ids = np.ascontiguousarray(jobs, dtype=np.int32)
+        nn = np.ascontiguousarray(n, dtype=np.uint64)
+        ff = np.ascontiguousarray(first if first is not None else np.zeros(k), dtype=np.uint64)
+        ss = np.ascontiguousarray(seeds if seeds is not None else np.ones(k), dtype=np.uint64)
+        self._call(self.lib.mdx_run_jobs_async, self._h, ctypes.c_int(k), _ptr(ids, ctypes.c_int),
+                   _ptr(ff, ctypes.c_uint64), _ptr(nn, ctypes.c_uint64), _ptr(ss, ctypes.c_uint64))
+
+    def wait(self):
+        st = _Stats()
+        self._call(self.lib.mdx_wait, self._h, ctypes.byref(st))
+        out = {name: int(st.counters[i]) for i, name in enumerate(STAT_NAMES)}
+        out["kernel_ms"] = st.kernel_ms
+        return out
+
     def tally(self, i):
         t = self.tallies[i]
         acc = np.zeros(NQ * NBUCKET * NFIELD)
@@ -389,14 +416,35 @@ class Context:
             out["image"] = image.reshape(t["ny"], t["nx"], 3)
         return out
 
-    def voxel_dose(self, g):
+    def voxel_dose(self, g, out=None, counts=True):
+        """(sum, sum^2, count) of the deposits per voxel, x fastest.  out: a
+        float64 buffer of 2 * voxels to read (sum, sum^2) pairs into;
+        counts=False skips the counts (returned as None)."""
         dims, _, _ = self.grid_info(g)
         nv = dims[0] * dims[1] * dims[2]
-        ss = np.zeros(2 * nv)
-        cnt = np.zeros(nv, dtype=np.uint32)
+        ss = np.zeros(2 * nv) if out is None else out[:2 * nv]
+        assert ss.dtype == np.float64 and ss.flags.c_contiguous and ss.size == 2 * nv
+        cnt = np.zeros(nv, dtype=np.uint32) if counts else None
         self._call(self.lib.mdx_get_voxel_dose, self._h, ctypes.c_int(g), _ptr(ss, ctypes.c_double),
-                   _ptr(cnt, ctypes.c_uint32))
+                   None if cnt is None else _ptr(cnt, ctypes.c_uint32))
         return ss[0::2], ss[1::2], cnt
+
+    def mem_info(self):
+        f, t = ctypes.c_uint64(), ctypes.c_uint64()
+        self._call(self.lib.mdx_mem_info, self._h, ctypes.byref(f), ctypes.byref(t))
+        return f.value, t.value
+
+    def pinned(self, n, dtype=np.float64):
+        """A numpy array of n elements in pinned host memory, freed when the
+        array (and every view of it) is garbage collected."""
+        nbytes = int(n) * np.dtype(dtype).itemsize
+        p = ctypes.c_void_p()
+        self._call(self.lib.mdx_host_alloc, self._h, ctypes.c_uint64(nbytes), ctypes.byref(p))
+        buf = (ctypes.c_char * max(1, nbytes)).from_address(p.value)
+        arr = np.frombuffer(buf, dtype=np.uint8, count=nbytes).view(dtype)
+        lib, addr = self.lib, p.value
+        weakref.finalize(buf, lambda: lib.mdx_host_free(ctypes.c_void_p(addr)))
+        return arr
 
     def background_dose(self):
         out = (ctypes.c_double * 3)()

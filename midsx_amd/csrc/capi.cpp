@@ -96,6 +96,7 @@ struct TallyHost {
 // A context holds several (one per served job); the per-scene calls of the
 // C-ABI act on the selected one.
 struct Job {
+  int domain_of = -1;  // >= 0: uses that job's domain (grids, dose) instead of its own
   bool domain_set = false;
   double dim_space[3] = {0, 0, 0};
   int background_id = -1;
@@ -137,6 +138,13 @@ struct mdx_ctx {
   std::vector<std::unique_ptr<Job>> jobs;
   int cur = 0;
   Job& J() { return *jobs[cur]; }
+  // The job owning the selected job's domain (the job itself unless it aliases another's).
+  Job& DJ() {
+    const int d = jobs[cur]->domain_of;
+    if (d < 0) return *jobs[cur];
+    if (d >= (int)jobs.size() || !jobs[d]) throw std::runtime_error("the job whose domain this job uses was released");
+    return *jobs[d];
+  }
 
   DBuf d_counters;
   // queue-kernel launch parameters (mdx_set_queue_params)
@@ -145,6 +153,15 @@ struct mdx_ctx {
   int interact_min = 16;
   DBuf d_work;  // queue kernel: work counter
   DBuf d_jobs, d_prefix;
+
+  // Round buffer of the graph path: [n_jobs, total, prefix[kRoundMax + 1], jobs[kRoundMax]]
+  bool use_graphs = false;
+  bool inflight = false;  // a round was launched by mdx_run_jobs_async and not yet waited for
+  hipStream_t io_stream = nullptr;
+  void* h_round = nullptr;  // pinned
+  DBuf d_round;
+  hipGraphExec_t graph = nullptr;
+  int graph_params[3] = {0, 0, 0};
 
   void ensure_registry() {
     if (registry_loaded) return;
@@ -155,6 +172,11 @@ struct mdx_ctx {
 };
 
 namespace {
+
+constexpr int kRoundMax = 1024;
+constexpr size_t kRoundHeader = 2 * sizeof(uint64_t);
+constexpr size_t kRoundPrefix = (kRoundMax + 1) * sizeof(uint64_t);
+constexpr size_t kRoundBytes = kRoundHeader + kRoundPrefix + kRoundMax * sizeof(mdx_job_dev);
 
 template <class F>
 int guard(F&& f) {
@@ -190,9 +212,23 @@ void need(bool ok, const char* msg) {
   if (!ok) throw std::invalid_argument(msg);
 }
 
-void set_device(mdx_ctx* c) {
+// For reads that may overlap a round in flight (they copy on the I/O stream).
+void set_device_nowait(mdx_ctx* c) {
   if (c->device < 0) throw std::invalid_argument("host-only context (device -1) cannot run device work");
   hip_check(hipSetDevice(c->device), "hipSetDevice");
+}
+
+// Every call that changes device state first waits for a round in flight.
+void set_device(mdx_ctx* c) {
+  set_device_nowait(c);
+  if (c->inflight) hip_check(hipStreamSynchronize(c->stream), "round in flight");
+}
+
+// Device-to-host copy on the I/O stream, which runs beside a round in flight.
+void read_back(mdx_ctx* c, void* dst, const void* src, size_t bytes, const char* what) {
+  if (!bytes) return;
+  hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->io_stream), what);
+  hip_check(hipStreamSynchronize(c->io_stream), what);
 }
 
 void upload_domain(mdx_ctx* c) {
@@ -300,6 +336,7 @@ void build_and_upload(mdx_ctx* c, const std::vector<int>& ids) {
   c->tables_ready = true;
   const int keep = c->cur;
   for (size_t j = 0; j < c->jobs.size(); ++j) {
+    if (!c->jobs[j] || c->jobs[j]->domain_of >= 0) continue;
     c->cur = (int)j;
     if (c->J().domain_set) upload_domain(c);
   }
@@ -379,34 +416,119 @@ void read_stats(mdx_ctx* c, mdx_stats* stats) {
   stats->kernel_ms = ms;
 }
 
-// One launch of the queue kernel over the listed jobs' history slices.
-void run_jobs(mdx_ctx* c, int n_jobs, const int* ids, const uint64_t* first, const uint64_t* n,
-              const uint64_t* seed, mdx_stats* stats) {
-  need(c->tables_ready, "physics tables not built (mdx_build_tables)");
-  set_device(c);
-  std::vector<mdx_job_dev> jd(n_jobs);
-  std::vector<uint64_t> prefix(n_jobs + 1, 0);
-  const int keep = c->cur;
+// Restores the selected job when a scope that switches it ends (also on throw).
+struct KeepCur {
+  mdx_ctx* c;
+  int keep;
+  explicit KeepCur(mdx_ctx* ctx) : c(ctx), keep(ctx->cur) {}
+  ~KeepCur() { c->cur = keep; }
+};
+
+Job& job_at(mdx_ctx* c, int id) {
+  if (id < 0 || id >= (int)c->jobs.size() || !c->jobs[id]) throw std::out_of_range("no such job");
+  return *c->jobs[id];
+}
+
+// The job whose domain a job uses (itself unless it aliases another's).
+int domain_owner(mdx_ctx* c, int id) {
+  int d = job_at(c, id).domain_of;
+  if (d < 0) return id;
+  job_at(c, d);
+  return d;
+}
+
+// Fills the device job records of one launch and validates every job.
+uint64_t fill_round(mdx_ctx* c, int n_jobs, const int* ids, const uint64_t* first, const uint64_t* n,
+                    const uint64_t* seed, mdx_job_dev* jd, uint64_t* prefix) {
+  KeepCur kc(c);
+  prefix[0] = 0;
   for (int i = 0; i < n_jobs; ++i) {
-    if (ids[i] < 0 || ids[i] >= (int)c->jobs.size() || !c->jobs[ids[i]]) throw std::out_of_range("no such job");
-    c->cur = ids[i];
+    job_at(c, ids[i]);
+    c->cur = domain_owner(c, ids[i]);
     check_materials(c);
+    const mdx_domain_dev* D = c->J().d_domain.as<mdx_domain_dev>();
+    c->cur = ids[i];
     need(c->J().source_set, "source not set (mdx_set_source)");
     if (!c->J().d_tallies.p) upload_tallies(c);
-    jd[i].D = c->J().d_domain.as<mdx_domain_dev>();
+    jd[i].D = D;
     jd[i].S = c->J().d_source.as<mdx_source_dev>();
     jd[i].TL = c->J().d_tallies.as<mdx_tallies_dev>();
     jd[i].first = first[i];
     jd[i].n = n[i];
     jd[i].seed = seed[i];
-    jd[i].job_id = (uint32_t)ids[i];
+    jd[i].stream = 0;
     prefix[i + 1] = prefix[i] + n[i];
   }
-  c->cur = keep;
-  const uint64_t total = prefix[n_jobs];
+  return prefix[n_jobs];
+}
+
+// Captures the graph of one multi-job round: copy of the pinned round buffer,
+// counter zeroing (kernels, no memset nodes) and the queue kernel, which reads
+// the job count and total from the copied header.
+void build_graph(mdx_ctx* c) {
+  if (c->graph) {
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  if (!c->h_round) hip_check(hipHostMalloc(&c->h_round, kRoundBytes, hipHostMallocDefault), "hipHostMalloc");
+  if (!c->d_round.p) c->d_round.alloc(kRoundBytes);
+  char* d = c->d_round.as<char>();
+  hipGraph_t g;
+  hip_check(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+  hip_check(hipMemcpyAsync(d, c->h_round, kRoundBytes, hipMemcpyHostToDevice, c->stream), "capture copy");
+  hip_check(mdx_launch::zero_u64(c->d_counters.as<unsigned long long>(), MDX_NCOUNTER, c->stream), "capture zero");
+  hip_check(mdx_launch::zero_u64(c->d_work.as<unsigned long long>(), 1, c->stream), "capture zero");
+  hip_check(mdx_launch::transport_queue_round(c->d_tables.as<mdx_tables_dev>(), d, c->d_work.as<unsigned long long>(),
+                                              c->d_counters.as<unsigned long long>(),
+                                              c->num_cu * c->queue_blocks_per_cu, c->refill_min, c->interact_min,
+                                              c->stream),
+            "capture kernel");
+  hip_check(hipStreamEndCapture(c->stream, &g), "end capture");
+  hip_check(hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+  (void)hipGraphDestroy(g);
+  c->graph_params[0] = c->queue_blocks_per_cu;
+  c->graph_params[1] = c->refill_min;
+  c->graph_params[2] = c->interact_min;
+}
+
+void finish_round(mdx_ctx* c, mdx_stats* stats) {
+  hip_check(hipStreamSynchronize(c->stream), "transport");
+  c->inflight = false;
+  read_stats(c, stats);
+}
+
+// One launch of the queue kernel over the listed jobs' history slices;
+// with wait = false it returns once the round is queued (mdx_wait ends it).
+void run_jobs(mdx_ctx* c, int n_jobs, const int* ids, const uint64_t* first, const uint64_t* n,
+              const uint64_t* seed, mdx_stats* stats, bool wait = true) {
+  need(c->tables_ready, "physics tables not built (mdx_build_tables)");
+  need(!c->inflight, "a round is in flight (mdx_wait first)");
+  set_device(c);
+  unsigned long long* cnt = c->d_counters.as<unsigned long long>();
+  if (c->use_graphs) {
+    need(n_jobs <= kRoundMax, "too many jobs in one graph round (max 1024)");
+    if (!c->graph || c->graph_params[0] != c->queue_blocks_per_cu || c->graph_params[1] != c->refill_min ||
+        c->graph_params[2] != c->interact_min)
+      build_graph(c);
+    char* h = static_cast<char*>(c->h_round);
+    uint64_t* hdr = reinterpret_cast<uint64_t*>(h);
+    uint64_t* prefix = reinterpret_cast<uint64_t*>(h + kRoundHeader);
+    mdx_job_dev* jd = reinterpret_cast<mdx_job_dev*>(h + kRoundHeader + kRoundPrefix);
+    const uint64_t total = fill_round(c, n_jobs, ids, first, n, seed, jd, prefix);
+    hdr[0] = (uint64_t)n_jobs;
+    hdr[1] = total;
+    hip_check(hipEventRecord(c->ev0, c->stream), "hipEventRecord");
+    hip_check(hipGraphLaunch(c->graph, c->stream), "hipGraphLaunch");
+    hip_check(hipEventRecord(c->ev1, c->stream), "hipEventRecord");
+    c->inflight = true;
+    if (wait) finish_round(c, stats);
+    return;
+  }
+  std::vector<mdx_job_dev> jd(n_jobs);
+  std::vector<uint64_t> prefix(n_jobs + 1, 0);
+  const uint64_t total = fill_round(c, n_jobs, ids, first, n, seed, jd.data(), prefix.data());
   c->d_jobs.upload(jd.data(), jd.size(), c->stream);
   c->d_prefix.upload(prefix.data(), prefix.size(), c->stream);
-  unsigned long long* cnt = c->d_counters.as<unsigned long long>();
   hip_check(mdx_launch::zero_u64(cnt, MDX_NCOUNTER, c->stream), "zero counters");
   hip_check(mdx_launch::zero_u64(c->d_work.as<unsigned long long>(), 1, c->stream), "zero work");
   hip_check(hipEventRecord(c->ev0, c->stream), "hipEventRecord");
@@ -417,8 +539,8 @@ void run_jobs(mdx_ctx* c, int n_jobs, const int* ids, const uint64_t* first, con
                                           c->refill_min, c->interact_min, c->stream),
               "queue launch");
   hip_check(hipEventRecord(c->ev1, c->stream), "hipEventRecord");
-  hip_check(hipStreamSynchronize(c->stream), "transport");
-  read_stats(c, stats);
+  c->inflight = true;
+  if (wait) finish_round(c, stats);
 }
 
 }  // namespace
@@ -448,6 +570,7 @@ int mdx_create(int device, mdx_ctx** out) {
     c->jobs.push_back(std::make_unique<Job>());
     set_device(c.get());
     hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipStreamCreateWithFlags(&c->io_stream, hipStreamNonBlocking), "hipStreamCreate");
     hip_check(hipEventCreate(&c->ev0), "hipEventCreate");
     hip_check(hipEventCreate(&c->ev1), "hipEventCreate");
     hipDeviceProp_t prop;
@@ -468,12 +591,17 @@ int mdx_destroy(mdx_ctx* c) {
     }
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    hipStream_t st = c->stream;
+    (void)hipStreamSynchronize(c->io_stream);
+    hipStream_t st = c->stream, io = c->io_stream;
     hipEvent_t e0 = c->ev0, e1 = c->ev1;
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    void* pinned = c->h_round;
     delete c;
+    if (pinned) (void)hipHostFree(pinned);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipStreamDestroy(st);
+    (void)hipStreamDestroy(io);
   });
 }
 
@@ -540,6 +668,7 @@ int mdx_set_domain(mdx_ctx* c, const double dim_space[3], int background_id) {
     }
     c->J().background_id = background_id;
     c->J().grids.clear();
+    c->J().domain_of = -1;
     c->J().domain_set = true;
     if (c->device >= 0) {
       set_device(c);
@@ -553,6 +682,7 @@ int mdx_add_grid(mdx_ctx* c, const uint8_t* ids, const int dims[3], const double
   return guard([&] {
     need(c && ids && dims && spacing && origin, "NULL argument");
     need(c->J().domain_set, "mdx_set_domain must come first");
+    need(c->J().domain_of < 0, "the selected job uses another job's domain");
     need((int)c->J().grids.size() < MDX_MAX_GRIDS, "too many grids");
     auto g = std::make_unique<GridHost>();
     size_t nv = 1;
@@ -591,6 +721,7 @@ int mdx_load_scene(mdx_ctx* c, const char* json_path) {
     c->J().dim_space[2] = ds[2];
     c->J().background_id = bg;
     c->J().grids.clear();
+    c->J().domain_of = -1;
     c->J().domain_set = true;
     for (size_t i = 0; i < grids.size(); ++i) {
       need(i < MDX_MAX_GRIDS, "too many grids");
@@ -614,18 +745,19 @@ int mdx_load_scene(mdx_ctx* c, const char* json_path) {
 int mdx_num_grids(mdx_ctx* c, int* n) {
   return guard([&] {
     need(c && n, "NULL argument");
-    *n = (int)c->J().grids.size();
+    *n = (int)c->DJ().grids.size();
   });
 }
 
 int mdx_grid_info(mdx_ctx* c, int grid, int dims[3], double spacing[3], double origin[3]) {
   return guard([&] {
     need(c, "NULL context");
-    if (grid < 0 || grid >= (int)c->J().grids.size()) throw std::out_of_range("grid index out of range");
+    Job& D = c->DJ();
+    if (grid < 0 || grid >= (int)D.grids.size()) throw std::out_of_range("grid index out of range");
     for (int k = 0; k < 3; ++k) {
-      if (dims) dims[k] = c->J().grids[grid]->dims[k];
-      if (spacing) spacing[k] = c->J().grids[grid]->spacing[k];
-      if (origin) origin[k] = c->J().grids[grid]->origin[k];
+      if (dims) dims[k] = D.grids[grid]->dims[k];
+      if (spacing) spacing[k] = D.grids[grid]->spacing[k];
+      if (origin) origin[k] = D.grids[grid]->origin[k];
     }
   });
 }
@@ -633,8 +765,9 @@ int mdx_grid_info(mdx_ctx* c, int grid, int dims[3], double spacing[3], double o
 int mdx_get_grid_ids(mdx_ctx* c, int grid, uint8_t* out) {
   return guard([&] {
     need(c && out, "NULL argument");
-    if (grid < 0 || grid >= (int)c->J().grids.size()) throw std::out_of_range("grid index out of range");
-    std::memcpy(out, c->J().grids[grid]->ids.data(), c->J().grids[grid]->ids.size());
+    Job& D = c->DJ();
+    if (grid < 0 || grid >= (int)D.grids.size()) throw std::out_of_range("grid index out of range");
+    std::memcpy(out, D.grids[grid]->ids.data(), D.grids[grid]->ids.size());
   });
 }
 
@@ -763,6 +896,7 @@ int mdx_set_tallies(mdx_ctx* c, const mdx_tally* t, int n) {
 int mdx_enable_voxel_dose(mdx_ctx* c, int enable) {
   return guard([&] {
     need(c, "NULL context");
+    need(c->J().domain_of < 0, "voxel dose belongs to the job that owns the domain");
     c->J().voxel_dose = enable != 0;
     set_device(c);
     if (c->J().domain_set) upload_domain(c);
@@ -781,12 +915,19 @@ int mdx_run(mdx_ctx* c, uint64_t n, uint64_t seed, uint64_t first, int kernel, m
   return guard([&] {
     need(c, "NULL context");
     need(kernel == MDX_KERNEL_PLAIN || kernel == MDX_KERNEL_QUEUE, "unknown kernel");
+    need(!c->inflight, "a round is in flight (mdx_wait first)");
     if (kernel == MDX_KERNEL_QUEUE) {
       int job = c->cur;
       run_jobs(c, 1, &job, &first, &n, &seed, stats);
       return;
     }
-    check_materials(c);
+    const mdx_domain_dev* D;
+    {
+      KeepCur kc(c);
+      c->cur = domain_owner(c, c->cur);
+      check_materials(c);
+      D = c->J().d_domain.as<mdx_domain_dev>();
+    }
     need(c->J().source_set, "source not set (mdx_set_source)");
     set_device(c);
     if (!c->J().d_tallies.p) upload_tallies(c);
@@ -794,7 +935,7 @@ int mdx_run(mdx_ctx* c, uint64_t n, uint64_t seed, uint64_t first, int kernel, m
     hip_check(mdx_launch::zero_u64(cnt, MDX_NCOUNTER, c->stream), "zero counters");
     hip_check(hipEventRecord(c->ev0, c->stream), "hipEventRecord");
     if (n > 0) {
-      hip_check(mdx_launch::transport_plain(c->d_tables.as<mdx_tables_dev>(), c->J().d_domain.as<mdx_domain_dev>(),
+      hip_check(mdx_launch::transport_plain(c->d_tables.as<mdx_tables_dev>(), D,
                                             c->J().d_source.as<mdx_source_dev>(), c->J().d_tallies.as<mdx_tallies_dev>(),
                                             first, n, seed, 0u, cnt, c->num_cu * 8, c->stream),
                 "transport launch");
@@ -831,6 +972,7 @@ int mdx_autotune(mdx_ctx* c, uint64_t probe_histories, int* blocks_per_cu, int* 
   return guard([&] {
     need(c, "NULL context");
     need(probe_histories > 0, "probe_histories must be positive");
+    need(!c->inflight, "a round is in flight (mdx_wait first)");
     const int job = c->cur;
     const uint64_t seed = 0x7475u;
     uint64_t first = 1ull << 40;  // a stream region no production run uses
@@ -889,6 +1031,27 @@ int mdx_job_create(mdx_ctx* c, int* job) {
   });
 }
 
+int mdx_job_alias_domain(mdx_ctx* c, int src) {
+  return guard([&] {
+    need(c, "NULL context");
+    const int owner = domain_owner(c, src);
+    need(owner != c->cur, "a job cannot alias its own domain");
+    Job& J = c->J();
+    need(J.grids.empty(), "the selected job already has grids");
+    J.domain_of = owner;
+    J.domain_set = true;
+    J.background_id = job_at(c, owner).background_id;
+    for (int k = 0; k < 3; ++k) J.dim_space[k] = job_at(c, owner).dim_space[k];
+  });
+}
+
+int mdx_set_graphs(mdx_ctx* c, int enable) {
+  return guard([&] {
+    need(c, "NULL context");
+    c->use_graphs = enable != 0;
+  });
+}
+
 int mdx_job_select(mdx_ctx* c, int job) {
   return guard([&] {
     need(c, "NULL context");
@@ -901,6 +1064,9 @@ int mdx_job_release(mdx_ctx* c, int job) {
   return guard([&] {
     need(c, "NULL context");
     if (job < 0 || job >= (int)c->jobs.size() || !c->jobs[job]) throw std::out_of_range("no such job");
+    for (size_t j = 0; j < c->jobs.size(); ++j)
+      if (c->jobs[j] && c->jobs[j]->domain_of == job)
+        throw std::invalid_argument("job " + std::to_string(j) + " uses this job's domain; release it first");
     if (c->device >= 0) {
       set_device(c);
       hip_check(hipStreamSynchronize(c->stream), "release");
@@ -935,35 +1101,81 @@ int mdx_run_jobs(mdx_ctx* c, int n_jobs, const int* jobs, const uint64_t* first,
   });
 }
 
+int mdx_mem_info(mdx_ctx* c, uint64_t* free_bytes, uint64_t* total_bytes) {
+  return guard([&] {
+    need(c && free_bytes && total_bytes, "NULL argument");
+    set_device_nowait(c);
+    size_t f = 0, t = 0;
+    hip_check(hipMemGetInfo(&f, &t), "hipMemGetInfo");
+    *free_bytes = f;
+    *total_bytes = t;
+  });
+}
+
+int mdx_host_alloc(mdx_ctx* c, uint64_t bytes, void** out) {
+  return guard([&] {
+    need(c && out, "NULL argument");
+    set_device_nowait(c);
+    *out = nullptr;
+    hip_check(hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault), "hipHostMalloc");
+  });
+}
+
+int mdx_host_free(void* p) {
+  return guard([&] {
+    if (p) hip_check(hipHostFree(p), "hipHostFree");
+  });
+}
+
+int mdx_run_jobs_async(mdx_ctx* c, int n_jobs, const int* jobs, const uint64_t* first, const uint64_t* n,
+                       const uint64_t* seed) {
+  return guard([&] {
+    need(c && n_jobs > 0 && jobs && first && n && seed, "bad argument");
+    run_jobs(c, n_jobs, jobs, first, n, seed, nullptr, false);
+  });
+}
+
+int mdx_wait(mdx_ctx* c, mdx_stats* stats) {
+  return guard([&] {
+    need(c, "NULL context");
+    need(c->inflight, "no round in flight");
+    set_device_nowait(c);
+    finish_round(c, stats);
+  });
+}
+
 int mdx_get_tally(mdx_ctx* c, int i, double* acc, double* hist, double* image) {
   return guard([&] {
     need(c, "NULL context");
     if (i < 0 || i >= (int)c->J().tallies.size()) throw std::out_of_range("tally index out of range");
-    set_device(c);
+    set_device_nowait(c);
     TallyHost& h = *c->J().tallies[i];
-    if (acc) hip_check(hipMemcpy(acc, h.acc.p, h.acc.bytes, hipMemcpyDeviceToHost), "read tally");
-    if (hist && h.hist.p) hip_check(hipMemcpy(hist, h.hist.p, h.hist.bytes, hipMemcpyDeviceToHost), "read hist");
-    if (image && h.image.p) hip_check(hipMemcpy(image, h.image.p, h.image.bytes, hipMemcpyDeviceToHost), "read image");
+    if (acc) read_back(c, acc, h.acc.p, h.acc.bytes, "read tally");
+    if (hist && h.hist.p) read_back(c, hist, h.hist.p, h.hist.bytes, "read hist");
+    if (image && h.image.p) read_back(c, image, h.image.p, h.image.bytes, "read image");
   });
 }
 
 int mdx_get_voxel_dose(mdx_ctx* c, int grid, double* ss, uint32_t* count) {
   return guard([&] {
     need(c, "NULL context");
-    if (grid < 0 || grid >= (int)c->J().grids.size()) throw std::out_of_range("grid index out of range");
-    need(c->J().voxel_dose, "voxel dose is not enabled");
-    set_device(c);
-    GridHost& g = *c->J().grids[grid];
-    if (ss) hip_check(hipMemcpy(ss, g.d_dose.p, g.d_dose.bytes, hipMemcpyDeviceToHost), "read dose");
-    if (count) hip_check(hipMemcpy(count, g.d_dose_n.p, g.d_dose_n.bytes, hipMemcpyDeviceToHost), "read dose n");
+    Job& D = c->DJ();
+    if (grid < 0 || grid >= (int)D.grids.size()) throw std::out_of_range("grid index out of range");
+    need(D.voxel_dose, "voxel dose is not enabled");
+    set_device_nowait(c);
+    GridHost& g = *D.grids[grid];
+    if (ss) read_back(c, ss, g.d_dose.p, g.d_dose.bytes, "read dose");
+    if (count) read_back(c, count, g.d_dose_n.p, g.d_dose_n.bytes, "read dose n");
   });
 }
 
 int mdx_get_background_dose(mdx_ctx* c, double out[3]) {
   return guard([&] {
     need(c && out, "NULL argument");
-    set_device(c);
-    hip_check(hipMemcpy(out, c->J().d_bg.p, 3 * sizeof(double), hipMemcpyDeviceToHost), "read bg");
+    set_device_nowait(c);
+    Job& D = c->DJ();
+    need(D.d_bg.p, "domain not uploaded");
+    read_back(c, out, D.d_bg.p, 3 * sizeof(double), "read bg");
   });
 }
 

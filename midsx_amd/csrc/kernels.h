@@ -16,6 +16,11 @@ hipError_t transport_plain(const mdx_tables_dev* T, const mdx_domain_dev* D, con
 hipError_t transport_queue(const mdx_tables_dev* T, const mdx_job_dev* jobs, const uint64_t* prefix, int n_jobs,
                            uint64_t total, unsigned long long* work, unsigned long long* counters, int grid_blocks,
                            int refill_min, int interact_min, hipStream_t st);
+// Graph form of the multi-job queue kernel: reads the job count, total,
+// prefix and job records from a device round buffer.
+hipError_t transport_queue_round(const mdx_tables_dev* T, const char* round, unsigned long long* work,
+                                 unsigned long long* counters, int grid_blocks, int refill_min, int interact_min,
+                                 hipStream_t st);
 hipError_t eval_curve(const mdx_tables_dev* T, int slot, int which, int n, const float* x, float* out,
                       hipStream_t st);
 hipError_t rita(const mdx_tables_dev* T, int slot, int n, const float* E, const float* y, float* mu,

@@ -142,7 +142,7 @@ struct mdx_tallies_dev {
 };
 
 // One job of a queue-kernel launch: its scene and its slice of histories
-// [first, first + n) with RNG streams keyed by (seed, job_id, history).
+// [first, first + n) with RNG streams keyed by (seed, stream, history).
 struct mdx_job_dev {
   const mdx_domain_dev* D;
   const mdx_source_dev* S;
@@ -150,7 +150,7 @@ struct mdx_job_dev {
   uint64_t first;
   uint64_t n;
   uint64_t seed;
-  uint32_t job_id;
+  uint32_t stream;  // 0 from mdx_run_jobs: a job's histories do not depend on its slot
   uint32_t pad;
 };
 

@@ -472,11 +472,12 @@ __device__ void tallies_step(const mdx_tallies_dev* TL, double* acc, const Step&
 #endif
 
 template <bool MULTI>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MDX_QUEUE_WAVES_PER_EU)))
-k_transport_queue(const mdx_tables_dev* __restrict__ T, const mdx_job_dev* __restrict__ jobs,
-                  const uint64_t* __restrict__ prefix, int n_jobs, uint64_t total,
-                  unsigned long long* __restrict__ work, unsigned long long* __restrict__ counters,
-                  int refill_min, int interact_min) {
+__device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __restrict__ T,
+                                                     const mdx_job_dev* __restrict__ jobs,
+                                                     const uint64_t* __restrict__ prefix, int n_jobs, uint64_t total,
+                                                     unsigned long long* __restrict__ work,
+                                                     unsigned long long* __restrict__ counters, int refill_min,
+                                                     int interact_min) {
   __shared__ double acc[MULTI ? 1 : MDX_MAX_TALLIES * kAccPerTally];
   __shared__ double lhist[MULTI ? 1 : kLdsHist];
   __shared__ unsigned long long cnt[MDX_NCOUNTER];
@@ -538,8 +539,8 @@ k_transport_queue(const mdx_tables_dev* __restrict__ T, const mdx_job_dev* __res
             S = jobs[j].S;
             TL = jobs[j].TL;
           }
-          rng.init(jobs[j].seed, jobs[j].job_id, jobs[j].first + local);
-          srng.init(jobs[j].seed, jobs[j].job_id, jobs[j].first + local);
+          rng.init(jobs[j].seed, jobs[j].stream, jobs[j].first + local);
+          srng.init(jobs[j].seed, jobs[j].stream, jobs[j].first + local);
           generate(S, rng, px, py, pz, dx, dy, dz, E);
           ++c_hist;
           nc = ni = 0;
@@ -675,6 +676,30 @@ k_transport_queue(const mdx_tables_dev* __restrict__ T, const mdx_job_dev* __res
   if (threadIdx.x < MDX_NCOUNTER && cnt[threadIdx.x]) atomicAdd(counters + threadIdx.x, cnt[threadIdx.x]);
 }
 
+template <bool MULTI>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MDX_QUEUE_WAVES_PER_EU)))
+k_transport_queue(const mdx_tables_dev* __restrict__ T, const mdx_job_dev* __restrict__ jobs,
+                  const uint64_t* __restrict__ prefix, int n_jobs, uint64_t total,
+                  unsigned long long* __restrict__ work, unsigned long long* __restrict__ counters,
+                  int refill_min, int interact_min) {
+  transport_queue_body<MULTI>(T, jobs, prefix, n_jobs, total, work, counters, refill_min, interact_min);
+}
+
+// Graph form: job count and total come from the round buffer's header
+// (uint64 n_jobs, uint64 total, uint64 prefix[1025], mdx_job_dev jobs[1024]).
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MDX_QUEUE_WAVES_PER_EU)))
+k_transport_queue_round(const mdx_tables_dev* __restrict__ T, const char* __restrict__ round,
+                        unsigned long long* __restrict__ work, unsigned long long* __restrict__ counters,
+                        int refill_min, int interact_min) {
+  const uint64_t* hdr = reinterpret_cast<const uint64_t*>(round);
+  const int n_jobs = (int)hdr[0];
+  const uint64_t total = hdr[1];
+  const uint64_t* prefix = hdr + 2;
+  const mdx_job_dev* jobs = reinterpret_cast<const mdx_job_dev*>(round + 2 * sizeof(uint64_t) + 1025 * sizeof(uint64_t));
+  if (n_jobs <= 0 || total == 0) return;
+  transport_queue_body<true>(T, jobs, prefix, n_jobs, total, work, counters, refill_min, interact_min);
+}
+
 // ------------------------------------------------------------------ inspection kernels
 __global__ void k_eval_curve(const mdx_tables_dev* T, int slot, int which, int n, const float* x, float* out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -770,6 +795,14 @@ hipError_t transport_queue(const mdx_tables_dev* T, const mdx_job_dev* jobs, con
   else
     hipLaunchKernelGGL(k_transport_queue<true>, dim3(blocks), dim3(kBlock), 0, st, T, jobs, prefix, n_jobs, total,
                        work, counters, refill_min, interact_min);
+  return hipGetLastError();
+}
+
+hipError_t transport_queue_round(const mdx_tables_dev* T, const char* round, unsigned long long* work,
+                                 unsigned long long* counters, int grid_blocks, int refill_min, int interact_min,
+                                 hipStream_t st) {
+  hipLaunchKernelGGL(k_transport_queue_round, dim3((unsigned)grid_blocks), dim3(kBlock), 0, st, T, round, work,
+                     counters, refill_min, interact_min);
   return hipGetLastError();
 }
 

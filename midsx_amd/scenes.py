@@ -73,7 +73,7 @@ C2_MATERIALS = (AIR, WATER, BONE, ADIPOSE)
 
 
 def setup_radiography(sim, shape=(256, 256, 160), voxel_cm=0.125, seed=2, voxel_dose=True, image=(256, 256),
-                      kvp=80.0):
+                      kvp=80.0, with_domain=True):
     """Radiography: air domain 30 x 30 cm laterally, phantom of seeded
     ellipsoids (water background, 3-8 cortical-bone ellipsoids of semi-axes
     0.5-3 cm, 2-5 adipose of 2-6 cm), point source 60 cm above the phantom's
@@ -81,14 +81,16 @@ def setup_radiography(sim, shape=(256, 256, 160), voxel_cm=0.125, seed=2, voxel_
     Kramers spectrum 15-80 keV in 1 keV bins, an energy-fluence image at the
     far face and per-voxel deposited energy.  The domain is extended in z so
     that it contains the source (a photon born outside the domain dies on its
-    first step in the reference)."""
+    first step in the reference).  with_domain=False sets only the source and
+    tallies (for a job that uses another job's phantom)."""
     nx, ny, nz = shape
     lx, ly, lz = nx * voxel_cm, ny * voxel_cm, nz * voxel_cm
     dim = (30.0, 30.0, lz + 60.0 + 1.0)
     z0 = 0.5  # phantom from z0 to z0 + lz; image plane just below it
-    sim.set_domain(dim, AIR)
     ox, oy = (30.0 - lx) / 2, (30.0 - ly) / 2
-    sim.add_grid(ellipsoid_phantom(shape, seed, voxel_cm=voxel_cm), (voxel_cm,) * 3, (ox, oy, z0))
+    if with_domain:
+        sim.set_domain(dim, AIR)
+        sim.add_grid(ellipsoid_phantom(shape, seed, voxel_cm=voxel_cm), (voxel_cm,) * 3, (ox, oy, z0))
     src = (15.0, 15.0, z0 + lz + 60.0)
     e, w = kramers(kvp)
     sim.set_source(direction=DIR_RECT_ISOTROPIC, position=src, a=(ox, oy, z0 + lz), b=(lx, 0.0, 0.0),
@@ -99,6 +101,77 @@ def setup_radiography(sim, shape=(256, 256, 160), voxel_cm=0.125, seed=2, voxel_
         tallies.append(dict(kind=TALLY_IMAGE, p0=(ox, oy, z0 - 0.25), p1=(0.0, ly, 0.0), p2=(lx, 0.0, 0.0),
                             nx=image[1], ny=image[0]))
     sim.set_tallies(tallies)
-    if voxel_dose:
+    if voxel_dose and with_domain:
         sim.enable_voxel_dose(True)
     return dict(shape=shape, voxel_cm=voxel_cm, origin=(ox, oy, z0), source=src)
+
+
+# --------------------------------------------------------------------------- dose jobs
+DOSE_MATERIALS = (AIR, WATER, LUNG, ADIPOSE, BONE)
+DOSE_SHAPES = (64, 128, 256, 512)
+DOSE_KVP = (60.0, 80.0, 100.0, 120.0)
+
+
+def dose_phantom(shape, voxel_cm, seed):
+    """Water phantom with seeded ellipsoids of lung (1-2, semi-axes 10-25 % of
+    the phantom's edge), adipose (2-5, 8-20 %) and cortical bone (3-8, 2-10 %)."""
+    edge = min(shape) * voxel_cm
+    specs = ((LUNG, (1, 2), (0.10 * edge, 0.25 * edge)), (ADIPOSE, (2, 5), (0.08 * edge, 0.20 * edge)),
+             (BONE, (3, 8), (0.02 * edge, 0.10 * edge)))
+    return ellipsoid_phantom(shape, seed, specs=specs, voxel_cm=voxel_cm)
+
+
+def draw_dose_jobs(n, seed, shapes=DOSE_SHAPES, h_lo=1e6, h_hi=1e8, max_edge_cm=35.0):
+    """n dose-job descriptions: cubic grid edge from shapes, voxel 0.5-2 mm
+    (capped so that the phantom's edge stays within max_edge_cm), source
+    monoenergetic 20-120 keV or Kramers with kVp from DOSE_KVP (one each half
+    of the time), histories log-uniform in [h_lo, h_hi], a 128 x 128 image."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        m = int(rng.choice(shapes))
+        vox = min(float(rng.uniform(0.05, 0.2)), max_edge_cm / m)
+        if rng.uniform() < 0.5:
+            src = dict(energy=float(rng.uniform(20e3, 120e3)))
+        else:
+            src = dict(kvp=float(rng.choice(DOSE_KVP)))
+        h = int(np.exp(rng.uniform(np.log(h_lo), np.log(h_hi))))
+        out.append(dict(shape=(m, m, m), voxel_cm=vox, phantom_seed=int(rng.integers(1 << 30)), histories=h,
+                        seed=int(rng.integers(1, 1 << 62)), image=(128, 128), **src))
+    return out
+
+
+def setup_dose_job(sim, job, ids=None):
+    """A dose job: air domain 2 cm wider than the phantom on each side, point
+    source 20 cm above the phantom's top face, cone beam over that face,
+    energy-fluence image just below the phantom and per-voxel deposited
+    energy.  ids: the phantom grid if already built."""
+    nx, ny, nz = job["shape"]
+    v = job["voxel_cm"]
+    lx, ly, lz = nx * v, ny * v, nz * v
+    ssd = 20.0
+    z0 = 0.5
+    sim.set_domain((lx + 4.0, ly + 4.0, lz + ssd + 1.5), AIR)
+    if ids is None:
+        ids = dose_phantom(job["shape"], v, job["phantom_seed"])
+    sim.add_grid(ids, (v, v, v), (2.0, 2.0, z0))
+    src = (2.0 + lx / 2, 2.0 + ly / 2, z0 + lz + ssd)
+    kw = dict(direction=DIR_RECT_ISOTROPIC, position=src, a=(2.0, 2.0, z0 + lz), b=(lx, 0.0, 0.0), c=(0.0, ly, 0.0))
+    if "kvp" in job:
+        e, w = kramers(job["kvp"])
+        kw["spectrum"] = (e, w)
+    else:
+        kw["energy"] = job["energy"]
+    sim.set_source(**kw)
+    ny_i, nx_i = job["image"]
+    sim.set_tallies([dict(kind=TALLY_IMAGE, p0=(2.0, 2.0, z0 - 0.25), p1=(0.0, ly, 0.0), p2=(lx, 0.0, 0.0),
+                          nx=nx_i, ny=ny_i)])
+    sim.enable_voxel_dose(True)
+
+
+def dose_job_bytes(job):
+    """Device bytes a dose job holds: material IDs, per-voxel (sum, sum^2,
+    count), the image's (sum, sum^2, count) per pixel, plus small records."""
+    nv = int(np.prod(job["shape"]))
+    ny_i, nx_i = job["image"]
+    return nv * (1 + 16 + 4) + 24 * nx_i * ny_i + (1 << 16)

@@ -1,0 +1,256 @@
+"""Continuous-batching request loop for dose jobs on one device.
+
+Jobs (a phantom, a source, tallies and a history count) arrive over time.
+The loop keeps the physics tables resident and shared, admits waiting jobs in
+arrival order while their device footprint fits the memory budget, and runs
+rounds: one multi-job launch of the queue kernel (a replayed hipGraph) in
+which every admitted job runs its next slice of histories.  While a round
+runs, the results of the jobs that finished in the round before are copied
+back on a second stream; the memory of a finished job is freed after the
+round, and jobs that arrived meanwhile are admitted before the next one.
+
+A job's histories are keyed by its seed and history index, so its tallies do
+not depend on the round structure nor on the other jobs in the mix (the FP64
+sums differ only in their order of summation).
+
+Baseline: `run_plain` serves the same stream first-in first-out with the
+plain kernel, one launch per job (one simulation per launch).
+"""
+import time
+from collections import deque
+
+import numpy as np
+
+from . import KERNEL_PLAIN
+from . import scenes
+
+
+class JobResult:
+    """What a finished job hands back: its voxel dose (sum, sum^2 per voxel,
+    optionally reduced by the caller's `on_result`), its image and timing."""
+
+    __slots__ = ("index", "arrival", "admitted", "finished", "histories", "summary")
+
+    def __init__(self, index, arrival):
+        self.index = index
+        self.arrival = arrival
+        self.admitted = None
+        self.finished = None
+        self.histories = 0
+        self.summary = None
+
+    @property
+    def latency(self):
+        return self.finished - self.arrival
+
+
+def default_summary(job, dose_sum, dose_sum2, image):
+    """Total deposited energy, the largest relative standard error over the
+    ROI (voxels holding at least 10 % of the peak dose), and the image's
+    total energy fluence."""
+    n = job["histories"]
+    mean = dose_sum / n
+    var = np.maximum(dose_sum2 / n - mean * mean, 0.0) / max(n - 1, 1)
+    roi = mean >= 0.1 * mean.max() if mean.max() > 0 else np.zeros(mean.shape, bool)
+    rse = np.sqrt(var[roi]) / mean[roi] if roi.any() else np.zeros(1)
+    return dict(energy=float(dose_sum.sum()), roi_voxels=int(roi.sum()), roi_max_rse=float(rse.max()),
+                image_energy=float(image[..., 0].sum()))
+
+
+class _Active:
+    __slots__ = ("job", "res", "slot", "done", "bytes")
+
+    def __init__(self, job, res, slot, nbytes):
+        self.job = job
+        self.res = res
+        self.slot = slot
+        self.done = 0
+        self.bytes = nbytes
+
+
+class Server:
+    """Continuous batching over one Context whose tables cover every job's
+    materials.
+
+    budget_bytes: device memory the resident jobs may hold (default: half of
+    the free memory at construction).  round_histories: histories per round,
+    shared equally by the admitted jobs (at least min_slice each, never more
+    than a job has left).  phantoms: callable job -> uint8 grid (default:
+    `scenes.dose_phantom`), so that a client can hand over its own data."""
+
+    def __init__(self, ctx, budget_bytes=None, round_histories=30_000_000, min_slice=100_000, max_active=512,
+                 graphs=True, phantoms=None, on_result=default_summary):
+        self.ctx = ctx
+        if budget_bytes is None:
+            free, _ = ctx.mem_info()
+            budget_bytes = free // 2
+        self.budget = int(budget_bytes)
+        self.round_histories = int(round_histories)
+        self.min_slice = int(min_slice)
+        self.max_active = min(int(max_active), 1024)
+        self.graphs = graphs
+        self.phantoms = phantoms or (lambda j: scenes.dose_phantom(j["shape"], j["voxel_cm"], j["phantom_seed"]))
+        self.on_result = on_result
+        self._buf = {}
+        self.rounds = 0
+        self.kernel_ms = 0.0
+        self.counters = {}
+
+    # ------------------------------------------------------------------ helpers
+    def _host_buf(self, n):
+        b = self._buf.get(n)
+        if b is None:
+            try:
+                b = self.ctx.pinned(2 * n)
+            except Exception:
+                b = np.empty(2 * n)
+            self._buf[n] = b
+        return b
+
+    def _admit(self, job, res, now):
+        ctx = self.ctx
+        slot = ctx.job_create()
+        scenes.setup_dose_job(ctx, job, ids=self.phantoms(job))
+        res.admitted = now
+        return _Active(job, res, slot, ctx.job_device_bytes(slot))
+
+    def _read(self, a):
+        """Copies a finished job's results back (overlaps a round in flight)."""
+        ctx = self.ctx
+        ctx.job_select(a.slot)
+        nv = int(np.prod(a.job["shape"]))
+        s, s2, _ = ctx.voxel_dose(0, out=self._host_buf(nv), counts=False)
+        image = ctx.tally(0)["image"]
+        a.res.summary = self.on_result(a.job, s, s2, image) if self.on_result else None
+        a.res.histories = a.done
+
+    def _plan(self, active):
+        """Histories of each admitted job in the next round: equal shares of
+        round_histories, the share of a job that has fewer left passed on to
+        the others."""
+        left = np.array([a.job["histories"] - a.done for a in active], dtype=np.int64)
+        take = np.zeros_like(left)
+        budget = max(self.round_histories, self.min_slice * len(active))
+        open_ = left > 0
+        while budget > 0 and open_.any():
+            share = max(self.min_slice, budget // int(open_.sum()))
+            add = np.where(open_, np.minimum(left - take, share), 0)
+            take += add
+            budget -= int(add.sum())
+            open_ = (left - take) > 0
+            if share == self.min_slice and budget <= 0:
+                break
+        return take
+
+    # ------------------------------------------------------------------ loop
+    def run(self, jobs, arrivals=None):
+        """Serves jobs[i] arriving at arrivals[i] seconds after the start
+        (all at 0 by default); returns the JobResults in job order."""
+        ctx = self.ctx
+        ctx.set_graphs(self.graphs)
+        n = len(jobs)
+        arrivals = np.zeros(n) if arrivals is None else np.asarray(arrivals, dtype=float)
+        for j in jobs:
+            if scenes.dose_job_bytes(j) > self.budget:
+                raise ValueError("a job's footprint exceeds the memory budget")
+        order = deque(sorted(range(n), key=lambda i: arrivals[i]))
+        results = [JobResult(i, float(arrivals[i])) for i in range(n)]
+        waiting = deque()
+        active = []
+        resident = 0
+        to_read = []  # finished in the last round, results not yet copied
+        t0 = time.perf_counter()
+        clock = lambda: time.perf_counter() - t0
+        while order or waiting or active or to_read:
+            now = clock()
+            while order and arrivals[order[0]] <= now:
+                waiting.append(order.popleft())
+            while waiting and len(active) < self.max_active:
+                i = waiting[0]
+                need = scenes.dose_job_bytes(jobs[i])
+                if resident + need > self.budget and (active or to_read):
+                    break
+                waiting.popleft()
+                a = self._admit(jobs[i], results[i], clock())
+                resident += a.bytes
+                active.append(a)
+            if not active and not to_read:
+                if order:
+                    time.sleep(max(0.0, min(arrivals[order[0]] - clock(), 0.05)))
+                continue
+            launched = False
+            if active:
+                take = self._plan(active)
+                ctx.run_jobs_async([a.slot for a in active], take.tolist(), [a.done for a in active],
+                                   [a.job["seed"] for a in active])
+                launched = True
+                for a, t in zip(active, take):
+                    a.done += int(t)
+            for a in to_read:  # beside the round in flight
+                self._read(a)
+                a.res.finished = clock()
+            if launched:
+                st = ctx.wait()
+                self.rounds += 1
+                self.kernel_ms += st["kernel_ms"]
+                for k, v in st.items():
+                    if k != "kernel_ms":
+                        self.counters[k] = self.counters.get(k, 0) + v
+            for a in to_read:
+                ctx.job_release(a.slot)
+                resident -= a.bytes
+            to_read = [a for a in active if a.done >= a.job["histories"]]
+            active = [a for a in active if a.done < a.job["histories"]]
+        self.makespan = clock()
+        return results
+
+
+def run_plain(ctx, jobs, arrivals=None, phantoms=None, on_result=default_summary):
+    """First-in first-out service with the plain kernel: each job in turn is
+    uploaded, run in one launch, read back and freed."""
+    phantoms = phantoms or (lambda j: scenes.dose_phantom(j["shape"], j["voxel_cm"], j["phantom_seed"]))
+    n = len(jobs)
+    arrivals = np.zeros(n) if arrivals is None else np.asarray(arrivals, dtype=float)
+    results = [JobResult(i, float(arrivals[i])) for i in range(n)]
+    bufs = {}
+    counters = {}
+    kernel_ms = 0.0
+    t0 = time.perf_counter()
+    for i in sorted(range(n), key=lambda k: arrivals[k]):
+        wait = arrivals[i] - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        j = jobs[i]
+        slot = ctx.job_create()
+        results[i].admitted = time.perf_counter() - t0
+        scenes.setup_dose_job(ctx, j, ids=phantoms(j))
+        st = ctx.run(j["histories"], seed=j["seed"], first=0, kernel=KERNEL_PLAIN)
+        kernel_ms += st["kernel_ms"]
+        for k, v in st.items():
+            if k != "kernel_ms":
+                counters[k] = counters.get(k, 0) + v
+        nv = int(np.prod(j["shape"]))
+        if nv not in bufs:
+            try:
+                bufs[nv] = ctx.pinned(2 * nv)
+            except Exception:
+                bufs[nv] = np.empty(2 * nv)
+        s, s2, _ = ctx.voxel_dose(0, out=bufs[nv], counts=False)
+        image = ctx.tally(0)["image"]
+        results[i].summary = on_result(j, s, s2, image) if on_result else None
+        results[i].histories = j["histories"]
+        results[i].finished = time.perf_counter() - t0
+        ctx.job_release(slot)
+    makespan = time.perf_counter() - t0
+    return results, dict(makespan=makespan, kernel_ms=kernel_ms, counters=counters)
+
+
+def poisson_arrivals(n, rate, seed):
+    """Arrival times (s) of n jobs of a Poisson process of the given rate (1/s)."""
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate, n)) - 0.0
+
+
+def latency_percentiles(results):
+    lat = np.array([r.latency for r in results])
+    return float(np.percentile(lat, 50)), float(np.percentile(lat, 99))

@@ -125,3 +125,82 @@ def test_job_footprint_and_release():
     with pytest.raises(M.MdxError):
         ctx.job_select(j)
     ctx.close()
+
+
+def play_rounds(ctx, jobs, rounds):
+    for jj in jobs:
+        ctx.job_select(jj)
+        ctx.reset_tallies()
+    counts = []
+    for r in rounds:
+        st = ctx.run_jobs(*r)
+        counts.append({k: v for k, v in st.items() if k != "kernel_ms"})
+    return [snapshot(ctx, j) for j in jobs], counts
+
+
+def test_graph_rounds_match_eager():
+    """Rounds replayed from one captured hipGraph (round-buffer copy, counter
+    zeroing kernels, queue kernel) give the eager launches' results; eager
+    work and synchronisations run between replays, and a change of queue
+    parameters forces a new capture."""
+    ctx = M.Context(0)
+    ctx.build_tables(list(set(scenes.C1_MATERIALS) | set(scenes.C2_MATERIALS)))
+    jobs = make_jobs(ctx)
+    rounds = [(jobs, [100000, 60000, 30000], [0, 0, 0], [5, 6, 7]),
+              ([jobs[1]], [50000], [60000], [6]),
+              ([jobs[2], jobs[0]], [20000, 40000], [30000, 100000], [7, 5])]
+    eager, ce = play_rounds(ctx, jobs, rounds)
+    ctx.set_graphs(True)
+    for rep in range(4):
+        if rep == 2:
+            ctx.set_queue_params(4, 4, 8)
+        g, cg = play_rounds(ctx, jobs, rounds)
+        assert cg == ce, rep  # integer event counters: exact
+        for j, a, m in zip(jobs, eager, g):
+            assert np.count_nonzero(a) > 10
+            assert np.allclose(m, a, rtol=1e-12, atol=1e-9), (rep, j, np.max(np.abs(m - a)))
+        # eager work between replays
+        ctx.job_select(jobs[0])
+        ctx.interpolate(scenes.WATER, M.XS_TOTAL, np.array([3e4], np.float32))
+    ctx.set_graphs(False)
+    again, ca = play_rounds(ctx, jobs, rounds)
+    assert ca == ce
+    for a, m in zip(eager, again):
+        assert np.allclose(m, a, rtol=1e-12, atol=1e-9)
+    ctx.close()
+
+
+def test_alias_domain_matches_own_copy():
+    """A job that aliases another job's phantom tallies what a job with its
+    own copy of the phantom tallies; the shared per-voxel dose holds both
+    jobs' deposits."""
+    ctx = M.Context(0)
+    ctx.build_tables(list(scenes.C2_MATERIALS))
+    kw = dict(shape=(32, 32, 24), voxel_cm=0.5, seed=7, image=(16, 16))
+    scenes.setup_radiography(ctx, **kw)  # job 0 owns the phantom
+    own = ctx.job_create()
+    scenes.setup_radiography(ctx, **kw)
+    alias = ctx.job_create()
+    ctx.job_alias_domain(0)  # job 0's phantom and dose, its own source and tallies
+    scenes.setup_radiography(ctx, **kw, with_domain=False)
+    for j in (0, own, alias):
+        ctx.job_select(j)
+        ctx.reset_tallies()
+    ctx.run_jobs([0], [100000], [0], [41])
+    ctx.job_select(0)
+    d0 = ctx.voxel_dose(0)[0].copy()
+    ctx.run_jobs([own, alias], [80000, 80000], [0, 0], [42, 42])
+    ctx.job_select(own)
+    d_own = ctx.voxel_dose(0)[0]
+    im_own = ctx.tally(0)["image"]
+    ctx.job_select(alias)
+    im_alias = ctx.tally(0)["image"]
+    assert np.count_nonzero(im_own) > 10
+    assert np.allclose(im_alias, im_own, rtol=1e-12, atol=1e-9)
+    d_shared = ctx.voxel_dose(0)[0]  # job 0's grid, read through the alias
+    assert np.allclose(d_shared, d0 + d_own, rtol=1e-9, atol=1e-6)
+    with pytest.raises(M.MdxError):
+        ctx.job_release(0)
+    ctx.job_release(alias)
+    ctx.job_release(0)
+    ctx.close()

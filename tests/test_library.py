@@ -101,3 +101,35 @@ def test_grid_layout_x_fastest():
     back = c.grid_ids(0)
     assert np.array_equal(back, ids)
     c.close()
+
+
+def test_job_alias_domain_host():
+    """A job aliasing another's domain sees its grids; it cannot add its own."""
+    c = M.Context(-1)
+    c.set_domain((4.0, 4.0, 4.0), 3)
+    ids = np.arange(2 * 3 * 4, dtype=np.uint8).reshape(2, 3, 4) % 3 + 48
+    c.add_grid(ids, (0.5, 0.5, 0.5), (1.0, 1.0, 1.0))
+    b = c.job_create()
+    assert c.num_grids() == 0
+    c.job_alias_domain(0)
+    assert c.num_grids() == 1
+    assert np.array_equal(c.grid_ids(0), ids)
+    with pytest.raises(M.MdxError):
+        c.add_grid(ids, (0.5, 0.5, 0.5), (1.0, 1.0, 1.0))
+    with pytest.raises(M.MdxError):
+        c.enable_voxel_dose(True)
+    d = c.job_create()
+    with pytest.raises(M.MdxError):
+        c.job_alias_domain(d)  # itself
+    c.job_alias_domain(b)  # alias of an alias resolves to the owner
+    assert c.num_grids() == 1
+    c.job_select(b)
+    c.set_domain((2.0, 2.0, 2.0), 3)  # a fresh domain of its own
+    assert c.num_grids() == 0
+    c.job_select(d)
+    assert c.num_grids() == 1
+    with pytest.raises(M.MdxError):
+        c.job_release(0)  # d still uses it
+    c.job_release(d)
+    c.job_release(0)
+    c.close()

@@ -1,0 +1,141 @@
+"""Host logic of the continuous-batching loop (no device): a stand-in
+context records every round, so the tests check that each job runs exactly
+its histories as consecutive slices with its own seed, that admission keeps
+the resident footprint within the budget and follows arrival order, and
+that results come back only after a job's last slice."""
+import numpy as np
+
+from midsx_amd import scenes
+from midsx_amd.serve import Server, poisson_arrivals
+
+
+class FakeCtx:
+    def __init__(self):
+        self.slots = {}
+        self.cur = 0
+        self.next = 1
+        self.rounds = []
+        self.inflight = None
+        self.resident = 0
+        self.peak = 0
+        self.bytes = {}
+        self.admit_order = []
+
+    # calls the server makes
+    def set_graphs(self, on):
+        pass
+
+    def mem_info(self):
+        return 1 << 40, 1 << 41
+
+    def pinned(self, n):
+        return np.empty(n)
+
+    def job_create(self):
+        s = self.next
+        self.next += 1
+        self.slots[s] = {}
+        self.cur = s
+        return s
+
+    def job_select(self, s):
+        assert s in self.slots
+        self.cur = s
+
+    def set_domain(self, dim, bg):
+        self.slots[self.cur]["domain"] = dim
+
+    def add_grid(self, ids, spacing, origin):
+        self.slots[self.cur]["shape"] = ids.shape
+        self.bytes[self.cur] = ids.size * 21 + 24 * 128 * 128 + (1 << 16)
+        self.resident += self.bytes[self.cur]
+        self.peak = max(self.peak, self.resident)
+
+    def set_source(self, **kw):
+        pass
+
+    def set_tallies(self, t):
+        pass
+
+    def enable_voxel_dose(self, on):
+        pass
+
+    def job_device_bytes(self, s):
+        return self.bytes[s]
+
+    def run_jobs_async(self, slots, n, first, seeds):
+        assert self.inflight is None
+        self.inflight = list(zip(slots, n, first, seeds))
+
+    def wait(self):
+        self.rounds.append(self.inflight)
+        self.inflight = None
+        return {"histories": 0, "kernel_ms": 1.0}
+
+    def voxel_dose(self, g, out=None, counts=True):
+        n = int(np.prod(self.slots[self.cur]["shape"]))
+        return np.ones(n), np.ones(n), None
+
+    def tally(self, i):
+        return {"image": np.zeros((128, 128, 3))}
+
+    def job_release(self, s):
+        self.resident -= self.bytes.pop(s)
+        del self.slots[s]
+
+
+def small_jobs(n, seed):
+    jobs = scenes.draw_dose_jobs(n, seed, shapes=(8, 12, 16), h_lo=1e4, h_hi=3e5)
+    return jobs
+
+
+def test_every_job_runs_its_histories_once():
+    ctx = FakeCtx()
+    jobs = small_jobs(30, 1)
+    srv = Server(ctx, budget_bytes=10 ** 9, round_histories=200_000, min_slice=5_000, on_result=None)
+    res = srv.run(jobs, poisson_arrivals(len(jobs), 200.0, 2))
+    per = {}
+    slot_job = {}
+    for r in ctx.rounds:
+        for slot, n, first, seed in r:
+            per.setdefault((slot, seed), []).append((first, n))
+    assert len(per) == len(jobs)  # one slot per job, seeds distinct
+    seeds = {j["seed"]: j for j in jobs}
+    for (slot, seed), sl in per.items():
+        j = seeds[seed]
+        pos = 0
+        for first, n in sl:
+            assert first == pos and n > 0
+            pos += n
+        assert pos == j["histories"]
+    for r in res:
+        assert r.finished >= r.admitted >= r.arrival - 1e-9
+        assert r.histories == jobs[r.index]["histories"]
+    assert srv.rounds == len(ctx.rounds)
+
+
+def test_admission_respects_budget_and_order():
+    ctx = FakeCtx()
+    jobs = small_jobs(20, 3)
+    budget = max(scenes.dose_job_bytes(j) for j in jobs) * 2
+    srv = Server(ctx, budget_bytes=budget, round_histories=100_000, min_slice=5_000, on_result=None)
+    res = srv.run(jobs)  # all arrive at once
+    assert ctx.peak <= budget
+    adm = sorted(range(len(jobs)), key=lambda i: (res[i].admitted, i))
+    assert adm == sorted(adm)  # first come, first admitted (ties in index order)
+    assert ctx.resident == 0
+
+
+def test_round_plan_shares_equally():
+    ctx = FakeCtx()
+    srv = Server(ctx, budget_bytes=10 ** 9, round_histories=1000, min_slice=10, on_result=None)
+
+    class A:
+        def __init__(self, h, done):
+            self.job = {"histories": h}
+            self.done = done
+
+    take = srv._plan([A(10_000, 0), A(50, 0), A(10_000, 9_990), A(10_000, 0)])
+    assert take[1] == 50 and take[2] == 10
+    assert take[0] == take[3] and take.sum() >= 1000
+    assert take.sum() <= 1000 + 4 * 10
