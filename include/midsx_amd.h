@@ -213,6 +213,17 @@ int mdx_tables_device_bytes(mdx_ctx* ctx, uint64_t* bytes);
  * its job slot, so give different jobs different seeds.  Blocking. */
 int mdx_run_jobs(mdx_ctx* ctx, int n_jobs, const int* jobs, const uint64_t* first, const uint64_t* n,
                  const uint64_t* seed, mdx_stats* stats);
+/* Background free flights for the selected job's queue-kernel runs (default
+ * on).  Through the background material outside every grid the photon flies
+ * with that material's own cross section instead of the global majorant
+ * (physics_engine.cpp:38-44 samples every step with the majorant), up to the
+ * next grid's face, where majorant steps resume, or out of the domain, with
+ * an overshoot drawn at the majorant as the reference's last step has.  The
+ * distribution of real interactions, tally crossings and escapes is the
+ * same; only the number of rejected steps changes.  Taken only when the job
+ * has no volume (AACuboid) tally, whose per-segment counts (volume_tally.cpp:
+ * 5-27) depend on where steps start. */
+int mdx_set_flights(mdx_ctx* ctx, int enable);
 /* Run mdx_run_jobs rounds as a replayed hipGraph (round buffer copy, counter
  * zeroing, queue kernel), at most 1024 jobs per round. */
 int mdx_set_graphs(mdx_ctx* ctx, int enable);
@@ -239,6 +250,15 @@ int mdx_get_tally(mdx_ctx* ctx, int tally, double* acc, double* hist, double* im
  * (sum, sum^2) interleaved and the deposit count; either may be NULL. */
 int mdx_get_voxel_dose(mdx_ctx* ctx, int grid, double* sum_sumsq, uint32_t* count);
 /* ComputationalDomain background voxel dose: sum, sum^2, count. */
+/* The voxel dose of a grid as FP32 pairs (mean deposit per history S/N,
+ * relative standard error of that mean) per voxel, x fastest: half the
+ * bytes of the FP64 sums, computed on the device.  n_histories = N, the
+ * histories the sums hold.  Runs beside a round in flight. */
+int mdx_get_voxel_dose_f32(mdx_ctx* ctx, int grid, uint64_t n_histories, float* out);
+/* Queues that FP32 conversion on the transport stream (behind a round in
+ * flight, before the next), so that a later mdx_get_voxel_dose_f32 with the
+ * same grid and N only copies, beside the next round. */
+int mdx_stage_voxel_dose_f32(mdx_ctx* ctx, int grid, uint64_t n_histories);
 int mdx_get_background_dose(mdx_ctx* ctx, double out[3]);
 
 /* MaterialData::interpolate*CrossSection / interpolateIncoherentScattering-

@@ -42,6 +42,7 @@ EXPORTS = [
     "mdx_tables_device_bytes", "mdx_run_jobs", "mdx_set_queue_params", "mdx_get_queue_params",
     "mdx_autotune", "mdx_job_alias_domain", "mdx_set_graphs",
     "mdx_run_jobs_async", "mdx_wait", "mdx_mem_info", "mdx_host_alloc", "mdx_host_free",
+    "mdx_set_flights", "mdx_get_voxel_dose_f32", "mdx_stage_voxel_dose_f32",
 ]
 
 
@@ -355,6 +356,10 @@ class Context:
         """The selected job uses job src's domain (grids, voxel dose)."""
         self._call(self.lib.mdx_job_alias_domain, self._h, ctypes.c_int(src))
 
+    def set_flights(self, on=True):
+        """Background free flights for the selected job (queue kernel)."""
+        self._call(self.lib.mdx_set_flights, self._h, ctypes.c_int(1 if on else 0))
+
     def set_graphs(self, on=True):
         self._call(self.lib.mdx_set_graphs, self._h, ctypes.c_int(1 if on else 0))
 
@@ -428,6 +433,23 @@ class Context:
         self._call(self.lib.mdx_get_voxel_dose, self._h, ctypes.c_int(g), _ptr(ss, ctypes.c_double),
                    None if cnt is None else _ptr(cnt, ctypes.c_uint32))
         return ss[0::2], ss[1::2], cnt
+
+    def voxel_dose_f32(self, g, n_histories, out=None):
+        """(mean deposit per history, relative standard error) per voxel as
+        float32 arrays (x fastest), computed on the device.  out: a float32
+        buffer of 2 * voxels."""
+        dims, _, _ = self.grid_info(g)
+        nv = dims[0] * dims[1] * dims[2]
+        buf = np.empty(2 * nv, np.float32) if out is None else out[:2 * nv]
+        assert buf.dtype == np.float32 and buf.flags.c_contiguous and buf.size == 2 * nv
+        self._call(self.lib.mdx_get_voxel_dose_f32, self._h, ctypes.c_int(g), ctypes.c_uint64(n_histories),
+                   _ptr(buf, ctypes.c_float))
+        return buf[0::2], buf[1::2]
+
+    def stage_voxel_dose_f32(self, g, n_histories):
+        """Queues the FP32 conversion behind the running round; the next
+        voxel_dose_f32(g, n_histories) then only copies."""
+        self._call(self.lib.mdx_stage_voxel_dose_f32, self._h, ctypes.c_int(g), ctypes.c_uint64(n_histories))
 
     def mem_info(self):
         f, t = ctypes.c_uint64(), ctypes.c_uint64()

@@ -36,28 +36,81 @@ void hip_check(hipError_t e, const char* what) {
   throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// Device allocation owned by the context.
+// Device blocks kept for reuse by exact size: jobs of the same shapes come
+// and go, and a block a finished job gave back serves the next one without a
+// hipFree (which waits for the device) and a hipMalloc.
+struct Pool {
+  std::multimap<size_t, void*> blocks;
+  size_t cached = 0, limit = 0;  // bytes held for reuse, and their cap
+  Pool() = default;
+  Pool(const Pool&) = delete;
+  Pool& operator=(const Pool&) = delete;
+  ~Pool() { trim(); }
+  void* get(size_t n) {
+    auto it = blocks.find(n);
+    if (it == blocks.end()) return nullptr;
+    void* p = it->second;
+    blocks.erase(it);
+    cached -= n;
+    return p;
+  }
+  void put(void* p, size_t n) {
+    if (cached + n > limit) {
+      (void)hipFree(p);
+      return;
+    }
+    blocks.emplace(n, p);
+    cached += n;
+  }
+  void trim() {
+    for (auto& b : blocks) (void)hipFree(b.second);
+    blocks.clear();
+    cached = 0;
+  }
+};
+
+// Device allocation owned by the context (from its pool when one is set).
 struct DBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  Pool* pool = nullptr;
   DBuf() = default;
+  explicit DBuf(Pool* pl) : pool(pl) {}
   DBuf(const DBuf&) = delete;
   DBuf& operator=(const DBuf&) = delete;
   ~DBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      if (pool) pool->put(p, bytes);
+      else (void)hipFree(p);
+    }
     p = nullptr;
     bytes = 0;
   }
   void alloc(size_t n) {
     release();
     if (n == 0) return;
-    hip_check(hipMalloc(&p, n), "hipMalloc");
+    if (pool && (p = pool->get(n))) {
+      bytes = n;
+      return;
+    }
+    hipError_t e = hipMalloc(&p, n);
+    if (e == hipErrorOutOfMemory && pool && pool->cached) {
+      (void)hipGetLastError();
+      pool->trim();
+      e = hipMalloc(&p, n);
+    }
+    if (e != hipSuccess) p = nullptr;
+    hip_check(e, "hipMalloc");
     bytes = n;
+  }
+  // Keeps an allocation of the same size (no free, which may wait for the device).
+  void ensure(size_t n) {
+    if (n != bytes || (n && !p)) alloc(n);
   }
   template <class T>
   void upload(const T* src, size_t count, hipStream_t st) {
-    alloc(count * sizeof(T));
+    ensure(count * sizeof(T));
     if (count) hip_check(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, st), "upload");
   }
   template <class T>
@@ -65,6 +118,24 @@ struct DBuf {
     return static_cast<T*>(p);
   }
 };
+
+// The material IDs present in a grid, eight voxels at a time (phantoms hold
+// long runs of one material, so most words repeat the one before).
+void scan_ids(const uint8_t* ids, size_t n, bool* present) {
+  std::fill(present, present + 256, false);
+  size_t i = 0;
+  uint64_t last = 0;
+  bool first = true;
+  for (; i + 8 <= n; i += 8) {
+    uint64_t w;
+    std::memcpy(&w, ids + i, 8);
+    if (!first && w == last) continue;
+    first = false;
+    last = w;
+    for (int b = 0; b < 8; ++b) present[(w >> (8 * b)) & 255u] = true;
+  }
+  for (; i < n; ++i) present[ids[i]] = true;
+}
 
 std::string default_data_dir() {
   if (const char* e = std::getenv("MIDSX_AMD_DATA")) return e;
@@ -79,15 +150,19 @@ std::string default_data_dir() {
 }
 
 struct GridHost {
-  std::vector<uint8_t> ids;
+  std::vector<uint8_t> ids;  // host copy (host-only contexts and NIfTI scenes)
   int dims[3];
   double spacing[3], origin[3];
+  bool present[256] = {false};  // material IDs the grid holds
   DBuf d_ids, d_dose, d_dose_n;
+  explicit GridHost(Pool* pl) : d_ids(pl), d_dose(pl), d_dose_n(pl) {}
+  size_t voxels() const { return (size_t)dims[0] * dims[1] * dims[2]; }
 };
 
 struct TallyHost {
   mdx_tally t;
   DBuf acc, hist, image;
+  explicit TallyHost(Pool* pl) : acc(pl), hist(pl), image(pl) {}
 };
 
 }  // namespace
@@ -109,6 +184,18 @@ struct Job {
   std::vector<std::unique_ptr<TallyHost>> tallies;
   DBuf d_tallies;
   DBuf d_counters;  // per-job event counters
+  bool flights = true;  // background free flights when the tallies allow them
+  // FP32 dose staged on the main stream (mdx_stage_voxel_dose_f32)
+  DBuf d_stage;
+  hipEvent_t ev_stage = nullptr;
+  int staged_grid = -1;
+  uint64_t staged_n = 0;
+  Job() = default;
+  Job(const Job&) = delete;
+  Job& operator=(const Job&) = delete;
+  ~Job() {
+    if (ev_stage) (void)hipEventDestroy(ev_stage);
+  }
   size_t device_bytes() const {
     size_t b = d_bg.bytes + d_domain.bytes + d_cdf.bytes + d_energies.bytes + d_source.bytes + d_tallies.bytes +
                d_counters.bytes;
@@ -119,6 +206,7 @@ struct Job {
 };
 
 struct mdx_ctx {
+  Pool pool;  // declared first: destroyed after every job that returns blocks to it
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -153,11 +241,14 @@ struct mdx_ctx {
   int interact_min = 16;
   DBuf d_work;  // queue kernel: work counter
   DBuf d_jobs, d_prefix;
+  DBuf d_dose32;  // staging of mdx_get_voxel_dose_f32 (on the I/O stream)
 
   // Round buffer of the graph path: [n_jobs, total, prefix[kRoundMax + 1], jobs[kRoundMax]]
   bool use_graphs = false;
   bool inflight = false;  // a round was launched by mdx_run_jobs_async and not yet waited for
+  std::vector<int> round_jobs, round_owners;  // the jobs of that round and the owners of their domains
   hipStream_t io_stream = nullptr;
+  hipStream_t js = nullptr;  // stream of the current call's job set-up work
   void* h_round = nullptr;  // pinned
   DBuf d_round;
   hipGraphExec_t graph = nullptr;
@@ -222,6 +313,20 @@ void set_device_nowait(mdx_ctx* c) {
 void set_device(mdx_ctx* c) {
   set_device_nowait(c);
   if (c->inflight) hip_check(hipStreamSynchronize(c->stream), "round in flight");
+  c->js = c->stream;
+}
+
+// Calls that change only the selected job's device state (domain, grids,
+// source, tallies) run beside a round in flight, on the I/O stream, unless
+// that round uses the job or the domain it owns.
+void set_device_job(mdx_ctx* c) {
+  set_device_nowait(c);
+  c->js = c->stream;
+  if (!c->inflight) return;
+  const bool used = std::find(c->round_jobs.begin(), c->round_jobs.end(), c->cur) != c->round_jobs.end() ||
+                    std::find(c->round_owners.begin(), c->round_owners.end(), c->cur) != c->round_owners.end();
+  if (used) hip_check(hipStreamSynchronize(c->stream), "round in flight");
+  else c->js = c->io_stream;
 }
 
 // Device-to-host copy on the I/O stream, which runs beside a round in flight.
@@ -252,8 +357,8 @@ void upload_domain(mdx_ctx* c) {
       if (G.d_dose.bytes != nv * 16) {
         G.d_dose.alloc(nv * 16);
         G.d_dose_n.alloc(nv * 4);
-        hip_check(mdx_launch::zero(G.d_dose.as<double>(), 2 * nv, c->stream), "zero dose");
-        hip_check(mdx_launch::zero_u32(G.d_dose_n.as<uint32_t>(), nv, c->stream), "zero dose count");
+        hip_check(mdx_launch::zero(G.d_dose.as<double>(), 2 * nv, c->js), "zero dose");
+        hip_check(mdx_launch::zero_u32(G.d_dose_n.as<uint32_t>(), nv, c->js), "zero dose count");
       }
       d.dose = G.d_dose.as<double>();
       d.dose_n = G.d_dose_n.as<uint32_t>();
@@ -264,11 +369,11 @@ void upload_domain(mdx_ctx* c) {
   }
   if (!c->J().d_bg.p) {
     c->J().d_bg.alloc(3 * sizeof(double));
-    hip_check(mdx_launch::zero(c->J().d_bg.as<double>(), 3, c->stream), "zero bg");
+    hip_check(mdx_launch::zero(c->J().d_bg.as<double>(), 3, c->js), "zero bg");
   }
   D.bg_dose = c->J().d_bg.as<double>();
-  c->J().d_domain.upload(&D, 1, c->stream);
-  hip_check(hipStreamSynchronize(c->stream), "upload domain");
+  c->J().d_domain.upload(&D, 1, c->js);
+  hip_check(hipStreamSynchronize(c->js), "upload domain");
 }
 
 void check_materials(mdx_ctx* c) {
@@ -277,8 +382,7 @@ void check_materials(mdx_ctx* c) {
   if (c->slot_of(c->J().background_id) == 255)
     throw std::out_of_range("background material " + std::to_string(c->J().background_id) + " has no tables");
   for (auto& g : c->J().grids) {
-    bool seen[256] = {false};
-    for (uint8_t v : g->ids) seen[v] = true;
+    const bool* seen = g->present;
     for (int v = 0; v < 256; ++v)
       if (seen[v] && c->slot_of(v) == 255)
         throw std::out_of_range("grid material " + std::to_string(v) + " has no tables");
@@ -369,22 +473,22 @@ void upload_tallies(mdx_ctx* c) {
     d.hist = H.hist.as<double>();
     d.image = H.image.as<double>();
   }
-  c->J().d_tallies.upload(&TL, 1, c->stream);
-  hip_check(hipStreamSynchronize(c->stream), "upload tallies");
+  c->J().d_tallies.upload(&TL, 1, c->js);
+  hip_check(hipStreamSynchronize(c->js), "upload tallies");
 }
 
 void zero_tallies(mdx_ctx* c) {
   for (auto& h : c->J().tallies) {
-    hip_check(mdx_launch::zero(h->acc.as<double>(), h->acc.bytes / 8, c->stream), "zero acc");
-    hip_check(mdx_launch::zero(h->hist.as<double>(), h->hist.bytes / 8, c->stream), "zero hist");
-    hip_check(mdx_launch::zero(h->image.as<double>(), h->image.bytes / 8, c->stream), "zero image");
+    hip_check(mdx_launch::zero(h->acc.as<double>(), h->acc.bytes / 8, c->js), "zero acc");
+    hip_check(mdx_launch::zero(h->hist.as<double>(), h->hist.bytes / 8, c->js), "zero hist");
+    hip_check(mdx_launch::zero(h->image.as<double>(), h->image.bytes / 8, c->js), "zero image");
   }
   for (auto& g : c->J().grids) {
-    hip_check(mdx_launch::zero(g->d_dose.as<double>(), g->d_dose.bytes / 8, c->stream), "zero dose");
-    hip_check(mdx_launch::zero_u32(g->d_dose_n.as<uint32_t>(), g->d_dose_n.bytes / 4, c->stream), "zero n");
+    hip_check(mdx_launch::zero(g->d_dose.as<double>(), g->d_dose.bytes / 8, c->js), "zero dose");
+    hip_check(mdx_launch::zero_u32(g->d_dose_n.as<uint32_t>(), g->d_dose_n.bytes / 4, c->js), "zero n");
   }
-  if (c->J().d_bg.p) hip_check(mdx_launch::zero(c->J().d_bg.as<double>(), 3, c->stream), "zero bg");
-  hip_check(hipStreamSynchronize(c->stream), "reset tallies");
+  if (c->J().d_bg.p) hip_check(mdx_launch::zero(c->J().d_bg.as<double>(), 3, c->js), "zero bg");
+  hip_check(hipStreamSynchronize(c->js), "reset tallies");
 }
 
 const mdx::MaterialTables& host_material(mdx_ctx* c, int id) {
@@ -442,9 +546,12 @@ uint64_t fill_round(mdx_ctx* c, int n_jobs, const int* ids, const uint64_t* firs
                     const uint64_t* seed, mdx_job_dev* jd, uint64_t* prefix) {
   KeepCur kc(c);
   prefix[0] = 0;
+  c->round_jobs.assign(ids, ids + n_jobs);
+  c->round_owners.clear();
   for (int i = 0; i < n_jobs; ++i) {
     job_at(c, ids[i]);
     c->cur = domain_owner(c, ids[i]);
+    c->round_owners.push_back(c->cur);
     check_materials(c);
     const mdx_domain_dev* D = c->J().d_domain.as<mdx_domain_dev>();
     c->cur = ids[i];
@@ -457,6 +564,9 @@ uint64_t fill_round(mdx_ctx* c, int n_jobs, const int* ids, const uint64_t* firs
     jd[i].n = n[i];
     jd[i].seed = seed[i];
     jd[i].stream = 0;
+    bool volume = false;
+    for (auto& t : c->J().tallies) volume = volume || t->t.kind == MDX_TALLY_AACUBOID;
+    jd[i].flights = c->J().flights && !volume ? 1u : 0u;
     prefix[i + 1] = prefix[i] + n[i];
   }
   return prefix[n_jobs];
@@ -571,11 +681,13 @@ int mdx_create(int device, mdx_ctx** out) {
     set_device(c.get());
     hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
     hip_check(hipStreamCreateWithFlags(&c->io_stream, hipStreamNonBlocking), "hipStreamCreate");
+    c->js = c->stream;
     hip_check(hipEventCreate(&c->ev0), "hipEventCreate");
     hip_check(hipEventCreate(&c->ev1), "hipEventCreate");
     hipDeviceProp_t prop;
     hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
     c->num_cu = prop.multiProcessorCount;
+    c->pool.limit = prop.totalGlobalMem / 4;
     c->d_counters.alloc(MDX_NCOUNTER * sizeof(unsigned long long));
     c->d_work.alloc(sizeof(unsigned long long));
     *out = c.release();
@@ -649,8 +761,7 @@ int mdx_build_tables_for_domain(mdx_ctx* c) {
     need(c && c->J().domain_set, "domain not set");
     std::vector<int> ids{c->J().background_id};
     for (auto& g : c->J().grids) {
-      bool seen[256] = {false};
-      for (uint8_t v : g->ids) seen[v] = true;
+      const bool* seen = g->present;
       for (int v = 0; v < 256; ++v)
         if (seen[v] && std::find(ids.begin(), ids.end(), v) == ids.end()) ids.push_back(v);
     }
@@ -662,6 +773,7 @@ int mdx_set_domain(mdx_ctx* c, const double dim_space[3], int background_id) {
   return guard([&] {
     need(c && dim_space, "NULL argument");
     need(background_id >= 0 && background_id < 256, "background material ID must be in [0, 255]");
+    if (c->device >= 0) set_device_job(c);  // before any buffer of the job is replaced
     for (int k = 0; k < 3; ++k) {
       need(dim_space[k] > 0, "dim_space must be positive");
       c->J().dim_space[k] = dim_space[k];
@@ -670,10 +782,7 @@ int mdx_set_domain(mdx_ctx* c, const double dim_space[3], int background_id) {
     c->J().grids.clear();
     c->J().domain_of = -1;
     c->J().domain_set = true;
-    if (c->device >= 0) {
-      set_device(c);
-      upload_domain(c);
-    }
+    if (c->device >= 0) upload_domain(c);
   });
 }
 
@@ -684,7 +793,7 @@ int mdx_add_grid(mdx_ctx* c, const uint8_t* ids, const int dims[3], const double
     need(c->J().domain_set, "mdx_set_domain must come first");
     need(c->J().domain_of < 0, "the selected job uses another job's domain");
     need((int)c->J().grids.size() < MDX_MAX_GRIDS, "too many grids");
-    auto g = std::make_unique<GridHost>();
+    auto g = std::make_unique<GridHost>(&c->pool);
     size_t nv = 1;
     for (int k = 0; k < 3; ++k) {
       need(dims[k] > 0 && spacing[k] > 0, "grid dims and spacing must be positive");
@@ -693,10 +802,13 @@ int mdx_add_grid(mdx_ctx* c, const uint8_t* ids, const int dims[3], const double
       g->origin[k] = origin[k];
       nv *= (size_t)dims[k];
     }
-    g->ids.assign(ids, ids + nv);
-    if (c->device >= 0) {
-      set_device(c);
-      g->d_ids.upload(g->ids.data(), nv, c->stream);
+    scan_ids(ids, nv, g->present);
+    if (c->device >= 0) {  // straight from the caller's memory, which it may reuse on return
+      set_device_job(c);
+      g->d_ids.upload(ids, nv, c->js);
+      hip_check(hipStreamSynchronize(c->js), "upload grid");
+    } else {
+      g->ids.assign(ids, ids + nv);
     }
     c->J().grids.push_back(std::move(g));
     if (c->device >= 0) upload_domain(c);
@@ -716,6 +828,7 @@ int mdx_load_scene(mdx_ctx* c, const char* json_path) {
     for (auto& g : sc.grids) grids.push_back(mdx::read_nifti(g.path));
     double ds[3] = {sc.dim_space[0], sc.dim_space[1], sc.dim_space[2]};
     for (int k = 0; k < 3; ++k) need(ds[k] > 0, "dim_space must be positive");
+    if (c->device >= 0) set_device_job(c);  // before any buffer of the job is replaced
     c->J().dim_space[0] = ds[0];
     c->J().dim_space[1] = ds[1];
     c->J().dim_space[2] = ds[2];
@@ -725,16 +838,17 @@ int mdx_load_scene(mdx_ctx* c, const char* json_path) {
     c->J().domain_set = true;
     for (size_t i = 0; i < grids.size(); ++i) {
       need(i < MDX_MAX_GRIDS, "too many grids");
-      auto g = std::make_unique<GridHost>();
+      auto g = std::make_unique<GridHost>(&c->pool);
       for (int k = 0; k < 3; ++k) {
         g->dims[k] = grids[i].dims[k];
         g->spacing[k] = grids[i].spacing[k];
         g->origin[k] = sc.grids[i].origin[k];
       }
       g->ids = std::move(grids[i].ids);
+      scan_ids(g->ids.data(), g->ids.size(), g->present);
       if (c->device >= 0) {
-        set_device(c);
-        g->d_ids.upload(g->ids.data(), g->ids.size(), c->stream);
+        set_device_job(c);
+        g->d_ids.upload(g->ids.data(), g->ids.size(), c->js);
       }
       c->J().grids.push_back(std::move(g));
     }
@@ -767,7 +881,13 @@ int mdx_get_grid_ids(mdx_ctx* c, int grid, uint8_t* out) {
     need(c && out, "NULL argument");
     Job& D = c->DJ();
     if (grid < 0 || grid >= (int)D.grids.size()) throw std::out_of_range("grid index out of range");
-    std::memcpy(out, D.grids[grid]->ids.data(), D.grids[grid]->ids.size());
+    const GridHost& g = *D.grids[grid];
+    if (!g.ids.empty()) {
+      std::memcpy(out, g.ids.data(), g.ids.size());
+    } else {
+      set_device_nowait(c);
+      read_back(c, out, g.d_ids.p, g.voxels(), "read grid");
+    }
   });
 }
 
@@ -822,7 +942,7 @@ int mdx_set_source(mdx_ctx* c, const mdx_source* s) {
       default:
         throw std::invalid_argument("unknown directionality");
     }
-    set_device(c);
+    set_device_job(c);
     if (s->spectrum == MDX_SPEC_MONO) {
       need(s->energy > 0, "energy must be positive");
       d.energy = (float)s->energy;
@@ -839,8 +959,8 @@ int mdx_set_source(mdx_ctx* c, const mdx_source* s) {
         cdf[i] = (float)acc;
         en[i] = (float)s->bin_energy[i];
       }
-      c->J().d_cdf.upload(cdf.data(), cdf.size(), c->stream);
-      c->J().d_energies.upload(en.data(), en.size(), c->stream);
+      c->J().d_cdf.upload(cdf.data(), cdf.size(), c->js);
+      c->J().d_energies.upload(en.data(), en.size(), c->js);
       d.n_bins = s->n_bins;
       d.cdf = c->J().d_cdf.as<float>();
       d.energies = c->J().d_energies.as<float>();
@@ -848,8 +968,8 @@ int mdx_set_source(mdx_ctx* c, const mdx_source* s) {
       throw std::invalid_argument("unknown spectrum kind");
     }
     c->J().src = d;
-    c->J().d_source.upload(&d, 1, c->stream);
-    hip_check(hipStreamSynchronize(c->stream), "upload source");
+    c->J().d_source.upload(&d, 1, c->js);
+    hip_check(hipStreamSynchronize(c->js), "upload source");
     c->J().source_set = true;
   });
 }
@@ -859,9 +979,9 @@ int mdx_set_tallies(mdx_ctx* c, const mdx_tally* t, int n) {
     need(c && (t || n == 0), "NULL argument");
     need(n >= 0 && n <= MDX_MAX_TALLIES, "too many tallies");
     std::vector<std::unique_ptr<TallyHost>> v;
-    set_device(c);
+    set_device_job(c);
     for (int i = 0; i < n; ++i) {
-      auto h = std::make_unique<TallyHost>();
+      auto h = std::make_unique<TallyHost>(&c->pool);
       h->t = t[i];
       mdx_tally& q = h->t;
       if (q.kind == MDX_TALLY_RECT || q.kind == MDX_TALLY_IMAGE) {
@@ -898,7 +1018,7 @@ int mdx_enable_voxel_dose(mdx_ctx* c, int enable) {
     need(c, "NULL context");
     need(c->J().domain_of < 0, "voxel dose belongs to the job that owns the domain");
     c->J().voxel_dose = enable != 0;
-    set_device(c);
+    set_device_job(c);
     if (c->J().domain_set) upload_domain(c);
   });
 }
@@ -906,7 +1026,7 @@ int mdx_enable_voxel_dose(mdx_ctx* c, int enable) {
 int mdx_reset_tallies(mdx_ctx* c) {
   return guard([&] {
     need(c, "NULL context");
-    set_device(c);
+    set_device_job(c);
     zero_tallies(c);
   });
 }
@@ -1045,6 +1165,13 @@ int mdx_job_alias_domain(mdx_ctx* c, int src) {
   });
 }
 
+int mdx_set_flights(mdx_ctx* c, int enable) {
+  return guard([&] {
+    need(c, "NULL context");
+    c->J().flights = enable != 0;
+  });
+}
+
 int mdx_set_graphs(mdx_ctx* c, int enable) {
   return guard([&] {
     need(c, "NULL context");
@@ -1166,6 +1293,52 @@ int mdx_get_voxel_dose(mdx_ctx* c, int grid, double* ss, uint32_t* count) {
     GridHost& g = *D.grids[grid];
     if (ss) read_back(c, ss, g.d_dose.p, g.d_dose.bytes, "read dose");
     if (count) read_back(c, count, g.d_dose_n.p, g.d_dose_n.bytes, "read dose n");
+  });
+}
+
+int mdx_get_voxel_dose_f32(mdx_ctx* c, int grid, uint64_t n_histories, float* out) {
+  return guard([&] {
+    need(c && out, "NULL argument");
+    need(n_histories > 0, "n_histories must be positive");
+    Job& D = c->DJ();
+    if (grid < 0 || grid >= (int)D.grids.size()) throw std::out_of_range("grid index out of range");
+    need(D.voxel_dose, "voxel dose is not enabled");
+    set_device_nowait(c);
+    GridHost& g = *D.grids[grid];
+    const size_t nv = g.voxels();
+    if (D.staged_grid == grid && D.staged_n == n_histories) {
+      hip_check(hipStreamWaitEvent(c->io_stream, D.ev_stage, 0), "wait staged dose");
+      read_back(c, out, D.d_stage.p, nv * 8, "read dose f32");
+      D.staged_grid = -1;
+      D.d_stage.release();
+      return;
+    }
+    c->d_dose32.ensure(nv * 8);
+    hip_check(mdx_launch::dose_f32(g.d_dose.as<double>(), nv, (double)n_histories, c->d_dose32.as<float>(),
+                                   c->io_stream),
+              "dose f32");
+    read_back(c, out, c->d_dose32.p, nv * 8, "read dose f32");
+  });
+}
+
+int mdx_stage_voxel_dose_f32(mdx_ctx* c, int grid, uint64_t n_histories) {
+  return guard([&] {
+    need(c, "NULL context");
+    need(n_histories > 0, "n_histories must be positive");
+    Job& D = c->DJ();
+    if (grid < 0 || grid >= (int)D.grids.size()) throw std::out_of_range("grid index out of range");
+    need(D.voxel_dose, "voxel dose is not enabled");
+    set_device_nowait(c);
+    GridHost& g = *D.grids[grid];
+    const size_t nv = g.voxels();
+    if (!D.d_stage.pool) D.d_stage.pool = &c->pool;
+    D.d_stage.ensure(nv * 8);
+    if (!D.ev_stage) hip_check(hipEventCreateWithFlags(&D.ev_stage, hipEventDisableTiming), "hipEventCreate");
+    hip_check(mdx_launch::dose_f32(g.d_dose.as<double>(), nv, (double)n_histories, D.d_stage.as<float>(), c->stream),
+              "dose f32");
+    hip_check(hipEventRecord(D.ev_stage, c->stream), "hipEventRecord");
+    D.staged_grid = grid;
+    D.staged_n = n_histories;
   });
 }
 

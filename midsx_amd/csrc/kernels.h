@@ -34,5 +34,7 @@ hipError_t kahn(const mdx_tables_dev* T, int slot, int n, const float* E, uint64
 hipError_t zero(double* p, size_t n, hipStream_t st);
 hipError_t zero_u32(uint32_t* p, size_t n, hipStream_t st);
 hipError_t zero_u64(unsigned long long* p, size_t n, hipStream_t st);
+// (mean per history, relative standard error) per voxel from (sum, sum^2).
+hipError_t dose_f32(const double* ss, size_t nv, double n, float* out, hipStream_t st);
 
 }  // namespace mdx_launch

@@ -150,8 +150,8 @@ struct mdx_job_dev {
   uint64_t first;
   uint64_t n;
   uint64_t seed;
-  uint32_t stream;  // 0 from mdx_run_jobs: a job's histories do not depend on its slot
-  uint32_t pad;
+  uint32_t stream;   // 0 from mdx_run_jobs: a job's histories do not depend on its slot
+  uint32_t flights;  // 1: free flights through the background (no volume tallies)
 };
 
 // Event counters, one int64 each; indexes into the counter array.

@@ -466,6 +466,78 @@ __device__ void tallies_step(const mdx_tallies_dev* TL, double* acc, const Step&
   }
 }
 
+// Distance along the flight line from (x, y, z) in direction d to the first
+// point where a step without an interaction can score in any tally, less a
+// margin that covers the FP32 drift of the stepped position from that line;
+// +inf if there is none.  Such a step scores only where its segment crosses a
+// surface tally's plane in the direction of the normal (surface_tally.cpp:
+// 16-34) or enters a volume tally's box from outside (volume_tally.cpp:
+// 38-60): starting inside a box it carries no deposit and no interaction, and
+// a straight line enters a box once.  Steps short of this distance skip the
+// tally replay; the step that reaches it is replayed in full.
+__device__ float next_tally_event(const mdx_tallies_dev* TL, float x, float y, float z, float dx, float dy,
+                                  float dz) {
+  float e = INFINITY;
+  for (int t = 0; t < TL->n; ++t) {
+    const mdx_tally_dev& T = TL->t[t];
+    if (T.kind == MDX_TALLY_AACUBOID) {
+      const float pos[3] = {x, y, z}, dir[3] = {dx, dy, dz};
+      float enter = -INFINITY, exit = INFINITY;
+      for (int i = 0; i < 3; ++i) {
+        if (fabsf(dir[i]) < kEps) continue;
+        float a = (T.p0[i] - pos[i]) / dir[i];
+        float b = (T.p1[i] - pos[i]) / dir[i];
+        if (a > b) { float tmp = a; a = b; b = tmp; }
+        enter = fmaxf(enter, a);
+        exit = fminf(exit, b);
+      }
+      if (enter > exit || exit < 0.0f || enter <= 0.0f) continue;
+      e = fminf(e, enter);
+    } else {
+      const float cosv = dx * T.nrm[0] + dy * T.nrm[1] + dz * T.nrm[2];
+      if (fabsf(cosv) < kEps || cosv < 0.0f) continue;
+      const float tt = ((T.p0[0] - x) * T.nrm[0] + (T.p0[1] - y) * T.nrm[1] + (T.p0[2] - z) * T.nrm[2]) / cosv;
+      if (tt < -0.01f) continue;
+      e = fminf(e, tt);
+    }
+  }
+  return e - (0.01f + 1.0e-4f * fabsf(e));
+}
+
+// From a point (x, y, z) in the domain along d: the distance to the first
+// face of a grid box the line enters (0 when the point already lies in a box,
+// +inf when it meets none) and to the domain's boundary.
+__device__ __forceinline__ void flight_bounds(const mdx_domain_dev* D, float x, float y, float z, float dx, float dy,
+                                              float dz, float& dgrid, float& dexit) {
+  const float pos[3] = {x, y, z}, dir[3] = {dx, dy, dz};
+  float inv[3];
+  dexit = INFINITY;
+  for (int i = 0; i < 3; ++i) {
+    inv[i] = 1.0f / dir[i];
+    if (dir[i] > 0.0f) dexit = fminf(dexit, (D->dim_space[i] - pos[i]) * inv[i]);
+    else if (dir[i] < 0.0f) dexit = fminf(dexit, -pos[i] * inv[i]);
+  }
+  dexit = fmaxf(dexit, 0.0f);
+  dgrid = INFINITY;
+  for (int g = 0; g < D->n_grids; ++g) {
+    const mdx_grid_dev& G = D->grids[g];
+    float enter = -INFINITY, exit = INFINITY;
+    bool miss = false;
+    for (int i = 0; i < 3; ++i) {
+      const float lo = G.origin[i] - pos[i], hi = G.origin[i] + G.dim[i] - pos[i];
+      if (dir[i] == 0.0f) {
+        miss = miss || lo > 0.0f || hi < 0.0f;
+        continue;
+      }
+      float a = lo * inv[i], b = hi * inv[i];
+      if (a > b) { float t = a; a = b; b = t; }
+      enter = fmaxf(enter, a);
+      exit = fminf(exit, b);
+    }
+    if (!miss && enter <= exit && exit >= 0.0f) dgrid = fminf(dgrid, fmaxf(enter, 0.0f));
+  }
+}
+
 // Waves per SIMD the queue kernel is compiled for (register budget).
 #ifndef MDX_QUEUE_WAVES_PER_EU
 #define MDX_QUEUE_WAVES_PER_EU 4
@@ -512,6 +584,9 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
   int qg = -1;
   uint64_t qlin = 0;
   uint32_t nstep = 0;
+  float tfl = 0;  // flight distance left before a step can score in a tally
+  // free flights: the job allows them / the photon sits in the background
+  bool flights = false, fly = false;
   for (;;) {
     // ---- refill: claim histories for free lanes, in batches
     const bool free_lane = !alive && !exhausted;
@@ -548,6 +623,9 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
           step_scale = 0.69314718055994531f / smax;
           cslot = -1;
           nstep = 0;
+          tfl = next_tally_event(TL, px, py, pz, dx, dy, dz);
+          flights = jobs[j].flights != 0;
+          fly = false;
           alive = true;
         } else {
           exhausted = true;
@@ -562,34 +640,83 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
       ++nstep;
       float u1, u2;
       srng.next(u1, u2);
-      const float s = -__log2f(u1) * step_scale;
       const float x0 = px, y0 = py, z0 = pz;
-      px = px + s * dx;
-      py = py + s * dy;
-      pz = pz + s * dz;
-      if (!in_domain(D, px, py, pz)) {
-        ++c_esc;
-        Step r{x0, y0, z0, dx, dy, dz, s, E, nc, ni, 0, 0, 0.0f, false};
-        tallies_step<MULTI>(TL, acc, r, c_surf, lhist, hoff);
-        alive = false;
-      } else {
-        uint64_t lin = 0;
-        const int g = voxel_lookup(D, px, py, pz, lin);
-        const int slot = g < 0 ? D->bg_slot : (int)T->id_to_slot[D->grids[g].ids[lin]];
-        if (slot != cslot) {
-          stot = curve_eval(T, T->mat[slot].total, E);
+      if (fly) {
+        // background free flight: distance at the background's own cross
+        // section, cut at the next grid face or the domain's boundary
+        const int bgs = D->bg_slot;
+        if (cslot != bgs) {
+          stot = curve_eval(T, T->mat[bgs].total, E);
           thr = 1.0f - stot / smax;
-          cslot = slot;
+          cslot = bgs;
         }
-        if (!(u2 < thr)) {  // real interaction: park it for the interaction phase
+        const float si = -__log2f(u1) * (0.69314718055994531f / stot);
+        float dgrid, dexit;
+        flight_bounds(D, x0, y0, z0, dx, dy, dz, dgrid, dexit);
+        if (si < dgrid && si < dexit) {  // real interaction in the background
+          px = x0 + si * dx;
+          py = y0 + si * dy;
+          pz = z0 + si * dz;
           pend = true;
-          qx = x0; qy = y0; qz = z0; qs = s;
-          qg = g;
-          qlin = lin;
-        } else {
+          qx = x0; qy = y0; qz = z0; qs = si;
+          qg = -1;
+          qlin = 0;
+        } else if (dgrid <= dexit) {  // up to the grid's face; majorant steps from there
+          px = x0 + dgrid * dx;
+          py = y0 + dgrid * dy;
+          pz = z0 + dgrid * dz;
+          fly = false;
+          tfl -= dgrid;
+          if (tfl <= 0.0f) {
+            Step r{x0, y0, z0, dx, dy, dz, dgrid, E, nc, ni, 0, 0, 0.0f, false};
+            tallies_step<MULTI>(TL, acc, r, c_surf, lhist, hoff);
+            tfl = next_tally_event(TL, px, py, pz, dx, dy, dz);
+          }
+          if (nstep >= kMaxSteps) alive = false;
+        } else {  // out of the domain, overshooting by a majorant step
+          const float s = dexit - __log2f(u2) * step_scale;
+          px = x0 + s * dx;
+          py = y0 + s * dy;
+          pz = z0 + s * dz;
+          ++c_esc;
           Step r{x0, y0, z0, dx, dy, dz, s, E, nc, ni, 0, 0, 0.0f, false};
           tallies_step<MULTI>(TL, acc, r, c_surf, lhist, hoff);
-          if (nstep >= kMaxSteps) alive = false;
+          alive = false;
+        }
+      } else {
+        const float s = -__log2f(u1) * step_scale;
+        px = px + s * dx;
+        py = py + s * dy;
+        pz = pz + s * dz;
+        if (!in_domain(D, px, py, pz)) {
+          ++c_esc;
+          Step r{x0, y0, z0, dx, dy, dz, s, E, nc, ni, 0, 0, 0.0f, false};
+          tallies_step<MULTI>(TL, acc, r, c_surf, lhist, hoff);
+          alive = false;
+        } else {
+          uint64_t lin = 0;
+          const int g = voxel_lookup(D, px, py, pz, lin);
+          const int slot = g < 0 ? D->bg_slot : (int)T->id_to_slot[D->grids[g].ids[lin]];
+          if (slot != cslot) {
+            stot = curve_eval(T, T->mat[slot].total, E);
+            thr = 1.0f - stot / smax;
+            cslot = slot;
+          }
+          if (!(u2 < thr)) {  // real interaction: park it for the interaction phase
+            pend = true;
+            qx = x0; qy = y0; qz = z0; qs = s;
+            qg = g;
+            qlin = lin;
+          } else {
+            tfl -= s;
+            if (tfl <= 0.0f) {  // the step may reach a tally: replay it
+              Step r{x0, y0, z0, dx, dy, dz, s, E, nc, ni, 0, 0, 0.0f, false};
+              tallies_step<MULTI>(TL, acc, r, c_surf, lhist, hoff);
+              tfl = next_tally_event(TL, px, py, pz, dx, dy, dz);
+            }
+            if (nstep >= kMaxSteps) alive = false;
+            fly = flights && g < 0;
+          }
         }
       }
     }
@@ -647,6 +774,7 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
         }
         tallies_step<MULTI>(TL, acc, r, c_surf, lhist, hoff);
         if (nstep >= kMaxSteps) alive = false;
+        if (alive) tfl = next_tally_event(TL, px, py, pz, dx, dy, dz);
       }
     }
   }
@@ -764,6 +892,18 @@ __global__ void k_zero_u64(unsigned long long* p, size_t n) {
   if (i < n) p[i] = 0ull;
 }
 
+// Per voxel: the mean deposit per history m = S / N and the relative
+// standard error of that mean, sqrt((S2 / N - m^2) / (N - 1)) / m (0 where
+// nothing was deposited), from the FP64 (S, S2) pairs, as FP32 pairs.
+__global__ void k_dose_f32(const double2* __restrict__ ss, size_t nv, double n, float2* __restrict__ out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  const double2 v = ss[i];
+  const double m = v.x / n;
+  const double var = fmax(v.y / n - m * m, 0.0) / fmax(n - 1.0, 1.0);
+  out[i] = make_float2((float)m, m > 0.0 ? (float)(sqrt(var) / m) : 0.0f);
+}
+
 inline unsigned blocks_for(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
 }  // namespace
@@ -845,6 +985,13 @@ hipError_t zero(double* p, size_t n, hipStream_t st) {
 hipError_t zero_u32(uint32_t* p, size_t n, hipStream_t st) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+hipError_t dose_f32(const double* ss, size_t nv, double n, float* out, hipStream_t st) {
+  if (nv == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_dose_f32, dim3(blocks_for(nv, 256)), dim3(256), 0, st, reinterpret_cast<const double2*>(ss),
+                     nv, n, reinterpret_cast<float2*>(out));
   return hipGetLastError();
 }
 

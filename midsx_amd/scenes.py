@@ -27,7 +27,7 @@ def ellipsoid_phantom(shape, seed, background=WATER, specs=((BONE, (3, 8), (0.5,
     """
     rng = np.random.default_rng(seed)
     nx, ny, nz = shape
-    ids = np.full(shape, background, dtype=np.uint8)
+    ids = np.full(shape, background, dtype=np.uint8, order="F")  # x fastest in memory, as the library stores it
     x = (np.arange(nx) + 0.5) * voxel_cm
     y = (np.arange(ny) + 0.5) * voxel_cm
     z = (np.arange(nz) + 0.5) * voxel_cm

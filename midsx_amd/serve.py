@@ -6,8 +6,9 @@ arrival order while their device footprint fits the memory budget, and runs
 rounds: one multi-job launch of the queue kernel (a replayed hipGraph) in
 which every admitted job runs its next slice of histories.  While a round
 runs, the results of the jobs that finished in the round before are copied
-back on a second stream; the memory of a finished job is freed after the
-round, and jobs that arrived meanwhile are admitted before the next one.
+back and newly admitted jobs are set up (phantom upload, tally zeroing), both
+on a second stream beside the kernel; new jobs join the next round, and the
+memory of a finished job is freed once its results are back.
 
 A job's histories are keyed by its seed and history index, so its tallies do
 not depend on the round structure nor on the other jobs in the mix (the FP64
@@ -44,17 +45,37 @@ class JobResult:
         return self.finished - self.arrival
 
 
-def default_summary(job, dose_sum, dose_sum2, image):
-    """Total deposited energy, the largest relative standard error over the
-    ROI (voxels holding at least 10 % of the peak dose), and the image's
-    total energy fluence."""
-    n = job["histories"]
-    mean = dose_sum / n
-    var = np.maximum(dose_sum2 / n - mean * mean, 0.0) / max(n - 1, 1)
+def roi_summary(job, mean, rel, image, stride=4):
+    """From an FP32 dose read (mean per history, relative standard error):
+    the largest and median relative standard error over the ROI (voxels
+    holding at least 10 % of the peak dose), taken on every stride-th voxel
+    along each axis, and the image's total energy fluence."""
+    nx, ny, nz = job["shape"]
+    sub = (slice(None, None, stride),) * 3
+    mean = mean.reshape(nz, ny, nx)[sub]
+    rel = rel.reshape(nz, ny, nx)[sub]
     roi = mean >= 0.1 * mean.max() if mean.max() > 0 else np.zeros(mean.shape, bool)
-    rse = np.sqrt(var[roi]) / mean[roi] if roi.any() else np.zeros(1)
-    return dict(energy=float(dose_sum.sum()), roi_voxels=int(roi.sum()), roi_max_rse=float(rse.max()),
+    rse = rel[roi] if roi.any() else np.zeros(1)
+    return dict(roi_voxels=int(roi.sum()), roi_max_rse=float(rse.max()), roi_median_rse=float(np.median(rse)),
                 image_energy=float(image[..., 0].sum()))
+
+
+def read_dose(ctx, job, bufs, dose):
+    """The selected job's voxel dose into a reusable (pinned) buffer: dose
+    "f32" gives (mean per history, relative standard error) as float32,
+    "f64" the (sum, sum^2) of the deposits as float64."""
+    nv = int(np.prod(job["shape"]))
+    key = (nv, dose)
+    if key not in bufs:
+        dt = np.float32 if dose == "f32" else np.float64
+        try:
+            bufs[key] = ctx.pinned(2 * nv, dt)
+        except Exception:
+            bufs[key] = np.empty(2 * nv, dt)
+    if dose == "f32":
+        return ctx.voxel_dose_f32(0, job["histories"], out=bufs[key])
+    s, s2, _ = ctx.voxel_dose(0, out=bufs[key], counts=False)
+    return s, s2
 
 
 class _Active:
@@ -76,10 +97,12 @@ class Server:
     the free memory at construction).  round_histories: histories per round,
     shared equally by the admitted jobs (at least min_slice each, never more
     than a job has left).  phantoms: callable job -> uint8 grid (default:
-    `scenes.dose_phantom`), so that a client can hand over its own data."""
+    `scenes.dose_phantom`), so that a client can hand over its own data.
+    dose: format of the dose read back ("f32" or "f64", see `read_dose`);
+    on_result(job, a, b, image) receives it (the buffers are reused)."""
 
     def __init__(self, ctx, budget_bytes=None, round_histories=30_000_000, min_slice=100_000, max_active=512,
-                 graphs=True, phantoms=None, on_result=default_summary):
+                 graphs=True, phantoms=None, on_result=None, dose="f32"):
         self.ctx = ctx
         if budget_bytes is None:
             free, _ = ctx.mem_info()
@@ -91,22 +114,14 @@ class Server:
         self.graphs = graphs
         self.phantoms = phantoms or (lambda j: scenes.dose_phantom(j["shape"], j["voxel_cm"], j["phantom_seed"]))
         self.on_result = on_result
+        self.dose = dose
         self._buf = {}
         self.rounds = 0
         self.kernel_ms = 0.0
         self.counters = {}
+        self.host_s = dict(admit=0.0, read=0.0, wait=0.0, release=0.0, launch=0.0)
 
     # ------------------------------------------------------------------ helpers
-    def _host_buf(self, n):
-        b = self._buf.get(n)
-        if b is None:
-            try:
-                b = self.ctx.pinned(2 * n)
-            except Exception:
-                b = np.empty(2 * n)
-            self._buf[n] = b
-        return b
-
     def _admit(self, job, res, now):
         ctx = self.ctx
         slot = ctx.job_create()
@@ -118,10 +133,9 @@ class Server:
         """Copies a finished job's results back (overlaps a round in flight)."""
         ctx = self.ctx
         ctx.job_select(a.slot)
-        nv = int(np.prod(a.job["shape"]))
-        s, s2, _ = ctx.voxel_dose(0, out=self._host_buf(nv), counts=False)
+        d1, d2 = read_dose(ctx, a.job, self._buf, self.dose)
         image = ctx.tally(0)["image"]
-        a.res.summary = self.on_result(a.job, s, s2, image) if self.on_result else None
+        a.res.summary = self.on_result(a.job, d1, d2, image) if self.on_result else None
         a.res.histories = a.done
 
     def _plan(self, active):
@@ -165,47 +179,67 @@ class Server:
             now = clock()
             while order and arrivals[order[0]] <= now:
                 waiting.append(order.popleft())
-            while waiting and len(active) < self.max_active:
-                i = waiting[0]
-                need = scenes.dose_job_bytes(jobs[i])
-                if resident + need > self.budget and (active or to_read):
-                    break
-                waiting.popleft()
-                a = self._admit(jobs[i], results[i], clock())
-                resident += a.bytes
-                active.append(a)
-            if not active and not to_read:
-                if order:
-                    time.sleep(max(0.0, min(arrivals[order[0]] - clock(), 0.05)))
-                continue
             launched = False
+            t = clock()
             if active:
                 take = self._plan(active)
                 ctx.run_jobs_async([a.slot for a in active], take.tolist(), [a.done for a in active],
                                    [a.job["seed"] for a in active])
                 launched = True
-                for a, t in zip(active, take):
-                    a.done += int(t)
-            for a in to_read:  # beside the round in flight
+                for a, t_ in zip(active, take):
+                    a.done += int(t_)
+            t1 = clock()
+            self.host_s["launch"] += t1 - t
+            # beside the round in flight: results of the jobs finished in the
+            # round before, then the set-up of newly admitted jobs
+            for a in to_read:
                 self._read(a)
                 a.res.finished = clock()
+            t2 = clock()
+            self.host_s["read"] += t2 - t1
+            admitted = []
+            queued = 0  # histories the admitted jobs bring
+            while waiting and len(active) + len(admitted) < self.max_active:
+                if not launched and queued >= self.round_histories:
+                    break  # start the first round; admit the rest beside it
+                i = waiting[0]
+                need = scenes.dose_job_bytes(jobs[i])
+                if resident + need > self.budget and (active or to_read or admitted):
+                    break
+                waiting.popleft()
+                a = self._admit(jobs[i], results[i], clock())
+                resident += a.bytes
+                queued += a.job["histories"]
+                admitted.append(a)
+            t3 = clock()
+            self.host_s["admit"] += t3 - t2
             if launched:
                 st = ctx.wait()
+                self.host_s["wait"] += clock() - t3
                 self.rounds += 1
                 self.kernel_ms += st["kernel_ms"]
                 for k, v in st.items():
                     if k != "kernel_ms":
                         self.counters[k] = self.counters.get(k, 0) + v
+            t = clock()
             for a in to_read:
                 ctx.job_release(a.slot)
                 resident -= a.bytes
+            self.host_s["release"] += clock() - t
+            idle = not launched and not to_read and not admitted
             to_read = [a for a in active if a.done >= a.job["histories"]]
-            active = [a for a in active if a.done < a.job["histories"]]
+            active = [a for a in active if a.done < a.job["histories"]] + admitted
+            if self.dose == "f32":
+                for a in to_read:  # conversion queued ahead of the next round; the copy runs beside it
+                    ctx.job_select(a.slot)
+                    ctx.stage_voxel_dose_f32(0, a.job["histories"])
+            if idle and order:
+                time.sleep(max(0.0, min(arrivals[order[0]] - clock(), 0.05)))
         self.makespan = clock()
         return results
 
 
-def run_plain(ctx, jobs, arrivals=None, phantoms=None, on_result=default_summary):
+def run_plain(ctx, jobs, arrivals=None, phantoms=None, on_result=None, dose="f32"):
     """First-in first-out service with the plain kernel: each job in turn is
     uploaded, run in one launch, read back and freed."""
     phantoms = phantoms or (lambda j: scenes.dose_phantom(j["shape"], j["voxel_cm"], j["phantom_seed"]))
@@ -215,34 +249,38 @@ def run_plain(ctx, jobs, arrivals=None, phantoms=None, on_result=default_summary
     bufs = {}
     counters = {}
     kernel_ms = 0.0
+    host = dict(admit=0.0, run=0.0, read=0.0, release=0.0)
     t0 = time.perf_counter()
     for i in sorted(range(n), key=lambda k: arrivals[k]):
         wait = arrivals[i] - (time.perf_counter() - t0)
         if wait > 0:
             time.sleep(wait)
         j = jobs[i]
+        ta = time.perf_counter()
         slot = ctx.job_create()
-        results[i].admitted = time.perf_counter() - t0
+        results[i].admitted = ta - t0
         scenes.setup_dose_job(ctx, j, ids=phantoms(j))
+        tb = time.perf_counter()
         st = ctx.run(j["histories"], seed=j["seed"], first=0, kernel=KERNEL_PLAIN)
+        tc = time.perf_counter()
         kernel_ms += st["kernel_ms"]
         for k, v in st.items():
             if k != "kernel_ms":
                 counters[k] = counters.get(k, 0) + v
-        nv = int(np.prod(j["shape"]))
-        if nv not in bufs:
-            try:
-                bufs[nv] = ctx.pinned(2 * nv)
-            except Exception:
-                bufs[nv] = np.empty(2 * nv)
-        s, s2, _ = ctx.voxel_dose(0, out=bufs[nv], counts=False)
+        d1, d2 = read_dose(ctx, j, bufs, dose)
         image = ctx.tally(0)["image"]
-        results[i].summary = on_result(j, s, s2, image) if on_result else None
+        results[i].summary = on_result(j, d1, d2, image) if on_result else None
         results[i].histories = j["histories"]
-        results[i].finished = time.perf_counter() - t0
+        td = time.perf_counter()
+        results[i].finished = td - t0
         ctx.job_release(slot)
+        te = time.perf_counter()
+        host["admit"] += tb - ta
+        host["run"] += tc - tb
+        host["read"] += td - tc
+        host["release"] += te - td
     makespan = time.perf_counter() - t0
-    return results, dict(makespan=makespan, kernel_ms=kernel_ms, counters=counters)
+    return results, dict(makespan=makespan, kernel_ms=kernel_ms, counters=counters, host_s=host)
 
 
 def poisson_arrivals(n, rate, seed):

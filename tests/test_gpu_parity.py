@@ -16,11 +16,11 @@ from helpers import batch_mean_sigma, chi2_pvalue, count_chi2, hotelling, welch_
 pytestmark = pytest.mark.gpu
 
 
-def gpu_batches(ctx, n, k, seed, extract):
+def gpu_batches(ctx, n, k, seed, extract, kernel=M.KERNEL_PLAIN):
     out = []
     for b in range(k):
         ctx.reset_tallies()
-        st = ctx.run(n, seed=seed, first=b * n)
+        st = ctx.run(n, seed=seed, first=b * n, kernel=kernel)
         out.append(extract(ctx, st, n))
     return out
 
@@ -164,10 +164,13 @@ def radiography_extract(src, st, n):
                            [st["steps"] / n, st["real"] / n, st["compton"] / n, st["photo"] / n]])
 
 
-def test_radiography_small_matches_oracle():
+@pytest.mark.parametrize("kernel", [M.KERNEL_PLAIN, M.KERNEL_QUEUE])
+def test_radiography_small_matches_oracle(kernel):
     """Polyenergetic Kramers source, cone beam, heterogeneous phantom
     (water / bone / adipose), image and per-voxel dose tallies: each coarse
-    bin within 3.5 combined standard errors and Hotelling T^2 on the whole."""
+    bin within 3.5 combined standard errors and Hotelling T^2 on the whole.
+    The queue kernel runs with background free flights (image and dose
+    tallies only)."""
     kw = dict(shape=(64, 64, 64), voxel_cm=0.5, seed=3, image=(32, 32))
     ctx = M.Context(0)
     ctx.build_tables(list(scenes.C2_MATERIALS))
@@ -175,8 +178,12 @@ def test_radiography_small_matches_oracle():
     sim = oracle.Simulation(scenes.C2_MATERIALS)
     scenes.setup_radiography(sim, **kw)
     k, n = 160, 10000
-    g = np.array(gpu_batches(ctx, n, k, 51, radiography_extract))
+    g = np.array(gpu_batches(ctx, n, k, 51, radiography_extract, kernel))
     o = np.array(orc_batches(sim, n, k, 151, radiography_extract))
+    if kernel == M.KERNEL_QUEUE:  # free flights take fewer (rejected) steps: drop the step count
+        g, o = np.delete(g, -4, axis=1), np.delete(o, -4, axis=1)
+        assert ctx.run(20000, seed=5, first=10 ** 9, kernel=kernel)["steps"] < 0.7 * ctx.run(
+            20000, seed=5, first=10 ** 9)["steps"]
     gm, gs = batch_mean_sigma(g)
     om, os_ = batch_mean_sigma(o)
     keep = (gs > 0) & (os_ > 0) & (np.abs(om) > 3 * os_)

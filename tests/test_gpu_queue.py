@@ -204,3 +204,49 @@ def test_alias_domain_matches_own_copy():
     ctx.job_release(alias)
     ctx.job_release(0)
     ctx.close()
+
+
+def test_free_flights_match_majorant_steps():
+    """Background free flights change only the number of rejected steps: the
+    image, dose and interaction counts of a radiography job agree with the
+    same job run with majorant steps everywhere (Welch scores and Hotelling
+    T^2 over 40 batches each), in far fewer steps.  A job with a volume
+    tally never takes them."""
+    ctx = M.Context(0)
+    ctx.build_tables(list(scenes.C2_MATERIALS))
+    scenes.setup_radiography(ctx, shape=(48, 48, 40), voxel_cm=0.5, seed=4, image=(16, 16))
+    k, n = 40, 40000
+
+    def batches(seed):
+        out, steps = [], 0
+        for b in range(k):
+            ctx.reset_tallies()
+            st = ctx.run(n, seed=seed, first=b * n, kernel=M.KERNEL_QUEUE)
+            steps += st["steps"]
+            im = ctx.tally(0)["image"][..., 0].reshape(4, 4, 4, 4).sum(axis=(1, 3)).ravel()
+            dose = ctx.voxel_dose(0)[0].reshape(4, 10, 4, 12, 4, 12).sum(axis=(1, 3, 5)).ravel()
+            out.append(np.concatenate([im, dose, [ctx.background_dose()[0], st["real"], st["compton"],
+                                                  st["photo"], st["escapes"]]]) / n)
+        return np.array(out), steps
+
+    ctx.set_flights(True)
+    f, sf = batches(61)
+    ctx.set_flights(False)
+    m, sm = batches(62)
+    assert sf < 0.5 * sm, (sf, sm)
+    fm, fs = batch_mean_sigma(f)
+    mm, ms = batch_mean_sigma(m)
+    keep = (fs > 0) & (ms > 0) & (np.abs(mm) > 3 * ms)
+    assert keep.sum() > 30
+    z = welch_normal_scores(fm, fs, k, mm, ms, k)[keep]
+    assert np.all(np.abs(z) < 3.5), np.max(np.abs(z))
+    p, t2, rank = hotelling(f[:, keep], m[:, keep])
+    assert p > 0.01, (p, t2, rank)
+    # with a volume tally the job keeps majorant steps: same counts as flights off
+    ctx.set_tallies([dict(kind=M.TALLY_AACUBOID, p0=(1, 1, 1), p1=(20, 20, 10))])
+    ctx.set_flights(True)
+    a = ctx.run(20000, seed=9, first=0, kernel=M.KERNEL_QUEUE)
+    ctx.set_flights(False)
+    b = ctx.run(20000, seed=9, first=0, kernel=M.KERNEL_QUEUE)
+    assert a["steps"] == b["steps"] and a["real"] == b["real"]
+    ctx.close()

@@ -36,11 +36,13 @@ def ctx():
 def test_served_same_alone_and_in_mix(ctx):
     jobs = scenes.draw_dose_jobs(10, 5, shapes=(16, 24, 32), h_lo=5e4, h_hi=4e5)
     budget = 3 * max(scenes.dose_job_bytes(j) for j in jobs)
-    srv = Server(ctx, budget_bytes=budget, round_histories=300_000, min_slice=20_000, on_result=keep_arrays)
+    srv = Server(ctx, budget_bytes=budget, round_histories=300_000, min_slice=20_000, on_result=keep_arrays,
+                 dose="f64")
     mixed = srv.run(jobs, poisson_arrivals(len(jobs), 50.0, 9))
     assert srv.rounds > 3
     for i in (0, 3, 7):
-        alone = Server(ctx, budget_bytes=budget, round_histories=10 ** 9, on_result=keep_arrays).run([jobs[i]])[0]
+        alone = Server(ctx, budget_bytes=budget, round_histories=10 ** 9, on_result=keep_arrays,
+                       dose="f64").run([jobs[i]])[0]
         for a, m in zip(alone.summary, mixed[i].summary):
             assert np.count_nonzero(a) > 10
             assert np.allclose(m, a, rtol=1e-12, atol=1e-12 * np.abs(a).max()), (i, np.max(np.abs(m - a)))
@@ -49,7 +51,7 @@ def test_served_same_alone_and_in_mix(ctx):
 
 def test_plain_fifo_serves_same_jobs(ctx):
     jobs = scenes.draw_dose_jobs(4, 6, shapes=(16,), h_lo=5e4, h_hi=1e5)
-    res, info = run_plain(ctx, jobs, on_result=keep_arrays)
+    res, info = run_plain(ctx, jobs, on_result=keep_arrays, dose="f64")
     assert info["counters"]["histories"] == sum(j["histories"] for j in jobs)
     for r in res:
         assert r.finished >= r.admitted and np.count_nonzero(r.summary[0]) > 10
@@ -66,7 +68,7 @@ def test_served_dose_matches_oracle(ctx):
                kvp=80.0)
     k = 24
     jobs = [dict(job, seed=1000 + b) for b in range(k)]
-    srv = Server(ctx, round_histories=100_000, min_slice=5_000, on_result=keep_arrays)
+    srv = Server(ctx, round_histories=100_000, min_slice=5_000, on_result=keep_arrays, dose="f64")
     res = srv.run(jobs)
 
     def extract(s, image):
@@ -90,3 +92,23 @@ def test_served_dose_matches_oracle(ctx):
     assert np.all(np.abs(z) < 4.0), np.max(np.abs(z))
     p, t2, rank = hotelling(g[:, keep], o[:, keep])
     assert p > 0.01, (p, t2, rank)
+
+
+def test_dose_f32_read_matches_f64(ctx):
+    job = dict(shape=(20, 16, 12), voxel_cm=0.3, phantom_seed=3, histories=200000, seed=77, image=(8, 8),
+               energy=50e3)
+    slot = ctx.job_create()
+    scenes.setup_dose_job(ctx, job)
+    ctx.run(job["histories"], seed=job["seed"], kernel=M.KERNEL_QUEUE)
+    s, s2, _ = ctx.voxel_dose(0)
+    mean, rel = ctx.voxel_dose_f32(0, job["histories"])
+    n = job["histories"]
+    m = s / n
+    r = np.where(m > 0, np.sqrt(np.maximum(s2 / n - m * m, 0) / (n - 1)) / np.where(m > 0, m, 1), 0)
+    assert np.count_nonzero(m) > 100
+    assert np.allclose(mean, m, rtol=1e-6, atol=0)
+    assert np.allclose(rel, r, rtol=1e-5, atol=1e-7)
+    ctx.stage_voxel_dose_f32(0, n)  # queued conversion, then a plain copy
+    mean2, rel2 = ctx.voxel_dose_f32(0, n)
+    assert np.array_equal(mean2, mean) and np.array_equal(rel2, rel)
+    ctx.job_release(slot)

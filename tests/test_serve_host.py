@@ -28,8 +28,8 @@ class FakeCtx:
     def mem_info(self):
         return 1 << 40, 1 << 41
 
-    def pinned(self, n):
-        return np.empty(n)
+    def pinned(self, n, dtype=np.float64):
+        return np.empty(n, dtype)
 
     def job_create(self):
         s = self.next
@@ -75,6 +75,13 @@ class FakeCtx:
     def voxel_dose(self, g, out=None, counts=True):
         n = int(np.prod(self.slots[self.cur]["shape"]))
         return np.ones(n), np.ones(n), None
+
+    def stage_voxel_dose_f32(self, g, n_histories):
+        pass
+
+    def voxel_dose_f32(self, g, n_histories, out=None):
+        n = int(np.prod(self.slots[self.cur]["shape"]))
+        return np.ones(n, np.float32), np.zeros(n, np.float32)
 
     def tally(self, i):
         return {"image": np.zeros((128, 128, 3))}
