@@ -12,6 +12,16 @@ Prints one JSON line: throughput, the roofline of the transport kernel
 (algorithmic bytes from its int64 event counters over its HIP-event
 duration), and the FP64 CPU restatement timed on a bounded sample of the same
 workload on the host cores.
+
+Served job stream (the "served" object): dose jobs drawn with a fixed seed
+(cubic phantoms of 64^3 to 512^3 voxels at 0.5-2 mm, monoenergetic 20-120 keV
+or Kramers 60-120 kVp sources, 1e6-1e8 histories log-uniform, FP64 voxel dose
+plus a 128 x 128 image each), served by the continuous-batching loop and,
+for comparison, first-in first-out by the plain kernel one job per launch:
+throughput with every job queued at the start, then p50/p99 latency under
+Poisson arrivals at a fraction of the served capacity.  Phantoms are built
+before the clock starts (they are the client's data); uploads, tally
+zeroing, transport and the read-back of every job's dose and image are timed.
 """
 import argparse
 import json
@@ -35,6 +45,19 @@ def algorithmic_bytes(st):
     sum^2 read-modify-write (32); per image hit 16; per history 8."""
     return (33 * st["steps"] + 32 * st["real"] + 48 * st["rayleigh"] + 32 * st["kahn_s_tests"]
             + 32 * st["deposits"] + 16 * st["surface_hits"] + 8 * st["histories"])
+
+
+def measured_traffic_per_history():
+    """HBM bytes per history of the queue kernel on this workload from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None if absent)."""
+    path = os.path.join(ROOT, "profiles", "k_transport_queue_radiography_256x256x160.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_history"):
+                return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(kw, seconds=12.0, threads=16):
@@ -70,14 +93,64 @@ def timed_steps(ctx, kernel, n, steps, warmup, first):
     return stats, wall, first
 
 
-def copy_probe(mb):
-    """A plain device-to-device copy of known size, for scaling HBM counters."""
-    import torch
-    a = torch.empty(mb * 2 ** 20, dtype=torch.uint8, device="cuda")
-    b = torch.empty_like(a)
-    b.copy_(a)
-    torch.cuda.synchronize()
-    return 2 * mb * 2 ** 20
+def copy_probe(ctx, mb):
+    """The library's streaming copy of mb MiB (read mb MiB, write mb MiB),
+    twice: the reference for scaling the HBM counters of a profiled run, and
+    the achievable copy bandwidth."""
+    nbytes = mb * 2 ** 20
+    ms = ctx.copy_probe(nbytes)
+    return {"bytes_read": nbytes, "bytes_written": nbytes, "ms": ms, "GB_per_s": 2 * nbytes / ms / 1e6}
+
+
+def served_stream(ctx, n_jobs, seed, load):
+    """The served job stream: saturated throughput of the continuous-batching
+    loop and of plain FIFO service, then latency under Poisson arrivals."""
+    from midsx_amd import scenes
+    from midsx_amd.serve import Server, run_plain, poisson_arrivals, latency_percentiles, roi_summary
+    jobs = scenes.draw_dose_jobs(n_jobs, seed)
+    cache = {}
+    for j in jobs:
+        cache[id(j)] = scenes.dose_phantom(j["shape"], j["voxel_cm"], j["phantom_seed"])
+    ph = lambda j: cache[id(j)]
+    H = sum(j["histories"] for j in jobs)
+    summaries = {}
+
+    def keep(job, mean, rel, image):
+        summaries[id(job)] = roi_summary(job, mean, rel, image)
+
+    srv = Server(ctx, phantoms=ph, on_result=keep)
+    t = time.perf_counter()
+    srv.run(jobs)
+    served_s = time.perf_counter() - t
+    _, pinfo = run_plain(ctx, jobs, phantoms=ph)
+    plain_s = pinfo["makespan"]
+    cap = n_jobs / served_s
+    arr = poisson_arrivals(n_jobs, load * cap, seed + 1)
+    srv2 = Server(ctx, phantoms=ph)
+    lat = srv2.run(jobs, arr)
+    p50, p99 = latency_percentiles(lat)
+    plat, _ = run_plain(ctx, jobs, arr, phantoms=ph)
+    q50, q99 = latency_percentiles(plat)
+    rse = [summaries[id(j)]["roi_median_rse"] for j in jobs]
+    return {
+        "workload": "%d dose jobs (seed %d): phantoms %s, histories 1e6-1e8 log-uniform, total %d" % (
+            n_jobs, seed, sorted(set(j["shape"][0] for j in jobs)), H),
+        "value": H / served_s,
+        "unit": "histories/s",
+        "plain_value": H / plain_s,
+        "speedup_vs_plain": plain_s / served_s,
+        "served_s": served_s,
+        "plain_s": plain_s,
+        "kernel_s": srv.kernel_ms / 1e3,
+        "plain_kernel_s": pinfo["kernel_ms"] / 1e3,
+        "rounds": srv.rounds,
+        "host_s": {k: round(v, 4) for k, v in srv.host_s.items()},
+        "steps_per_history": srv.counters["steps"] / H,
+        "plain_steps_per_history": pinfo["counters"]["steps"] / H,
+        "latency": {"arrival_rate_jobs_per_s": load * cap, "load": load, "p50_s": p50, "p99_s": p99,
+                    "plain_fifo_p50_s": q50, "plain_fifo_p99_s": q99},
+        "roi_median_rse": {"median_over_jobs": float(np.median(rse)), "max_over_jobs": float(np.max(rse))},
+    }
 
 
 def main():
@@ -89,9 +162,15 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--queue-params", default=None,
+                    help="blocks_per_cu,refill_min,interact_min for the queue kernel (skips the autotuner)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--copy-probe-mb", type=int, default=0)
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--served-jobs", type=int, default=32)
+    ap.add_argument("--served-seed", type=int, default=2026)
+    ap.add_argument("--served-load", type=float, default=0.7)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: sharding histories over several GPUs is not built in this "
@@ -103,10 +182,14 @@ def main():
 
     kw = dict(shape=(256, 256, 160), voxel_cm=0.125, seed=2, image=(256, 256))
     ctx = M.Context(0)
-    ctx.build_tables(list(scenes.C2_MATERIALS))
+    ctx.build_tables(sorted(set(scenes.C2_MATERIALS) | set(scenes.DOSE_MATERIALS)))
     scenes.setup_radiography(ctx, **kw)
     n = args.histories_per_step
-    tuned = None if args.no_autotune else ctx.autotune(probe_histories=1000000)
+    tuned = None
+    if args.queue_params:
+        ctx.set_queue_params(*[int(v) for v in args.queue_params.split(",")])
+    elif not args.no_autotune:
+        tuned = ctx.autotune(probe_histories=1000000)
     stats, wall, first = timed_steps(ctx, M.KERNEL_QUEUE, n, args.steps, args.warmup, 0)
     plain = None
     if not args.no_plain:
@@ -121,8 +204,7 @@ def main():
     value = total / wall
 
     out = {
-        "metric": "photon histories/sec (radiography workload; queue kernel vs the plain one-simulation-per-launch "
-                  "kernel at equal histories, hence equal tally sigma)",
+        "metric": "photon histories/sec vs plain GPU at matched tally \u03c3; p50/p99 request latency",
         "value": value,
         "unit": "histories/s",
         "n_gpus": 1,
@@ -138,7 +220,9 @@ def main():
             "workload": "radiography: 256x256x160 uint8 phantom at 1.25 mm, 15-80 keV Kramers cone beam, "
                         "256x256 image + FP64 voxel dose, 1e8 histories",
             "histories_per_step": n,
-            "kernel": "queue",
+            "kernel": "queue (event-regrouped, background free flights); plain = one thread per history, "
+                      "majorant steps everywhere, every step replayed for the tallies; equal histories, "
+                      "hence equal tally sigma",
             "queue_params": dict(zip(("blocks_per_cu", "refill_min", "interact_min"), ctx.queue_params())),
             "steps_per_history": agg["steps"] / total,
         },
@@ -150,7 +234,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": (measured_traffic_per_history() or 0.0) * n or None,
+            "traffic_source": "rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 per launch (factors from the "
+                              "k_copy16 streaming copy), profiles/k_transport_queue_radiography_256x256x160.md",
             "kernel_ms_avg": float(kern_ms.mean()),
             "algorithmic_bytes_per_launch": float(bytes_per_launch),
         },
@@ -159,8 +245,10 @@ def main():
     }
     if tuned:
         out["config"]["autotune_probe_ms"] = tuned["probe_ms"]
+    if not args.no_served:
+        out["served"] = served_stream(ctx, args.served_jobs, args.served_seed, args.served_load)
     if args.copy_probe_mb:
-        out["copy_probe_bytes"] = copy_probe(args.copy_probe_mb)
+        out["copy_probe"] = copy_probe(ctx, args.copy_probe_mb)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(kw, seconds=args.cpu_seconds, threads=args.cpu_threads)
     print(json.dumps(out), flush=True)

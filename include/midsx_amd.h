@@ -237,6 +237,9 @@ int mdx_run_jobs_async(mdx_ctx* ctx, int n_jobs, const int* jobs, const uint64_t
 int mdx_wait(mdx_ctx* ctx, mdx_stats* stats);
 /* Free and total device memory (hipMemGetInfo), for admission by footprint. */
 int mdx_mem_info(mdx_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
+/* A streaming device-to-device copy of `bytes` (read and write once each),
+ * timed: the reference kernel for HBM bandwidth and traffic counters. */
+int mdx_copy_probe(mdx_ctx* ctx, uint64_t bytes, double* ms);
 /* Pinned host memory for fast tally and dose reads (hipHostMalloc). */
 int mdx_host_alloc(mdx_ctx* ctx, uint64_t bytes, void** out);
 int mdx_host_free(void* p);

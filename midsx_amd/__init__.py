@@ -42,7 +42,7 @@ EXPORTS = [
     "mdx_tables_device_bytes", "mdx_run_jobs", "mdx_set_queue_params", "mdx_get_queue_params",
     "mdx_autotune", "mdx_job_alias_domain", "mdx_set_graphs",
     "mdx_run_jobs_async", "mdx_wait", "mdx_mem_info", "mdx_host_alloc", "mdx_host_free",
-    "mdx_set_flights", "mdx_get_voxel_dose_f32", "mdx_stage_voxel_dose_f32",
+    "mdx_set_flights", "mdx_get_voxel_dose_f32", "mdx_stage_voxel_dose_f32", "mdx_copy_probe",
 ]
 
 
@@ -450,6 +450,12 @@ class Context:
         """Queues the FP32 conversion behind the running round; the next
         voxel_dose_f32(g, n_histories) then only copies."""
         self._call(self.lib.mdx_stage_voxel_dose_f32, self._h, ctypes.c_int(g), ctypes.c_uint64(n_histories))
+
+    def copy_probe(self, nbytes):
+        """Milliseconds of one streaming copy of nbytes (read + write)."""
+        ms = ctypes.c_double()
+        self._call(self.lib.mdx_copy_probe, self._h, ctypes.c_uint64(nbytes), ctypes.byref(ms))
+        return ms.value
 
     def mem_info(self):
         f, t = ctypes.c_uint64(), ctypes.c_uint64()

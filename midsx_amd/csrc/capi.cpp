@@ -1239,6 +1239,27 @@ int mdx_mem_info(mdx_ctx* c, uint64_t* free_bytes, uint64_t* total_bytes) {
   });
 }
 
+int mdx_copy_probe(mdx_ctx* c, uint64_t bytes, double* ms) {
+  return guard([&] {
+    need(c && ms, "NULL argument");
+    need(bytes >= 16 && bytes % 16 == 0, "bytes must be a positive multiple of 16");
+    set_device(c);
+    DBuf a, b;
+    a.alloc(bytes);
+    b.alloc(bytes);
+    hip_check(mdx_launch::zero(a.as<double>(), bytes / 8, c->stream), "zero");
+    const int blocks = c->num_cu * 8;
+    hip_check(mdx_launch::copy16(a.p, b.p, bytes / 16, blocks, c->stream), "copy");  // warm
+    hip_check(hipEventRecord(c->ev0, c->stream), "hipEventRecord");
+    hip_check(mdx_launch::copy16(a.p, b.p, bytes / 16, blocks, c->stream), "copy");
+    hip_check(hipEventRecord(c->ev1, c->stream), "hipEventRecord");
+    hip_check(hipStreamSynchronize(c->stream), "copy");
+    float t = 0;
+    hip_check(hipEventElapsedTime(&t, c->ev0, c->ev1), "hipEventElapsedTime");
+    *ms = t;
+  });
+}
+
 int mdx_host_alloc(mdx_ctx* c, uint64_t bytes, void** out) {
   return guard([&] {
     need(c && out, "NULL argument");

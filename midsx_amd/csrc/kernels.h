@@ -34,6 +34,8 @@ hipError_t kahn(const mdx_tables_dev* T, int slot, int n, const float* E, uint64
 hipError_t zero(double* p, size_t n, hipStream_t st);
 hipError_t zero_u32(uint32_t* p, size_t n, hipStream_t st);
 hipError_t zero_u64(unsigned long long* p, size_t n, hipStream_t st);
+// Streaming copy of n16 16-byte words (HBM reference for traffic counters).
+hipError_t copy16(const void* src, void* dst, size_t n16, int blocks, hipStream_t st);
 // (mean per history, relative standard error) per voxel from (sum, sum^2).
 hipError_t dose_f32(const double* ss, size_t nv, double n, float* out, hipStream_t st);
 

@@ -904,6 +904,13 @@ __global__ void k_dose_f32(const double2* __restrict__ ss, size_t nv, double n, 
   out[i] = make_float2((float)m, m > 0.0 ? (float)(sqrt(var) / m) : 0.0f);
 }
 
+// Plain streaming copy (16 B per lane per iteration): the HBM reference for
+// scaling traffic counters.
+__global__ void k_copy16(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 inline unsigned blocks_for(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
 }  // namespace
@@ -992,6 +999,13 @@ hipError_t dose_f32(const double* ss, size_t nv, double n, float* out, hipStream
   if (nv == 0) return hipSuccess;
   hipLaunchKernelGGL(k_dose_f32, dim3(blocks_for(nv, 256)), dim3(256), 0, st, reinterpret_cast<const double2*>(ss),
                      nv, n, reinterpret_cast<float2*>(out));
+  return hipGetLastError();
+}
+
+hipError_t copy16(const void* src, void* dst, size_t n16, int blocks, hipStream_t st) {
+  if (n16 == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(256), 0, st, static_cast<const float4*>(src),
+                     static_cast<float4*>(dst), n16);
   return hipGetLastError();
 }
 
