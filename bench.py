@@ -153,6 +153,62 @@ def served_stream(ctx, n_jobs, seed, load):
     }
 
 
+def ct_scan(ctx, n_views, scale, seed=4):
+    """CT: 512 x 512 x 256 phantom at 0.8 mm (water, lung, adipose, bone),
+    n_views views about its z axis, 120 kVp, a 512 x 64 detector per view,
+    per-view histories U[2e6, 2e7] x scale.  All views in one multi-job launch
+    of the queue kernel (views alias the phantom job; the phantom's voxel dose
+    sums over views) versus the plain kernel one view per launch."""
+    import midsx_amd as M
+    from midsx_amd import scenes
+    owner = ctx.job_create()
+    geo, views, angles = scenes.setup_ct(ctx, n_views=n_views, seed=seed)
+    n = [max(1, int(h * scale)) for h in scenes.ct_histories(n_views, seed)]
+    seeds = [10_000 + v for v in range(n_views)]
+    H = sum(n)
+    ctx.job_select(views[0])
+    tuned = ctx.autotune(probe_histories=1000000)
+    ctx.set_graphs(True)
+    ctx.run_jobs(views[:8], [100000] * 8, [1 << 40] * 8, seeds[:8])  # warm (capture)
+    for j in [owner] + views:
+        ctx.job_select(j)
+        ctx.reset_tallies()
+    t = time.perf_counter()
+    st = ctx.run_jobs(views, n, [0] * n_views, seeds)
+    q_s = time.perf_counter() - t
+    for j in [owner] + views:
+        ctx.job_select(j)
+        ctx.reset_tallies()
+    t = time.perf_counter()
+    pk = 0.0
+    psteps = 0
+    for v, nv, sv in zip(views, n, seeds):
+        ctx.job_select(v)
+        ps = ctx.run(nv, seed=sv, first=0, kernel=M.KERNEL_PLAIN)
+        pk += ps["kernel_ms"]
+        psteps += ps["steps"]
+    p_s = time.perf_counter() - t
+    ctx.set_graphs(False)
+    for j in views + [owner]:
+        ctx.job_release(j)
+    return {
+        "workload": "CT: 512x512x256 phantom at 0.8 mm, %d views, 120 kVp, 512x64 detector per view, "
+                    "histories per view U[2e6,2e7] x %g (total %d)" % (n_views, scale, H),
+        "value": H / q_s,
+        "unit": "histories/s",
+        "plain_value": H / p_s,
+        "speedup_vs_plain": p_s / q_s,
+        "queue_s": q_s,
+        "queue_kernel_ms": st["kernel_ms"],
+        "plain_s": p_s,
+        "plain_kernel_ms": pk,
+        "launches": {"queue": 1, "plain": n_views},
+        "steps_per_history": st["steps"] / H,
+        "plain_steps_per_history": psteps / H,
+        "queue_params": tuned,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,6 +227,10 @@ def main():
     ap.add_argument("--served-jobs", type=int, default=32)
     ap.add_argument("--served-seed", type=int, default=2026)
     ap.add_argument("--served-load", type=float, default=0.7)
+    ap.add_argument("--no-ct", action="store_true")
+    ap.add_argument("--ct-views", type=int, default=360)
+    ap.add_argument("--ct-scale", type=float, default=0.05,
+                    help="fraction of the CT workload's per-view histories (1.0 = the full scan, ~4e9 histories)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: sharding histories over several GPUs is not built in this "
@@ -247,6 +307,8 @@ def main():
         out["config"]["autotune_probe_ms"] = tuned["probe_ms"]
     if not args.no_served:
         out["served"] = served_stream(ctx, args.served_jobs, args.served_seed, args.served_load)
+    if not args.no_ct:
+        out["ct"] = ct_scan(ctx, args.ct_views, args.ct_scale)
     if args.copy_probe_mb:
         out["copy_probe"] = copy_probe(ctx, args.copy_probe_mb)
     if not args.no_cpu_baseline:

@@ -335,6 +335,11 @@ class Context:
         return a.value, b.value, c.value
 
     # ---------------------------------------------------------- jobs
+    @property
+    def job(self):
+        """The selected job."""
+        return self._job
+
     def job_create(self):
         j = ctypes.c_int()
         self._call(self.lib.mdx_job_create, self._h, ctypes.byref(j))

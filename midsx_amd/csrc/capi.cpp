@@ -942,7 +942,8 @@ int mdx_set_source(mdx_ctx* c, const mdx_source* s) {
       default:
         throw std::invalid_argument("unknown directionality");
     }
-    set_device_job(c);
+    const bool dev = c->device >= 0;  // a host-only context validates and keeps the source
+    if (dev) set_device_job(c);
     if (s->spectrum == MDX_SPEC_MONO) {
       need(s->energy > 0, "energy must be positive");
       d.energy = (float)s->energy;
@@ -959,8 +960,10 @@ int mdx_set_source(mdx_ctx* c, const mdx_source* s) {
         cdf[i] = (float)acc;
         en[i] = (float)s->bin_energy[i];
       }
-      c->J().d_cdf.upload(cdf.data(), cdf.size(), c->js);
-      c->J().d_energies.upload(en.data(), en.size(), c->js);
+      if (dev) {
+        c->J().d_cdf.upload(cdf.data(), cdf.size(), c->js);
+        c->J().d_energies.upload(en.data(), en.size(), c->js);
+      }
       d.n_bins = s->n_bins;
       d.cdf = c->J().d_cdf.as<float>();
       d.energies = c->J().d_energies.as<float>();
@@ -968,8 +971,10 @@ int mdx_set_source(mdx_ctx* c, const mdx_source* s) {
       throw std::invalid_argument("unknown spectrum kind");
     }
     c->J().src = d;
-    c->J().d_source.upload(&d, 1, c->js);
-    hip_check(hipStreamSynchronize(c->js), "upload source");
+    if (dev) {
+      c->J().d_source.upload(&d, 1, c->js);
+      hip_check(hipStreamSynchronize(c->js), "upload source");
+    }
     c->J().source_set = true;
   });
 }
@@ -979,7 +984,8 @@ int mdx_set_tallies(mdx_ctx* c, const mdx_tally* t, int n) {
     need(c && (t || n == 0), "NULL argument");
     need(n >= 0 && n <= MDX_MAX_TALLIES, "too many tallies");
     std::vector<std::unique_ptr<TallyHost>> v;
-    set_device_job(c);
+    const bool dev = c->device >= 0;  // a host-only context validates and keeps the tallies
+    if (dev) set_device_job(c);
     for (int i = 0; i < n; ++i) {
       auto h = std::make_unique<TallyHost>(&c->pool);
       h->t = t[i];
@@ -1002,14 +1008,18 @@ int mdx_set_tallies(mdx_ctx* c, const mdx_tally* t, int n) {
         throw std::invalid_argument("unknown tally kind");
       }
       if (q.kind != MDX_TALLY_AACUBOID && q.hist_bins > 0) need(q.hist_width > 0, "histogram width must be positive");
-      h->acc.alloc(MDX_TALLY_ACC_SIZE * sizeof(double));
-      if (q.kind != MDX_TALLY_AACUBOID && q.hist_bins > 0) h->hist.alloc((size_t)4 * q.hist_bins * sizeof(double));
-      if (q.kind == MDX_TALLY_IMAGE) h->image.alloc((size_t)3 * q.nx * q.ny * sizeof(double));
+      if (dev) {
+        h->acc.alloc(MDX_TALLY_ACC_SIZE * sizeof(double));
+        if (q.kind != MDX_TALLY_AACUBOID && q.hist_bins > 0) h->hist.alloc((size_t)4 * q.hist_bins * sizeof(double));
+        if (q.kind == MDX_TALLY_IMAGE) h->image.alloc((size_t)3 * q.nx * q.ny * sizeof(double));
+      }
       v.push_back(std::move(h));
     }
     c->J().tallies = std::move(v);
-    upload_tallies(c);
-    zero_tallies(c);
+    if (dev) {
+      upload_tallies(c);
+      zero_tallies(c);
+    }
   });
 }
 
@@ -1018,6 +1028,7 @@ int mdx_enable_voxel_dose(mdx_ctx* c, int enable) {
     need(c, "NULL context");
     need(c->J().domain_of < 0, "voxel dose belongs to the job that owns the domain");
     c->J().voxel_dose = enable != 0;
+    if (c->device < 0) return;
     set_device_job(c);
     if (c->J().domain_set) upload_domain(c);
   });

@@ -384,11 +384,49 @@ k_transport_plain(const mdx_tables_dev* __restrict__ T, const mdx_domain_dev* __
 // (prefix[] holds their offsets); each job has its own scene and tallies and
 // shares the tables.  Scalar tallies are pre-reduced in LDS for one job and
 // go to global FP64 atomics for several.
+// One scalar-tally event (count 1, value v, v^2) at acc[q][b].  The active
+// lanes of the wave that hit the same record are summed first (their values
+// are read with shuffles from active lanes only) and one of them adds the
+// sums: a job's tally records are few, so without this the atomics of a
+// whole wave would queue on one L2 line.
 __device__ __forceinline__ void event_global(double* acc, int q, int b, double v) {
   double* p = acc + (q * MDX_NBUCKET + b) * MDX_NFIELD;
-  atomicAdd(p, 1.0);
-  atomicAdd(p + 1, v);
-  atomicAdd(p + 2, v * v);
+  const int lane = threadIdx.x & 63;
+  const double v2 = v * v;
+  unsigned long long todo = __ballot(true);
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const unsigned long long lp = __shfl((unsigned long long)(uintptr_t)p, leader);
+    const unsigned long long same = __ballot((unsigned long long)(uintptr_t)p == lp);
+    todo &= ~same;
+    double s = 0.0, s2 = 0.0;
+    for (unsigned long long m = same; m; m &= m - 1) {
+      const int src = __ffsll((long long)m) - 1;
+      s += __shfl(v, src);
+      s2 += __shfl(v2, src);
+    }
+    if (lane == leader) {
+      atomicAdd(p, (double)__popcll(same));
+      atomicAdd(p + 1, s);
+      atomicAdd(p + 2, s2);
+    }
+  }
+}
+
+// atomicAdd(p, v) with the active lanes that hit the same address summed
+// first (spectrum bins: a monoenergetic beam puts every primary in one bin).
+__device__ __forceinline__ void wave_add(double* p, double v) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long todo = __ballot(true);
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const unsigned long long lp = __shfl((unsigned long long)(uintptr_t)p, leader);
+    const unsigned long long same = __ballot((unsigned long long)(uintptr_t)p == lp);
+    todo &= ~same;
+    double s = 0.0;
+    for (unsigned long long m = same; m; m &= m - 1) s += __shfl(v, __ffsll((long long)m) - 1);
+    if (lane == leader) atomicAdd(p, s);
+  }
 }
 
 template <bool MULTI>
@@ -460,7 +498,7 @@ __device__ void tallies_step(const mdx_tallies_dev* TL, double* acc, const Step&
       event_global(T.acc, MDX_SQ_NUMBER_OF_PHOTONS, b, 1.0);
       if (T.hist_bins > 0) {
         float fb = (r.E - T.hist_lo) * T.hist_inv_w;
-        if (fb >= 0.0f && fb < (float)T.hist_bins) atomicAdd(T.hist + b * T.hist_bins + (int)fb, 1.0);
+        if (fb >= 0.0f && fb < (float)T.hist_bins) wave_add(T.hist + b * T.hist_bins + (int)fb, 1.0);
       }
     }
   }

@@ -175,3 +175,82 @@ def dose_job_bytes(job):
     nv = int(np.prod(job["shape"]))
     ny_i, nx_i = job["image"]
     return nv * (1 + 16 + 4) + 24 * nx_i * ny_i + (1 << 16)
+
+
+# --------------------------------------------------------------------------- CT
+CT_MATERIALS = (AIR, WATER, LUNG, ADIPOSE, BONE)
+
+
+def ct_phantom(shape=(512, 512, 256), voxel_cm=0.08, seed=4):
+    """Water block with seeded ellipsoids of lung (2-3, semi-axes 3-7 cm),
+    adipose (2-5, 2-6 cm) and cortical bone (3-8, 0.5-3 cm)."""
+    specs = ((LUNG, (2, 3), (3.0, 7.0)), (ADIPOSE, (2, 5), (2.0, 6.0)), (BONE, (3, 8), (0.5, 3.0)))
+    return ellipsoid_phantom(shape, seed, specs=specs, voxel_cm=voxel_cm)
+
+
+def ct_geometry(shape=(512, 512, 256), voxel_cm=0.08, sod=60.0, odd=40.0, det_w=110.0, det_h=16.0):
+    """Axial CT: rotation axis = the phantom's z axis through its centre;
+    source at distance sod from the axis, a flat detector odd beyond it
+    (source-detector 100 cm), det_w x det_h cm.  Returns the domain size, the
+    phantom origin and the axis point."""
+    nx, ny, nz = shape
+    lx, ly, lz = nx * voxel_cm, ny * voxel_cm, nz * voxel_cm
+    half = max(sod, float(np.hypot(odd, det_w / 2))) + 2.0
+    dim = (2 * half, 2 * half, max(lz, det_h) + 4.0)
+    axis = (half, half, dim[2] / 2)
+    origin = (half - lx / 2, half - ly / 2, axis[2] - lz / 2)
+    return dict(dim=dim, origin=origin, axis=axis, sod=sod, odd=odd, det_w=det_w, det_h=det_h)
+
+
+def setup_ct_view(sim, geo, angle_deg, kvp=120.0, pixels=(512, 64)):
+    """Source and detector of one view at angle_deg about the axis: a point
+    source, a cone beam aimed over the detector (RectangularIsotropic over
+    its rectangle) and an energy-fluence image of pixels = (columns, rows);
+    edge1 runs along z (rows), edge2 along the detector's columns, so that
+    edge1 x edge2 points from the source to the detector."""
+    th = np.radians(angle_deg)
+    c, s = float(np.cos(th)), float(np.sin(th))
+    ax = geo["axis"]
+    src = (ax[0] + geo["sod"] * c, ax[1] + geo["sod"] * s, ax[2])
+    dc = (ax[0] - geo["odd"] * c, ax[1] - geo["odd"] * s, ax[2])
+    u = (-s, c, 0.0)
+    W, H = geo["det_w"], geo["det_h"]
+    p0 = (dc[0] - W / 2 * u[0], dc[1] - W / 2 * u[1], dc[2] - H / 2)
+    e1 = (0.0, 0.0, H)
+    e2 = (W * u[0], W * u[1], 0.0)
+    e, w = kramers(kvp)
+    sim.set_source(direction=DIR_RECT_ISOTROPIC, position=src, a=p0, b=e2, c=e1, spectrum=(e, w))
+    cols, rows = pixels
+    sim.set_tallies([dict(kind=TALLY_IMAGE, p0=p0, p1=e1, p2=e2, nx=rows, ny=cols)])
+
+
+def setup_ct(ctx, n_views=360, shape=(512, 512, 256), voxel_cm=0.08, seed=4, kvp=120.0, pixels=(512, 64),
+             ids=None):
+    """The CT scan on a Context: the current job holds the phantom (and its
+    FP64 voxel dose, accumulated over all views); one job per view aliases
+    it and carries its own source and detector image.  Returns (geometry,
+    view job ids, view angles)."""
+    geo = ct_geometry(shape, voxel_cm)
+    owner = ctx.job
+    ctx.set_domain(geo["dim"], AIR)
+    if ids is None:
+        ids = ct_phantom(shape, voxel_cm, seed)
+    ctx.add_grid(ids, (voxel_cm,) * 3, geo["origin"])
+    ctx.enable_voxel_dose(True)
+    ctx.set_tallies([])
+    views, angles = [], []
+    for v in range(n_views):
+        a = 360.0 * v / n_views
+        j = ctx.job_create()
+        ctx.job_alias_domain(owner)
+        setup_ct_view(ctx, geo, a, kvp, pixels)
+        views.append(j)
+        angles.append(a)
+    ctx.job_select(owner)
+    return geo, views, angles
+
+
+def ct_histories(n_views, seed=4, lo=2e6, hi=2e7):
+    """Per-view history counts, uniform in [lo, hi] (seeded)."""
+    rng = np.random.default_rng(seed)
+    return rng.integers(int(lo), int(hi) + 1, n_views)

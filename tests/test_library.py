@@ -133,3 +133,30 @@ def test_job_alias_domain_host():
     c.job_release(d)
     c.job_release(0)
     c.close()
+
+
+def test_ct_scene_host():
+    """The CT set-up on a host-only context: view jobs alias the phantom job,
+    each view's detector faces its source, and views cover the circle."""
+    from midsx_amd import scenes
+    c = M.Context(-1)
+    geo, views, angles = scenes.setup_ct(c, n_views=4, shape=(16, 16, 8), voxel_cm=0.5, seed=1)
+    assert len(views) == 4 and angles == [0.0, 90.0, 180.0, 270.0]
+    dim = geo["dim"]
+    for v, a in zip(views, angles):
+        c.job_select(v)
+        assert c.num_grids() == 1  # the phantom job's grid
+        t = c.tallies[0]
+        n = np.cross(t["p1"], t["p2"])
+        th = np.radians(a)
+        travel = -np.array([np.cos(th), np.sin(th), 0.0])
+        assert np.dot(n, travel) > 0.99 * np.linalg.norm(n)
+        # source and detector inside the domain
+        src = np.array(geo["axis"]) - geo["sod"] * travel
+        assert np.all(src > 0) and np.all(src < dim)
+        corner = np.array(t["p0"])
+        for q in (corner, corner + t["p1"], corner + t["p2"], corner + t["p1"] + t["p2"]):
+            assert np.all(q > 0) and np.all(q < dim)
+    h = scenes.ct_histories(360)
+    assert h.min() >= 2e6 and h.max() <= 2e7 and 3e9 < h.sum() < 5e9
+    c.close()
