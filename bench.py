@@ -209,6 +209,43 @@ def ct_scan(ctx, n_views, scale, seed=4):
     }
 
 
+def precision_study(ctx, n_hist, step=10 ** 8):
+    """Config 3: the slab phantom (40 x 40 x 10 water voxels, 60 keV pencil
+    beam), n_hist histories on the queue kernel with the voxel dose
+    accumulated in FP64, FP32 and compensated FP32 on identical RNG streams;
+    per-voxel relative error of each against the FP64 run."""
+    import midsx_amd as M
+    from midsx_amd import scenes
+    job = ctx.job_create()
+    scenes.setup_slab(ctx)
+    out = {}
+    res = {}
+    for name, mode in (("fp64", M.DOSE_F64), ("fp32", M.DOSE_F32), ("fp32_compensated", M.DOSE_F32C)):
+        ctx.set_dose_precision(mode)
+        ctx.reset_tallies()
+        ms = 0.0
+        done = 0
+        while done < n_hist:
+            k = min(step, n_hist - done)
+            ms += ctx.run(k, seed=33, first=done, kernel=M.KERNEL_QUEUE)["kernel_ms"]
+            done += k
+        s, s2, cnt = ctx.voxel_dose(0)
+        out[name] = (s, s2, cnt)
+        res[name] = {"kernel_s": ms / 1e3, "histories_per_s": n_hist / (ms / 1e3)}
+    s64, q64, c64 = out["fp64"]
+    roi = c64 >= 1000
+    for name in ("fp32", "fp32_compensated"):
+        s, s2, cnt = out[name]
+        e = np.abs(s[roi] / s64[roi] - 1)
+        e2 = np.abs(s2[roi] / q64[roi] - 1)
+        res[name].update({"rel_err_sum_max": float(e.max()), "rel_err_sum_median": float(np.median(e)),
+                          "rel_err_sum2_max": float(e2.max()), "same_deposits": bool(np.array_equal(cnt, c64))})
+    ctx.job_release(job)
+    return {"workload": "slab 40x40x10 water voxels, 60 keV beam, %d histories per mode, identical RNG streams; "
+                        "errors over the %d voxels with >= 1000 deposits" % (n_hist, int(roi.sum())),
+            "modes": res}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -228,6 +265,8 @@ def main():
     ap.add_argument("--served-seed", type=int, default=2026)
     ap.add_argument("--served-load", type=float, default=0.7)
     ap.add_argument("--no-ct", action="store_true")
+    ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--precision-histories", type=int, default=10 ** 9)
     ap.add_argument("--ct-views", type=int, default=360)
     ap.add_argument("--ct-scale", type=float, default=0.05,
                     help="fraction of the CT workload's per-view histories (1.0 = the full scan, ~4e9 histories)")
@@ -309,6 +348,8 @@ def main():
         out["served"] = served_stream(ctx, args.served_jobs, args.served_seed, args.served_load)
     if not args.no_ct:
         out["ct"] = ct_scan(ctx, args.ct_views, args.ct_scale)
+    if not args.no_precision:
+        out["precision"] = precision_study(ctx, args.precision_histories)
     if args.copy_probe_mb:
         out["copy_probe"] = copy_probe(ctx, args.copy_probe_mb)
     if not args.no_cpu_baseline:

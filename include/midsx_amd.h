@@ -224,6 +224,16 @@ int mdx_run_jobs(mdx_ctx* ctx, int n_jobs, const int* jobs, const uint64_t* firs
  * has no volume (AACuboid) tally, whose per-segment counts (volume_tally.cpp:
  * 5-27) depend on where steps start. */
 int mdx_set_flights(mdx_ctx* ctx, int enable);
+/* Voxel-dose accumulation precision of the selected job's grids (it must own
+ * its domain): MDX_DOSE_F64 (default, FP64 atomics), MDX_DOSE_F32 (FP32
+ * atomics), MDX_DOSE_F32C (FP32 atomics plus a second FP32 that collects the
+ * exact rounding error of every add).  The reference keeps FP64 VectorValue
+ * sums per voxel (voxel.h:14); mdx_get_voxel_dose returns FP64 in every
+ * mode.  Resets the grids' dose. */
+#define MDX_DOSE_F64 0
+#define MDX_DOSE_F32 1
+#define MDX_DOSE_F32C 2
+int mdx_set_dose_precision(mdx_ctx* ctx, int mode);
 /* Run mdx_run_jobs rounds as a replayed hipGraph (round buffer copy, counter
  * zeroing, queue kernel), at most 1024 jobs per round. */
 int mdx_set_graphs(mdx_ctx* ctx, int enable);

@@ -24,6 +24,7 @@ SPEC_MONO, SPEC_POLY = 0, 1
 TALLY_AACUBOID, TALLY_DISC, TALLY_RECT, TALLY_IMAGE = 0, 1, 2, 3
 XS_TOTAL, XS_COHERENT, XS_INCOHERENT, XS_PHOTOELECTRIC, SF_INCOHERENT, FF_COHERENT = 0, 1, 2, 3, 4, 5
 KERNEL_PLAIN, KERNEL_QUEUE = 0, 1
+DOSE_F64, DOSE_F32, DOSE_F32C = 0, 1, 2
 NQ, NBUCKET, NFIELD = 4, 4, 3
 STAT_NAMES = ["histories", "steps", "real", "rayleigh", "compton", "photo", "kahn_trials", "kahn_s_tests",
               "deposits", "surface_hits", "escapes", "reserved"]
@@ -42,7 +43,7 @@ EXPORTS = [
     "mdx_tables_device_bytes", "mdx_run_jobs", "mdx_set_queue_params", "mdx_get_queue_params",
     "mdx_autotune", "mdx_job_alias_domain", "mdx_set_graphs",
     "mdx_run_jobs_async", "mdx_wait", "mdx_mem_info", "mdx_host_alloc", "mdx_host_free",
-    "mdx_set_flights", "mdx_get_voxel_dose_f32", "mdx_stage_voxel_dose_f32", "mdx_copy_probe",
+    "mdx_set_flights", "mdx_set_dose_precision", "mdx_get_voxel_dose_f32", "mdx_stage_voxel_dose_f32", "mdx_copy_probe",
 ]
 
 
@@ -360,6 +361,11 @@ class Context:
     def job_alias_domain(self, src):
         """The selected job uses job src's domain (grids, voxel dose)."""
         self._call(self.lib.mdx_job_alias_domain, self._h, ctypes.c_int(src))
+
+    def set_dose_precision(self, mode):
+        """DOSE_F64, DOSE_F32 or DOSE_F32C accumulation of the voxel dose
+        (selected job, which must own its domain); resets the dose."""
+        self._call(self.lib.mdx_set_dose_precision, self._h, ctypes.c_int(mode))
 
     def set_flights(self, on=True):
         """Background free flights for the selected job (queue kernel)."""

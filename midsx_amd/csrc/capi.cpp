@@ -154,7 +154,8 @@ struct GridHost {
   int dims[3];
   double spacing[3], origin[3];
   bool present[256] = {false};  // material IDs the grid holds
-  DBuf d_ids, d_dose, d_dose_n;
+  DBuf d_ids, d_dose, d_dose_n;  // d_dose holds the sums in the job's MDX_DOSE_* layout
+  int dose_mode = -1;            // layout d_dose was set up for
   explicit GridHost(Pool* pl) : d_ids(pl), d_dose(pl), d_dose_n(pl) {}
   size_t voxels() const { return (size_t)dims[0] * dims[1] * dims[2]; }
 };
@@ -177,6 +178,7 @@ struct Job {
   int background_id = -1;
   std::vector<std::unique_ptr<GridHost>> grids;
   bool voxel_dose = false;
+  int dose_mode = MDX_DOSE_F64;  // accumulation precision of the voxel dose
   DBuf d_bg, d_domain;
   bool source_set = false;
   mdx_source_dev src{};
@@ -354,17 +356,23 @@ void upload_domain(mdx_ctx* c) {
     }
     d.ids = G.d_ids.as<uint8_t>();
     if (c->J().voxel_dose) {
-      if (G.d_dose.bytes != nv * 16) {
-        G.d_dose.alloc(nv * 16);
+      const int mode = c->J().dose_mode;
+      const size_t per = mode == MDX_DOSE_F32 ? 8 : 16;
+      if (G.d_dose.bytes != nv * per || G.dose_mode != mode || !G.d_dose_n.p) {
+        G.d_dose.alloc(nv * per);
         G.d_dose_n.alloc(nv * 4);
-        hip_check(mdx_launch::zero(G.d_dose.as<double>(), 2 * nv, c->js), "zero dose");
+        G.dose_mode = mode;
+        hip_check(mdx_launch::zero(G.d_dose.as<double>(), nv * per / 8, c->js), "zero dose");
         hip_check(mdx_launch::zero_u32(G.d_dose_n.as<uint32_t>(), nv, c->js), "zero dose count");
       }
-      d.dose = G.d_dose.as<double>();
+      d.dose_mode = mode;
+      d.dose = mode == MDX_DOSE_F64 ? G.d_dose.as<double>() : nullptr;
+      d.dose32 = mode == MDX_DOSE_F64 ? nullptr : G.d_dose.as<float>();
       d.dose_n = G.d_dose_n.as<uint32_t>();
     } else {
       G.d_dose.release();
       G.d_dose_n.release();
+      G.dose_mode = -1;
     }
   }
   if (!c->J().d_bg.p) {
@@ -1176,6 +1184,18 @@ int mdx_job_alias_domain(mdx_ctx* c, int src) {
   });
 }
 
+int mdx_set_dose_precision(mdx_ctx* c, int mode) {
+  return guard([&] {
+    need(c, "NULL context");
+    need(mode == MDX_DOSE_F64 || mode == MDX_DOSE_F32 || mode == MDX_DOSE_F32C, "unknown dose precision");
+    need(c->J().domain_of < 0, "voxel dose belongs to the job that owns the domain");
+    c->J().dose_mode = mode;
+    if (c->device < 0) return;
+    set_device_job(c);
+    if (c->J().domain_set) upload_domain(c);
+  });
+}
+
 int mdx_set_flights(mdx_ctx* c, int enable) {
   return guard([&] {
     need(c, "NULL context");
@@ -1323,7 +1343,23 @@ int mdx_get_voxel_dose(mdx_ctx* c, int grid, double* ss, uint32_t* count) {
     need(D.voxel_dose, "voxel dose is not enabled");
     set_device_nowait(c);
     GridHost& g = *D.grids[grid];
-    if (ss) read_back(c, ss, g.d_dose.p, g.d_dose.bytes, "read dose");
+    const size_t nv = g.voxels();
+    if (ss && g.dose_mode == MDX_DOSE_F64) {
+      read_back(c, ss, g.d_dose.p, g.d_dose.bytes, "read dose");
+    } else if (ss) {  // FP32 layouts, widened: (sum, sum^2) or (sum + error, sum^2 + error)
+      const int w = g.dose_mode == MDX_DOSE_F32 ? 2 : 4;
+      std::vector<float> f(nv * w);
+      read_back(c, f.data(), g.d_dose.p, f.size() * sizeof(float), "read dose");
+      for (size_t i = 0; i < nv; ++i) {
+        if (w == 2) {
+          ss[2 * i] = f[2 * i];
+          ss[2 * i + 1] = f[2 * i + 1];
+        } else {
+          ss[2 * i] = (double)f[4 * i] + (double)f[4 * i + 1];
+          ss[2 * i + 1] = (double)f[4 * i + 2] + (double)f[4 * i + 3];
+        }
+      }
+    }
     if (count) read_back(c, count, g.d_dose_n.p, g.d_dose_n.bytes, "read dose n");
   });
 }
@@ -1337,6 +1373,7 @@ int mdx_get_voxel_dose_f32(mdx_ctx* c, int grid, uint64_t n_histories, float* ou
     need(D.voxel_dose, "voxel dose is not enabled");
     set_device_nowait(c);
     GridHost& g = *D.grids[grid];
+    need(g.dose_mode == MDX_DOSE_F64, "FP32 dose reads need FP64 accumulation (MDX_DOSE_F64)");
     const size_t nv = g.voxels();
     if (D.staged_grid == grid && D.staged_n == n_histories) {
       hip_check(hipStreamWaitEvent(c->io_stream, D.ev_stage, 0), "wait staged dose");
@@ -1362,6 +1399,7 @@ int mdx_stage_voxel_dose_f32(mdx_ctx* c, int grid, uint64_t n_histories) {
     need(D.voxel_dose, "voxel dose is not enabled");
     set_device_nowait(c);
     GridHost& g = *D.grids[grid];
+    need(g.dose_mode == MDX_DOSE_F64, "FP32 dose reads need FP64 accumulation (MDX_DOSE_F64)");
     const size_t nv = g.voxels();
     if (!D.d_stage.pool) D.d_stage.pool = &c->pool;
     D.d_stage.ensure(nv * 8);

@@ -72,6 +72,11 @@ struct mdx_tables_dev {
   uint8_t id_to_slot[256];  // material ID -> slot, 255 = unknown
 };
 
+// Voxel-dose accumulation precision.
+#define MDX_DOSE_F64 0   // FP64 atomics
+#define MDX_DOSE_F32 1   // FP32 atomics
+#define MDX_DOSE_F32C 2  // FP32 atomics with the rounding error of each add kept in a second FP32
+
 struct mdx_grid_dev {
   float origin[3];
   float spacing[3];
@@ -79,8 +84,11 @@ struct mdx_grid_dev {
   float dim[3];
   int32_t n[3];
   const uint8_t* ids;  // x-fastest: i + j*nx + k*nx*ny (voxel_grid.cpp:119-121)
-  double* dose;        // per voxel: sum, sum^2 interleaved (nullptr = off)
+  double* dose;        // MDX_DOSE_F64: per voxel sum, sum^2 interleaved (nullptr = off)
+  float* dose32;       // MDX_DOSE_F32: (sum, sum^2); MDX_DOSE_F32C: (sum, its error, sum^2, its error)
   uint32_t* dose_n;    // per voxel deposit count (nullptr = off)
+  int32_t dose_mode;   // MDX_DOSE_*
+  int32_t pad;
 };
 
 struct mdx_domain_dev {

@@ -77,6 +77,41 @@ __device__ void flush_hist(const mdx_tallies_dev* TL, const int* off, double* lh
 
 // surface_tally.cpp:16-34 (+ DiscSurfaceTally :63-69, RectangularSurfaceTally
 // :100-112) and volume_tally.cpp:5-91 for one recorded step.
+// FP32 atomic add that keeps the rounding error: the atomic returns the old
+// value, so the stored sum is fl(old + v) and TwoSum gives the exact error,
+// which goes to p[1].  hi + lo then carries the sum to the precision of the
+// error sum lo, itself an FP32 sum of values below half an ulp of hi.
+__device__ __forceinline__ void compensated_add(float* p, float v) {
+  const float old = atomicAdd(p, v);
+  const float s = __fadd_rn(old, v);
+  const float bb = __fsub_rn(s, old);
+  const float err = __fadd_rn(__fsub_rn(old, __fsub_rn(s, bb)), __fsub_rn(v, bb));
+  if (err != 0.0f) atomicAdd(p + 1, err);
+}
+
+// One energy deposit in voxel lin of grid G (physics_engine.cpp:70-75,
+// 160-168) in the grid's accumulation precision.
+__device__ __forceinline__ void voxel_deposit(const mdx_grid_dev& G, uint64_t lin, float dep) {
+  if (G.dose_mode == MDX_DOSE_F32) {
+    if (G.dose32) {
+      atomicAdd(G.dose32 + 2 * lin, dep);
+      atomicAdd(G.dose32 + 2 * lin + 1, dep * dep);
+    }
+  } else if (G.dose_mode == MDX_DOSE_F32C) {
+    if (G.dose32) {
+      compensated_add(G.dose32 + 4 * lin, dep);
+      const float q = __fmul_rn(dep, dep);  // dep^2 = q + ql exactly (never contracted into an fma)
+      const float ql = __builtin_fmaf(dep, dep, -q);
+      compensated_add(G.dose32 + 4 * lin + 2, q);
+      if (ql != 0.0f) atomicAdd(G.dose32 + 4 * lin + 3, ql);
+    }
+  } else if (G.dose) {
+    atomicAdd(G.dose + 2 * lin, (double)dep);
+    atomicAdd(G.dose + 2 * lin + 1, (double)dep * (double)dep);
+  }
+  if (G.dose_n) atomicAdd(G.dose_n + lin, 1u);
+}
+
 __device__ void process_tallies(const mdx_tallies_dev* TL, double* acc, const Step& r, uint32_t& surf_hits,
                                 double* lhist, const int* hoff) {
   for (int t = 0; t < TL->n; ++t) {
@@ -331,21 +366,13 @@ k_transport_plain(const mdx_tables_dev* __restrict__ T, const mdx_domain_dev* __
         r.dep = dep;
         if (dep != 0.0f) {  // physics_engine.cpp:70-75, 205-208
           ++c_dep;
-          double* dd;
-          uint32_t* dn;
           if (g >= 0) {
-            dd = D->grids[g].dose ? D->grids[g].dose + 2 * lin : nullptr;
-            dn = D->grids[g].dose_n ? D->grids[g].dose_n + lin : nullptr;
-          } else {
-            dd = D->bg_dose;
-            dn = nullptr;
+            voxel_deposit(D->grids[g], lin, dep);
+          } else if (D->bg_dose) {
+            atomicAdd(D->bg_dose, (double)dep);
+            atomicAdd(D->bg_dose + 1, (double)dep * (double)dep);
+            atomicAdd(D->bg_dose + 2, 1.0);
           }
-          if (dd) {
-            atomicAdd(dd, (double)dep);
-            atomicAdd(dd + 1, (double)dep * (double)dep);
-            if (g < 0) atomicAdd(dd + 2, 1.0);
-          }
-          if (dn) atomicAdd(dn, 1u);
         }
       }
       process_tallies(TL, acc, r, c_surf, lhist, hoff);
@@ -798,12 +825,7 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
         if (dep != 0.0f) {
           ++c_dep;
           if (qg >= 0) {
-            const mdx_grid_dev& G = D->grids[qg];
-            if (G.dose) {
-              atomicAdd(G.dose + 2 * qlin, (double)dep);
-              atomicAdd(G.dose + 2 * qlin + 1, (double)dep * (double)dep);
-            }
-            if (G.dose_n) atomicAdd(G.dose_n + qlin, 1u);
+            voxel_deposit(D->grids[qg], qlin, dep);
           } else if (D->bg_dose) {
             atomicAdd(D->bg_dose, (double)dep);
             atomicAdd(D->bg_dose + 1, (double)dep * (double)dep);
