@@ -224,6 +224,11 @@ int mdx_run_jobs(mdx_ctx* ctx, int n_jobs, const int* jobs, const uint64_t* firs
  * has no volume (AACuboid) tally, whose per-segment counts (volume_tally.cpp:
  * 5-27) depend on where steps start. */
 int mdx_set_flights(mdx_ctx* ctx, int enable);
+/* Where mdx_build_tables builds the Rayleigh RITA tables (FP64 either way):
+ * on the device, one wavefront per energy (default for device contexts), or
+ * on the host threads (always for host-only contexts).  The adaptive scheme
+ * of probability_dist.cpp:62-71,121-175,267-287 in both. */
+int mdx_set_rita_builder(mdx_ctx* ctx, int on_device);
 /* Voxel-dose accumulation precision of the selected job's grids (it must own
  * its domain): MDX_DOSE_F64 (default, FP64 atomics), MDX_DOSE_F32 (FP32
  * atomics), MDX_DOSE_F32C (FP32 atomics plus a second FP32 that collects the

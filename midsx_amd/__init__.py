@@ -43,7 +43,7 @@ EXPORTS = [
     "mdx_tables_device_bytes", "mdx_run_jobs", "mdx_set_queue_params", "mdx_get_queue_params",
     "mdx_autotune", "mdx_job_alias_domain", "mdx_set_graphs",
     "mdx_run_jobs_async", "mdx_wait", "mdx_mem_info", "mdx_host_alloc", "mdx_host_free",
-    "mdx_set_flights", "mdx_set_dose_precision", "mdx_get_voxel_dose_f32", "mdx_stage_voxel_dose_f32", "mdx_copy_probe",
+    "mdx_set_flights", "mdx_set_rita_builder", "mdx_set_dose_precision", "mdx_get_voxel_dose_f32", "mdx_stage_voxel_dose_f32", "mdx_copy_probe",
 ]
 
 
@@ -366,6 +366,10 @@ class Context:
         """DOSE_F64, DOSE_F32 or DOSE_F32C accumulation of the voxel dose
         (selected job, which must own its domain); resets the dose."""
         self._call(self.lib.mdx_set_dose_precision, self._h, ctypes.c_int(mode))
+
+    def set_rita_builder(self, on_device=True):
+        """Build the RITA tables on the device (default) or the host, both FP64."""
+        self._call(self.lib.mdx_set_rita_builder, self._h, ctypes.c_int(1 if on_device else 0))
 
     def set_flights(self, on=True):
         """Background free flights for the selected job (queue kernel)."""

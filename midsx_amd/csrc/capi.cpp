@@ -223,6 +223,7 @@ struct mdx_ctx {
   mdx::Packed packed;
   DBuf d_xs, d_nodes, d_hash, d_rE, d_rinv, d_roff, d_rrows, d_tables;
   bool tables_ready = false;
+  bool rita_on_device = true;  // RITA tables built by k_rita_build (FP64) instead of on the host
 
   // scenes
   std::vector<std::unique_ptr<Job>> jobs;
@@ -397,6 +398,50 @@ void check_materials(mdx_ctx* c) {
   }
 }
 
+// The RITA tables of one material built on the device in FP64 (k_rita_build)
+// and kept in its host tables, whose packing into FP32 rows follows.
+void build_rita_device(mdx_ctx* c, mdx::MaterialTables& m) {
+  mdx::Rita& R = m.rita;
+  R.energies = m.incoh.x;  // material_data.cpp:70: the incoherent-CS grid
+  const int ne = (int)R.energies.size();
+  const int cap = mdx_launch::rita_cap();
+  DBuf lx, ly, en, xs, ys, as, bs, cnt;
+  lx.upload(m.ff_i.lx.data(), m.ff_i.lx.size(), c->stream);
+  ly.upload(m.ff_i.ly.data(), m.ff_i.ly.size(), c->stream);
+  en.upload(R.energies.data(), R.energies.size(), c->stream);
+  const size_t cells = (size_t)ne * cap;
+  xs.alloc(cells * 8);
+  ys.alloc(cells * 8);
+  as.alloc(cells * 8);
+  bs.alloc(cells * 8);
+  cnt.alloc((size_t)ne * 4);
+  hip_check(mdx_launch::rita_build(lx.as<double>(), ly.as<double>(), (int)m.ff_i.lx.size(), mdx::alpha_inv_angstrom(),
+                                   1.0 / mdx::kElectronRestMass, en.as<double>(), ne, xs.as<double>(), ys.as<double>(),
+                                   as.as<double>(), bs.as<double>(), cnt.as<int>(), c->stream),
+            "RITA build");
+  std::vector<double> hx(cells), hy(cells), ha(cells), hb(cells);
+  std::vector<int> hc(ne);
+  hip_check(hipMemcpyAsync(hx.data(), xs.p, cells * 8, hipMemcpyDeviceToHost, c->stream), "read RITA");
+  hip_check(hipMemcpyAsync(hy.data(), ys.p, cells * 8, hipMemcpyDeviceToHost, c->stream), "read RITA");
+  hip_check(hipMemcpyAsync(ha.data(), as.p, cells * 8, hipMemcpyDeviceToHost, c->stream), "read RITA");
+  hip_check(hipMemcpyAsync(hb.data(), bs.p, cells * 8, hipMemcpyDeviceToHost, c->stream), "read RITA");
+  hip_check(hipMemcpyAsync(hc.data(), cnt.p, (size_t)ne * 4, hipMemcpyDeviceToHost, c->stream), "read RITA");
+  hip_check(hipStreamSynchronize(c->stream), "RITA build");
+  R.x.assign(ne, {});
+  R.y.assign(ne, {});
+  R.a.assign(ne, {});
+  R.b.assign(ne, {});
+  for (int ie = 0; ie < ne; ++ie) {
+    const int n = hc[ie];
+    if (n < 2) throw std::runtime_error("RITA table did not converge within " + std::to_string(cap) + " points");
+    const size_t o = (size_t)ie * cap;
+    R.x[ie].assign(hx.begin() + o, hx.begin() + o + n);
+    R.y[ie].assign(hy.begin() + o, hy.begin() + o + n);
+    R.a[ie].assign(ha.begin() + o, ha.begin() + o + n - 1);
+    R.b[ie].assign(hb.begin() + o, hb.begin() + o + n - 1);
+  }
+}
+
 void build_and_upload(mdx_ctx* c, const std::vector<int>& ids) {
   c->ensure_registry();
   need(!ids.empty(), "no materials given");
@@ -406,12 +451,15 @@ void build_and_upload(mdx_ctx* c, const std::vector<int>& ids) {
     if (std::find(uniq.begin(), uniq.end(), id) == uniq.end()) uniq.push_back(id);
   int threads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
   std::vector<std::unique_ptr<mdx::MaterialTables>> mats;
+  const bool rita_dev = c->device >= 0 && c->rita_on_device;
+  if (rita_dev) set_device(c);
   for (int id : uniq) {
     auto it = c->registry.find(id);
     if (it == c->registry.end())
       throw std::runtime_error("material ID " + std::to_string(id) + " not in the registry");
     mats.push_back(std::make_unique<mdx::MaterialTables>(
-        mdx::build_material(it->second, c->data_dir + "/epdl", true, threads)));
+        mdx::build_material(it->second, c->data_dir + "/epdl", !rita_dev, threads)));
+    if (rita_dev) build_rita_device(c, *mats.back());
   }
   std::vector<const mdx::MaterialTables*> mp;
   for (auto& m : mats) mp.push_back(m.get());
@@ -1193,6 +1241,13 @@ int mdx_set_dose_precision(mdx_ctx* c, int mode) {
     if (c->device < 0) return;
     set_device_job(c);
     if (c->J().domain_set) upload_domain(c);
+  });
+}
+
+int mdx_set_rita_builder(mdx_ctx* c, int on_device) {
+  return guard([&] {
+    need(c, "NULL context");
+    c->rita_on_device = on_device != 0;
   });
 }
 

@@ -34,6 +34,13 @@ hipError_t kahn(const mdx_tables_dev* T, int slot, int n, const float* E, uint64
 hipError_t zero(double* p, size_t n, hipStream_t st);
 hipError_t zero_u32(uint32_t* p, size_t n, hipStream_t st);
 hipError_t zero_u64(unsigned long long* p, size_t n, hipStream_t st);
+// FP64 RITA tables of the Rayleigh angular distribution, one wave per energy:
+// per energy ie, grid xs, CDF ys (count[ie] points, -1 if the cap was hit)
+// and parameters as, bs (count - 1) at offset ie * rita_cap().
+hipError_t rita_build(const double* lx, const double* ly, int n_ff, double alpha, double inv_mc2,
+                      const double* energies, int ne, double* xs, double* ys, double* as, double* bs, int* count,
+                      hipStream_t st);
+int rita_cap();
 // Streaming copy of n16 16-byte words (HBM reference for traffic counters).
 hipError_t copy16(const void* src, void* dst, size_t n16, int blocks, hipStream_t st);
 // (mean per history, relative standard error) per voxel from (sum, sum^2).

@@ -888,6 +888,159 @@ k_transport_queue_round(const mdx_tables_dev* __restrict__ T, const char* __rest
   transport_queue_body<true>(T, jobs, prefix, n_jobs, total, work, counters, refill_min, interact_min);
 }
 
+// ------------------------------------------------------------------ RITA table build (FP64)
+// The adaptive PENELOPE RITA tables of the Rayleigh angular distribution
+// (probability_dist.cpp:62-71,121-175,267-287), one wave per energy of the
+// incoherent-CS grid: the pdf (1 + mu^2) FF(x)^2 with FF log-log linear
+// (interpolators.cpp:71-76 over log10), normalised by 1001-point Simpson;
+// from 10 equal intervals, every interval whose error (51-point rule)
+// exceeds 1e-4 is halved until none does.  Lanes take intervals; the
+// running sums (normalisation, CDF) are added by one lane in the order the
+// CPU path adds them.
+constexpr int kRitaCap = 256;  // grid points per energy
+
+struct RitaFF {
+  const double* lx;  // log10 x of the FF table
+  const double* ly;  // log10 FF
+  int n;
+  double alpha;      // x = alpha * (E / mc^2) * sqrt(1 - mu), 1/Angstrom
+  double inv_mc2;
+};
+
+__device__ double ff_loglog(const RitaFF& F, double x) {
+#pragma clang fp contract(off)
+  const double l = log10(x);
+  if (l <= F.lx[0]) return pow(10.0, F.ly[0]);
+  if (l >= F.lx[F.n - 1]) return pow(10.0, F.ly[F.n - 1]);
+  int lo = 0, hi = F.n;  // upper_bound
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (F.lx[mid] <= l) lo = mid + 1; else hi = mid;
+  }
+  const double x1 = F.lx[lo - 1], x2 = F.lx[lo], y1 = F.ly[lo - 1], y2 = F.ly[lo];
+  const double slope = (y2 - y1) / (x2 - x1);
+  return pow(10.0, y1 + slope * (l - x1));
+}
+
+__device__ __forceinline__ double rayleigh_pdf(const RitaFF& F, double E, double mu) {
+#pragma clang fp contract(off)
+  const double x = F.alpha * (E * F.inv_mc2) * sqrt(1.0 - mu);
+  const double ff = ff_loglog(F, x);
+  return (1.0 + mu * mu) * ff * ff;
+}
+
+__device__ __forceinline__ double rita_pnum(double x, double x0, double x1, double y0, double y1, double a,
+                                           double b) {
+#pragma clang fp contract(off)
+  const double tau = (x - x0) / (x1 - x0);
+  double eta = 0.0;
+  if (tau != 0.0) {
+    const double c = 1.0 + a + b - a * tau;
+    eta = c / (2.0 * b * tau) * (1.0 - sqrt(1.0 - 4.0 * b * tau * tau / (c * c)));
+  }
+  const double q = 1.0 + a * eta + b * eta * eta;
+  return q * q / ((1.0 + a + b) * (1.0 - b * eta * eta)) * (y1 - y0) / (x1 - x0);
+}
+
+__global__ void __launch_bounds__(64) k_rita_build(RitaFF F, const double* __restrict__ energies,
+                                                    double* __restrict__ xs, double* __restrict__ ys,
+                                                    double* __restrict__ as, double* __restrict__ bs,
+                                                    int* __restrict__ count) {
+#pragma clang fp contract(off)  // the CPU path's operation order, no fused multiply-adds
+  __shared__ double g[kRitaCap], c[kRitaCap], inc[kRitaCap], ng[kRitaCap];
+  __shared__ double terms[1001];
+  __shared__ int split[kRitaCap];
+  __shared__ int n_sh, done_sh;
+  __shared__ double norm_sh;
+  const int lane = threadIdx.x;
+  const int ie = blockIdx.x;
+  const double E = energies[ie];
+  // probability_dist.cpp:267-287: Simpson normalisation with 1001 points
+  const double h0 = 2.0 / 1000;
+  for (int k = lane; k <= 1000; k += 64) {
+    const double f = rayleigh_pdf(F, E, -1.0 + k * h0);
+    terms[k] = (k == 0 || k == 1000) ? f : ((k % 2 == 0) ? 2 * f : 4 * f);
+  }
+  __syncthreads();
+  if (lane == 0) {
+    double s = 0;
+    for (int k = 0; k <= 1000; ++k) s += terms[k];
+    norm_sh = h0 / 3 * s;
+    for (int i = 0; i < 10; ++i) g[i] = -1.0 + i * (2.0 / 9.0);
+    g[9] = 1.0;
+    n_sh = 10;
+  }
+  __syncthreads();
+  for (int iter = 0; iter < 64; ++iter) {
+    const int n = n_sh, m = n - 1;
+    // CDF increments, 51-point Simpson per interval
+    for (int j = lane; j < m; j += 64) {
+      const double h = (g[j + 1] - g[j]) / 50.0;
+      double s = 0;
+      for (int k = 0; k < 51; ++k) {
+        const double f = rayleigh_pdf(F, E, g[j] + k * h) / norm_sh;
+        s += (k == 0 || k == 50) ? f : ((k % 2 == 0) ? 2 * f : 4 * f);
+      }
+      inc[j] = h / 3 * s;
+    }
+    __syncthreads();
+    if (lane == 0) {
+      c[0] = 0.0;
+      for (int j = 0; j < m; ++j) c[j + 1] = c[j] + inc[j];
+    }
+    __syncthreads();
+    // interpolation parameters and the interval error (probability_dist.cpp:138-175)
+    for (int j = lane; j < m; j += 64) {
+      const double p0 = rayleigh_pdf(F, E, g[j]) / norm_sh, p1 = rayleigh_pdf(F, E, g[j + 1]) / norm_sh;
+      const double sl = (c[j + 1] - c[j]) / (g[j + 1] - g[j]);
+      const double b = 1 - sl * sl * (1 / (p0 * p1));
+      const double a = sl * (1 / p0) - b - 1;
+      const double x0 = g[j], x1 = g[j + 1], y0 = c[j], y1 = c[j + 1];
+      const int N = 51;
+      const double d = (x1 - x0) / N;
+      double s = 0;
+      for (int i = 1; i < N; ++i) {
+        const double xk = x0 + i * d;
+        s += fabs(rayleigh_pdf(F, E, xk) / norm_sh - rita_pnum(xk, x0, x1, y0, y1, a, b));
+      }
+      const double err = d * (s + (fabs(p1 - rita_pnum(x1, x0, x1, y0, y1, a, b)) +
+                                   fabs(p0 - rita_pnum(x0, x0, x1, y0, y1, a, b))) / 2);
+      split[j] = err > 1e-4;
+      xs[(size_t)ie * kRitaCap + j] = g[j];
+      as[(size_t)ie * kRitaCap + j] = a;
+      bs[(size_t)ie * kRitaCap + j] = b;
+    }
+    __syncthreads();
+    if (lane == 0) {
+      int k = 0;
+      bool ok = true;
+      for (int j = 0; j < m && k < kRitaCap - 1; ++j) {
+        ng[k++] = g[j];
+        if (split[j] && k < kRitaCap - 1) {
+          ng[k++] = (g[j] + g[j + 1]) / 2;
+          ok = false;
+        }
+      }
+      ng[k++] = g[m];
+      done_sh = ok ? 1 : (k >= kRitaCap - 1 ? -1 : 0);
+      if (ok) {
+        for (int j = 0; j < n; ++j) {
+          ys[(size_t)ie * kRitaCap + j] = c[j];
+        }
+        xs[(size_t)ie * kRitaCap + m] = g[m];
+        count[ie] = n;
+      } else {
+        n_sh = k;
+      }
+    }
+    __syncthreads();
+    if (done_sh != 0) break;
+    for (int j = lane; j < n_sh; j += 64) g[j] = ng[j];
+    __syncthreads();
+  }
+  if (lane == 0 && done_sh != 1) count[ie] = -1;  // did not converge within the cap
+}
+
 // ------------------------------------------------------------------ inspection kernels
 __global__ void k_eval_curve(const mdx_tables_dev* T, int slot, int which, int n, const float* x, float* out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1061,6 +1214,17 @@ hipError_t dose_f32(const double* ss, size_t nv, double n, float* out, hipStream
                      nv, n, reinterpret_cast<float2*>(out));
   return hipGetLastError();
 }
+
+hipError_t rita_build(const double* lx, const double* ly, int n_ff, double alpha, double inv_mc2,
+                      const double* energies, int ne, double* xs, double* ys, double* as, double* bs, int* count,
+                      hipStream_t st) {
+  if (ne == 0) return hipSuccess;
+  RitaFF F{lx, ly, n_ff, alpha, inv_mc2};
+  hipLaunchKernelGGL(k_rita_build, dim3((unsigned)ne), dim3(64), 0, st, F, energies, xs, ys, as, bs, count);
+  return hipGetLastError();
+}
+
+int rita_cap() { return kRitaCap; }
 
 hipError_t copy16(const void* src, void* dst, size_t n16, int blocks, hipStream_t st) {
   if (n16 == 0) return hipSuccess;

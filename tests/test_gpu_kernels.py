@@ -66,6 +66,32 @@ def test_scattering_function_within_2_ulp(ctx, mid):
     assert u.max() <= 2.0
 
 
+def test_rita_tables_built_on_device_match_host(ctx):
+    """The FP64 RITA tables built on the device (one wavefront per energy,
+    the fixture's context) against the host build of the same tables: the
+    same adaptive grids (grid points are dyadic, hence exact) at every energy
+    and CDF / parameters within 1e-12."""
+    host = M.Context(-1)
+    host.build_tables(list(MATS))
+    for mid in MATS:
+        ie, same = 0, 0
+        while True:
+            try:
+                e_d, x_d, y_d, a_d, b_d = ctx.rita_row(mid, ie)
+            except M.MdxError:
+                break
+            e_h, x_h, y_h, a_h, b_h = host.rita_row(mid, ie)
+            assert e_d == e_h
+            if x_d.size == x_h.size and np.array_equal(x_d, x_h):
+                same += 1
+                assert np.allclose(y_d, y_h, rtol=1e-12, atol=1e-15), (mid, ie)
+                assert np.allclose(a_d, a_h, rtol=1e-9, atol=1e-12), (mid, ie)
+                assert np.allclose(b_d, b_h, rtol=1e-9, atol=1e-12), (mid, ie)
+            ie += 1
+        assert ie > 300 and same >= ie - 2, (mid, same, ie)
+    host.close()
+
+
 @pytest.mark.parametrize("mid", (48, 169))
 def test_rita_device_matches_fp64(ctx, mid):
     rng = np.random.default_rng(4)
