@@ -139,6 +139,16 @@ int mdx_set_data_dir(mdx_ctx* ctx, const char* dir);
 /* InteractionDataHelpers::convertMaterialNameToId / convertMaterialIdToName
  * (include/MIDSX/Core/material_helpers.h:110-121). */
 int mdx_material_id(mdx_ctx* ctx, const char* name, int* id);
+/* Mass-energy absorption coefficient of dry air (cm^2/g) at n energies (eV),
+ * log-log linear between table nodes, as
+ * MaterialData::interpolateMassEnergyAbsorptionCoefficient of material 3
+ * (material_data.cpp:39-41, derived_quantities.cpp:51-53), whose database is
+ * not available: table 0 (default) holds NIST's values for dry air on NIST's
+ * energy grid (data/air_muen_nist.csv), the grid the reference's database
+ * interpolates; table 1 the AAPM TG-195 values every 0.5 keV
+ * (data/air_muen_tg195.csv). */
+int mdx_air_muen(mdx_ctx* ctx, int n, const double* energy, double* out);
+int mdx_set_air_muen_table(mdx_ctx* ctx, int table);
 int mdx_material_name(mdx_ctx* ctx, int id, char* buf, int buflen);
 
 /* InteractionData(material names) (interaction_data.cpp:3-73): builds the

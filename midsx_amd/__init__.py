@@ -43,7 +43,7 @@ EXPORTS = [
     "mdx_tables_device_bytes", "mdx_run_jobs", "mdx_set_queue_params", "mdx_get_queue_params",
     "mdx_autotune", "mdx_job_alias_domain", "mdx_set_graphs",
     "mdx_run_jobs_async", "mdx_wait", "mdx_mem_info", "mdx_host_alloc", "mdx_host_free",
-    "mdx_set_flights", "mdx_set_rita_builder", "mdx_set_dose_precision", "mdx_get_voxel_dose_f32", "mdx_stage_voxel_dose_f32", "mdx_copy_probe",
+    "mdx_set_flights", "mdx_set_rita_builder", "mdx_set_dose_precision", "mdx_get_voxel_dose_f32", "mdx_stage_voxel_dose_f32", "mdx_copy_probe", "mdx_air_muen", "mdx_set_air_muen_table",
 ]
 
 
@@ -471,6 +471,18 @@ class Context:
         ms = ctypes.c_double()
         self._call(self.lib.mdx_copy_probe, self._h, ctypes.c_uint64(nbytes), ctypes.byref(ms))
         return ms.value
+
+    def air_muen(self, energy):
+        """mu_en / rho of dry air (cm^2/g) at energies in eV."""
+        e = _f64(energy)
+        out = np.empty_like(e)
+        self._call(self.lib.mdx_air_muen, self._h, ctypes.c_int(e.size), _ptr(e, ctypes.c_double),
+                   _ptr(out, ctypes.c_double))
+        return out
+
+    def set_air_muen_table(self, table):
+        """0: NIST grid (default), 1: TG-195 0.5 keV grid."""
+        self._call(self.lib.mdx_set_air_muen_table, self._h, ctypes.c_int(table))
 
     def mem_info(self):
         f, t = ctypes.c_uint64(), ctypes.c_uint64()

@@ -229,6 +229,8 @@ class SurfaceQuantityContainer {
       }
   }
   std::vector<double> spectrum;  // IncidentEnergy histogram [bucket][bin] when requested
+  double spectrum_lo = 0, spectrum_width = 0;  // its first bin's lower edge and bin width (eV)
+  int spectrum_bins = 0;
 
  private:
   std::unordered_map<VectorSurfaceQuantityType, VectorSurfaceQuantity> vq_;
@@ -591,10 +593,58 @@ class InteractionData {
     return (uint8_t)id;
   }
   const std::vector<int>& ids() const { return ids_; }
+  // mu_en / rho of dry air (cm^2/g), TG-195 table (derived_quantities.cpp:53-57)
+  double interpolateAirMassEnergyAbsorptionCoefficient(double energy) const {
+    double out = 0;
+    midsx_amd_detail::check(mdx_air_muen(host_->p, 1, &energy, &out));
+    return out;
+  }
 
  private:
   std::shared_ptr<midsx_amd_detail::Ctx> host_;
   std::vector<int> ids_;
+};
+
+// derived_quantities.cpp:3-68 from a surface tally's IncidentEnergy
+// histogram (SurfaceTally::setSpectrum): the reference keeps every incident
+// energy, this API keeps their histogram, so a fluence "at E +- w/2" counts
+// the primary photons of the bins whose centres lie in [E - w/2, E + w/2].
+// With bins centred on the source spectrum's energies that is the
+// reference's count for primary photons.  Without a histogram, the fluence
+// counts every primary photon (a monoenergetic source's, as the reference's
+// monoenergetic drivers use it).  Cosine weighting is not available.
+struct DerivedQuantity {
+  static double getPrimaryFluence(SurfaceQuantityContainer& c, double energy, double energy_width,
+                                  bool is_cosine_weighted) {
+    if (is_cosine_weighted) throw std::invalid_argument("cosine-weighted fluence is not available");
+    if (c.getArea() <= 0) throw std::runtime_error("surface area unknown");
+    double total = 0;
+    if (c.spectrum_bins > 0 && !c.spectrum.empty()) {
+      for (int i = 0; i < c.spectrum_bins; ++i) {
+        const double centre = c.spectrum_lo + (i + 0.5) * c.spectrum_width;
+        if (centre >= energy - energy_width / 2 && centre <= energy + energy_width / 2) total += c.spectrum[i];
+      }
+    } else {
+      auto& q = c.getCountQuantities().at(CountSurfaceQuantityType::NumberOfPhotons);
+      total = q.getPrimaryValues().getCount();
+    }
+    return total / c.getArea();
+  }
+  static double getPrimaryAirKerma(SurfaceQuantityContainer& c, InteractionData& data, double energy,
+                                   double energy_width, bool is_cosine_weighted) {
+    const double mu_en = data.interpolateAirMassEnergyAbsorptionCoefficient(energy);
+    return energy * getPrimaryFluence(c, energy, energy_width, is_cosine_weighted) * mu_en;
+  }
+  // energy_spectrum: evenly spaced bin centres (eV); any type with size() and operator[]
+  template <class V>
+  static double getPrimarySpectrumAirKerma(SurfaceQuantityContainer& c, InteractionData& data,
+                                           const V& energy_spectrum, bool is_cosine_weighted) {
+    const double w = energy_spectrum[1] - energy_spectrum[0];
+    double total = 0;
+    for (size_t i = 0; i < (size_t)energy_spectrum.size(); ++i)
+      total += getPrimaryAirKerma(c, data, energy_spectrum[i], w, is_cosine_weighted);
+    return total;
+  }
 };
 
 InteractionData ComputationalDomain::getInteractionData() const {
@@ -656,7 +706,7 @@ class PhysicsEngine {
   void run(const PhotonSource& source, uint64_t n) {
     source.upload(ctx_.p);
     mdx_stats st;
-    midsx_amd_detail::check(mdx_run(ctx_.p, n, seed, histories_done_, MDX_KERNEL_PLAIN, &st));
+    midsx_amd_detail::check(mdx_run(ctx_.p, n, seed, histories_done_, kernel, &st));
     histories_done_ += n;
     last_ = st;
     std::vector<double> acc(MDX_TALLY_ACC_SIZE);
@@ -697,6 +747,10 @@ class PhysicsEngine {
       SurfaceQuantityContainer c = surface_[i]->getSurfaceQuantityContainer();
       c.load(acc_[k].data());
       c.spectrum = hist_[k];
+      const mdx_tally& d = surface_[i]->descriptor();
+      c.spectrum_bins = d.hist_bins;
+      c.spectrum_lo = d.hist_lo;
+      c.spectrum_width = d.hist_width;
       if (auto* im = dynamic_cast<ImageSurfaceTally*>(surface_[i].get())) im->image = img_[k];
       out.push_back(c);
     }
@@ -708,6 +762,8 @@ class PhysicsEngine {
   const mdx_stats& lastStats() const { return last_; }
   mdx_ctx* context() const { return ctx_.p; }
   uint64_t seed = 1;
+  // MDX_KERNEL_QUEUE (default: the event-regrouped kernel) or MDX_KERNEL_PLAIN
+  int kernel = MDX_KERNEL_QUEUE;
 
  private:
   static void add(std::vector<double>& a, const std::vector<double>& b) {

@@ -262,6 +262,25 @@ struct mdx_ctx {
     registry = mdx::load_registry(data_dir + "/materials.csv");
     registry_loaded = true;
   }
+  // mass-energy absorption coefficients of dry air (E eV, mu_en/rho cm^2/g)
+  std::vector<double> muen_e, muen_v;
+  int muen_table = 0;  // 0: NIST grid (air_muen_nist.csv), 1: TG-195 0.5 keV grid (air_muen_tg195.csv)
+  void ensure_air_muen() {
+    if (!muen_e.empty()) return;
+    const std::string path = data_dir + (muen_table == 1 ? "/air_muen_tg195.csv" : "/air_muen_nist.csv");
+    FILE* f = std::fopen(path.c_str(), "r");
+    if (!f) throw std::runtime_error("air mu_en table not found: " + path);
+    char line[256];
+    while (std::fgets(line, sizeof line, f)) {
+      double e, v;
+      if (std::sscanf(line, "%lf,%lf", &e, &v) == 2) {
+        muen_e.push_back(e * 1e3);
+        muen_v.push_back(v);
+      }
+    }
+    std::fclose(f);
+    if (muen_e.size() < 2) throw std::runtime_error("air mu_en table is empty: " + path);
+  }
   int slot_of(int id) const { return (id >= 0 && id < 256) ? packed.id_to_slot[id] : 255; }
 };
 
@@ -792,6 +811,38 @@ int mdx_material_id(mdx_ctx* c, const char* name, int* id) {
         return;
       }
     throw std::runtime_error(std::string("unknown material name '") + name + "'");
+  });
+}
+
+int mdx_air_muen(mdx_ctx* c, int n, const double* energy, double* out) {
+  return guard([&] {
+    need(c && (n == 0 || (energy && out)) && n >= 0, "bad argument");
+    c->ensure_air_muen();
+    const auto& X = c->muen_e;
+    const auto& Y = c->muen_v;
+    for (int i = 0; i < n; ++i) {
+      const double e = energy[i];
+      if (e <= X.front()) { out[i] = Y.front(); continue; }
+      if (e >= X.back()) { out[i] = Y.back(); continue; }
+      const size_t k = std::upper_bound(X.begin(), X.end(), e) - X.begin();  // above an edge's repeated energy
+      const double x0 = X[k - 1], x1 = X[k], y0 = Y[k - 1], y1 = Y[k];
+      if (e == x0) out[i] = y0;
+      else if (y0 > 0 && y1 > 0)  // log-log linear, as the reference's tables
+        out[i] = std::exp(std::log(y0) + (std::log(y1) - std::log(y0)) * (std::log(e) - std::log(x0)) /
+                                             (std::log(x1) - std::log(x0)));
+      else  // across the zeros below the first tabulated absorption
+        out[i] = y0 + (y1 - y0) * (e - x0) / (x1 - x0);
+    }
+  });
+}
+
+int mdx_set_air_muen_table(mdx_ctx* c, int table) {
+  return guard([&] {
+    need(c, "NULL context");
+    need(table == 0 || table == 1, "unknown air mu_en table");
+    c->muen_table = table;
+    c->muen_e.clear();
+    c->muen_v.clear();
   });
 }
 

@@ -54,6 +54,15 @@ def primary_fraction(sim, runner, n):
     return sim.tally(0)["acc"][2, 0, 0] / n
 
 
+def test_recorded_kvp_ratios_are_consistent():
+    """The recorded kVp runs: HVL halves and QVL quarters the air kerma."""
+    for kvp in (30, 100):
+        r, s = recorded_ratio_kvp(kvp, "hvl")
+        assert abs(r - 0.5) < 5e-3 and s < 1e-3, (kvp, r, s)
+        r, s = recorded_ratio_kvp(kvp, "qvl")
+        assert abs(r - 0.25) < 5e-3, (kvp, r)
+
+
 def test_recorded_ratios_are_consistent():
     r, s = recorded_ratio(30, "hvl")
     assert abs(r - 0.5) < 1e-3 and s < 1e-4
@@ -97,6 +106,58 @@ def test_gpu_layer_ratio_matches_recorded(energy_kev, layer):
     assert abs(ratio - ref) < 4 * math.hypot(sig, ref_s), (ratio, ref, sig, ref_s)
 
 
+KVP_SPECTRA = {30: "RQRM3_MO_MO_30kVp_E_spectrum.csv", 100: "RQR8_W_AL_100kVp_E_spectrum.csv"}
+
+
+def kvp_spectrum(kvp):
+    """The reference's spectrum file (data/source_distributions/, keV and
+    weight per 0.5 keV bin centre), energies in eV as the drivers convert
+    them (hvl_30kVp.cpp:21-25)."""
+    d = np.loadtxt(os.path.join(GOLDEN, "spectra", KVP_SPECTRA[kvp]), delimiter=",")
+    return d[:, 0] * 1e3, d[:, 1]
+
+
+def recorded_ratio_kvp(kvp, layer):
+    a0 = np.loadtxt(os.path.join(GOLDEN, "hvl", "Al_%dkVp_AK_0.txt" % kvp))
+    a1 = np.loadtxt(os.path.join(GOLDEN, "hvl", "Al_%dkVp_AK_%s.txt" % (kvp, layer)))
+    r = a1.mean() / a0.mean()
+    s = r * math.hypot(a1.std(ddof=1) / a1.mean() / math.sqrt(len(a1)), a0.std(ddof=1) / a0.mean() / math.sqrt(len(a0)))
+    return r, s
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kvp,layer", [(30, "hvl"), (30, "qvl"), (100, "hvl"), (100, "qvl")])
+def test_gpu_kvp_air_kerma_ratio_matches_recorded(kvp, layer):
+    """Polyenergetic beams (hvl_30kVp.cpp, hvl_100kVp.cpp): the primary air
+    kerma at the disc, sum over the spectrum's bins of E * fluence * mu_en/rho
+    of air (DerivedQuantity::getPrimarySpectrumAirKerma), with and without the
+    Al layer; the ratio within 4 combined sigma (40 batches each) of the
+    reference's recorded outputs."""
+    from midsx_amd import derived
+    e, w = kvp_spectrum(kvp)
+    k, n = 40, 1000000
+    ak = {}
+    for name in ("air_only.json", "%s_%dkVp.json" % (layer, kvp)):
+        ctx = M.Context(0)
+        scene_into(ctx, name)
+        ctx.build_tables([3, 61])
+        ctx.set_source(direction=M.DIR_DISC_ISOTROPIC, position=(2, 2, 0), a=(2, 2, 10), b=(0, 0, 1), radius=0.1,
+                       spectrum=(e, w))
+        ctx.set_tallies([derived.spectrum_disc_tally((2, 2, 100), (0, 0, 1), 1.0, e)])
+        vals = []
+        for b in range(k):
+            ctx.reset_tallies()
+            ctx.run(n, seed=kvp, first=b * n, kernel=M.KERNEL_QUEUE)
+            vals.append(derived.primary_spectrum_air_kerma(ctx, 0, e) / n)
+        ak[name] = np.array(vals)
+        ctx.close()
+    a0, a1 = ak["air_only.json"], ak["%s_%dkVp.json" % (layer, kvp)]
+    ratio = a1.mean() / a0.mean()
+    sig = ratio * math.hypot(a1.std(ddof=1) / a1.mean(), a0.std(ddof=1) / a0.mean()) / math.sqrt(k)
+    ref, ref_s = recorded_ratio_kvp(kvp, layer)
+    assert abs(ratio - ref) < 4 * math.hypot(sig, ref_s), (ratio, ref, sig, ref_s)
+
+
 def build_cpp_driver(tmpdir):
     exe = os.path.join(str(tmpdir), "hvl_slab")
     lib = os.path.join(ROOT, "midsx_amd", "_lib")
@@ -128,4 +189,39 @@ def test_cpp_driver_hvl_ratio(tmp_path):
     ratio = res["hvl_30keV.json"] / res["air_only.json"]
     ref, ref_s = recorded_ratio(30, "hvl")
     sig = ratio * math.sqrt(2.2 / n)
+    assert abs(ratio - ref) < 4 * math.hypot(sig, ref_s), (ratio, ref)
+
+
+def build_cpp_kvp_driver(tmpdir):
+    exe = os.path.join(str(tmpdir), "hvl_kvp")
+    lib = os.path.join(ROOT, "midsx_amd", "_lib")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Werror", "-I", os.path.join(ROOT, "midsx_amd", "cpp"),
+           os.path.join(ROOT, "midsx_amd", "cpp", "examples", "hvl_kvp.cpp"), "-o", exe, "-L", lib,
+           "-lmidsx_amd", "-Wl,-rpath," + lib]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout
+    return exe
+
+
+def test_cpp_kvp_driver_compiles(tmp_path):
+    assert os.path.exists(build_cpp_kvp_driver(tmp_path))
+
+
+@pytest.mark.gpu
+def test_cpp_kvp_driver_air_kerma_ratio(tmp_path):
+    """The reference's 30 kVp driver, ported to this API, reproduces the
+    recorded AK(HVL) / AK(0)."""
+    exe = build_cpp_kvp_driver(tmp_path)
+    spec = os.path.join(GOLDEN, "spectra", KVP_SPECTRA[30])
+    n = 20000000
+    ak = {}
+    for name in ("air_only.json", "hvl_30kVp.json"):
+        r = subprocess.run([exe, os.path.join(SCENES, name), spec, str(n)], stdout=subprocess.PIPE,
+                           stderr=subprocess.STDOUT, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout
+        kv = dict(line.split(" ", 1) for line in r.stdout.strip().splitlines())
+        ak[name] = float(kv["primary_air_kerma"])
+    ratio = ak["hvl_30kVp.json"] / ak["air_only.json"]
+    ref, ref_s = recorded_ratio_kvp(30, "hvl")
+    sig = ratio * math.sqrt(3.0 / n)
     assert abs(ratio - ref) < 4 * math.hypot(sig, ref_s), (ratio, ref)

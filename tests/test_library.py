@@ -160,3 +160,47 @@ def test_ct_scene_host():
     h = scenes.ct_histories(360)
     assert h.min() >= 2e6 and h.max() <= 2e7 and 3e9 < h.sum() < 5e9
     c.close()
+
+
+def test_air_muen_table_and_air_kerma_arithmetic():
+    """The air coefficients at the table's nodes are the TG-195 values (the
+    CSV is extracted verbatim), log-log between nodes; the air kerma sums
+    E * fluence * mu_en over the spectrum's bins."""
+    from midsx_amd import derived
+    c = M.Context(-1)
+    nist = np.loadtxt(os.path.join(os.path.dirname(M.__file__), "data", "air_muen_nist.csv"), delimiter=",",
+                      skiprows=1)
+    keep = np.r_[np.diff(nist[:, 0]) > 0, True]  # at an edge's energy the value above the edge
+    got = c.air_muen(nist[keep, 0] * 1e3)
+    assert np.allclose(got[1:], nist[keep, 1][1:], rtol=1e-14)
+    # log-log between nodes: exact for a power law through two nodes
+    e60, e80 = 60e3, 80e3
+    v60, v80 = c.air_muen([e60, e80])
+    mid = c.air_muen([70e3])[0]
+    assert abs(mid - v60 * (v80 / v60) ** (np.log(70 / 60) / np.log(80 / 60))) < 1e-12 * mid
+    assert c.air_muen([3.25e3])[0] == pytest.approx(146.0 * (76.36 / 146.0) ** (np.log(3.25 / 3.2029) / np.log(4 / 3.2029)))
+    c.set_air_muen_table(1)
+    tab = np.loadtxt(os.path.join(os.path.dirname(M.__file__), "data", "air_muen_tg195.csv"), delimiter=",",
+                     skiprows=1)
+    e = tab[:, 0] * 1e3
+    assert np.array_equal(c.air_muen(e), tab[:, 1])
+    c.set_air_muen_table(0)
+    energies = np.arange(0.25, 30.0, 0.5) * 1e3
+    t = derived.spectrum_disc_tally((2, 2, 100), (0, 0, 1), 1.0, energies)
+    assert t["hist_bins"] == energies.size and t["hist_lo"] == 0.0 and t["hist_width"] == 500.0
+
+    class Fake:
+        tallies = [t]
+
+        def tally(self, i):
+            h = np.zeros((4, energies.size))
+            h[0, 20] = 1000.0  # primaries in the 10.25 keV bin
+            h[1, 21] = 77.0    # scattered ones do not count
+            return {"hist": h}
+
+        def air_muen(self, x):
+            return c.air_muen(x)
+
+    ak = derived.primary_spectrum_air_kerma(Fake(), 0, energies)
+    assert abs(ak - 10.25e3 * 1000.0 / np.pi * c.air_muen([10.25e3])[0]) < 1e-9 * ak
+    c.close()
