@@ -212,15 +212,17 @@ def ct_scan(ctx, n_views, scale, seed=4):
 def precision_study(ctx, n_hist, step=10 ** 8):
     """Config 3: the slab phantom (40 x 40 x 10 water voxels, 60 keV pencil
     beam), n_hist histories on the queue kernel with the voxel dose
-    accumulated in FP64, FP32 and compensated FP32 on identical RNG streams;
-    per-voxel relative error of each against the FP64 run."""
+    accumulated in FP64, FP32, compensated FP32 and compensated FP32 summed
+    per block in LDS first, on identical RNG streams; per-voxel relative
+    error of each against the FP64 run."""
     import midsx_amd as M
     from midsx_amd import scenes
     job = ctx.job_create()
     scenes.setup_slab(ctx)
     out = {}
     res = {}
-    for name, mode in (("fp64", M.DOSE_F64), ("fp32", M.DOSE_F32), ("fp32_compensated", M.DOSE_F32C)):
+    for name, mode in (("fp64", M.DOSE_F64), ("fp32", M.DOSE_F32), ("fp32_compensated", M.DOSE_F32C),
+                       ("fp32_compensated_lds", M.DOSE_F32C_LDS)):
         ctx.set_dose_precision(mode)
         ctx.reset_tallies()
         ms = 0.0
@@ -234,7 +236,7 @@ def precision_study(ctx, n_hist, step=10 ** 8):
         res[name] = {"kernel_s": ms / 1e3, "histories_per_s": n_hist / (ms / 1e3)}
     s64, q64, c64 = out["fp64"]
     roi = c64 >= 1000
-    for name in ("fp32", "fp32_compensated"):
+    for name in ("fp32", "fp32_compensated", "fp32_compensated_lds"):
         s, s2, cnt = out[name]
         e = np.abs(s[roi] / s64[roi] - 1)
         e2 = np.abs(s2[roi] / q64[roi] - 1)

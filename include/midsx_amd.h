@@ -248,6 +248,12 @@ int mdx_set_rita_builder(mdx_ctx* ctx, int on_device);
 #define MDX_DOSE_F64 0
 #define MDX_DOSE_F32 1
 #define MDX_DOSE_F32C 2
+/* MDX_DOSE_F32C_LDS: as MDX_DOSE_F32C, with each block of the single-job
+ * queue kernel summing the deposits of grid 0 in LDS (compensated FP32) and
+ * adding its block sums to the global compensated sums at its end; grid 0
+ * must fit (8 bytes per voxel within the LDS a block can take, about 17 000
+ * voxels).  Other launches use the global form. */
+#define MDX_DOSE_F32C_LDS 3
 int mdx_set_dose_precision(mdx_ctx* ctx, int mode);
 /* Run mdx_run_jobs rounds as a replayed hipGraph (round buffer copy, counter
  * zeroing, queue kernel), at most 1024 jobs per round. */

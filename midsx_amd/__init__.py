@@ -24,7 +24,7 @@ SPEC_MONO, SPEC_POLY = 0, 1
 TALLY_AACUBOID, TALLY_DISC, TALLY_RECT, TALLY_IMAGE = 0, 1, 2, 3
 XS_TOTAL, XS_COHERENT, XS_INCOHERENT, XS_PHOTOELECTRIC, SF_INCOHERENT, FF_COHERENT = 0, 1, 2, 3, 4, 5
 KERNEL_PLAIN, KERNEL_QUEUE = 0, 1
-DOSE_F64, DOSE_F32, DOSE_F32C = 0, 1, 2
+DOSE_F64, DOSE_F32, DOSE_F32C, DOSE_F32C_LDS = 0, 1, 2, 3
 NQ, NBUCKET, NFIELD = 4, 4, 3
 STAT_NAMES = ["histories", "steps", "real", "rayleigh", "compton", "photo", "kahn_trials", "kahn_s_tests",
               "deposits", "surface_hits", "escapes", "reserved"]

@@ -213,6 +213,7 @@ struct mdx_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int num_cu = 256;
+  size_t lds_optin = 65536;  // LDS a block can take (opt-in limit)
   std::string data_dir;
   bool registry_loaded = false;
   std::map<int, mdx::MaterialDef> registry;
@@ -712,6 +713,11 @@ void run_jobs(mdx_ctx* c, int n_jobs, const int* ids, const uint64_t* first, con
   std::vector<mdx_job_dev> jd(n_jobs);
   std::vector<uint64_t> prefix(n_jobs + 1, 0);
   const uint64_t total = fill_round(c, n_jobs, ids, first, n, seed, jd.data(), prefix.data());
+  uint32_t lds_voxels = 0;  // MDX_DOSE_F32C_LDS: grid 0 summed per block in LDS (single-job launches)
+  if (n_jobs == 1) {
+    const Job& O = job_at(c, domain_owner(c, ids[0]));
+    if (O.voxel_dose && O.dose_mode == MDX_DOSE_F32C_LDS && !O.grids.empty()) lds_voxels = (uint32_t)O.grids[0]->voxels();
+  }
   c->d_jobs.upload(jd.data(), jd.size(), c->stream);
   c->d_prefix.upload(prefix.data(), prefix.size(), c->stream);
   hip_check(mdx_launch::zero_u64(cnt, MDX_NCOUNTER, c->stream), "zero counters");
@@ -721,7 +727,7 @@ void run_jobs(mdx_ctx* c, int n_jobs, const int* ids, const uint64_t* first, con
     hip_check(mdx_launch::transport_queue(c->d_tables.as<mdx_tables_dev>(), c->d_jobs.as<mdx_job_dev>(),
                                           c->d_prefix.as<uint64_t>(), n_jobs, total,
                                           c->d_work.as<unsigned long long>(), cnt, c->num_cu * c->queue_blocks_per_cu,
-                                          c->refill_min, c->interact_min, c->stream),
+                                          c->refill_min, c->interact_min, lds_voxels, c->stream),
               "queue launch");
   hip_check(hipEventRecord(c->ev1, c->stream), "hipEventRecord");
   c->inflight = true;
@@ -763,6 +769,7 @@ int mdx_create(int device, mdx_ctx** out) {
     hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
     c->num_cu = prop.multiProcessorCount;
     c->pool.limit = prop.totalGlobalMem / 4;
+    c->lds_optin = std::max(prop.sharedMemPerBlockOptin, prop.sharedMemPerBlock);
     c->d_counters.alloc(MDX_NCOUNTER * sizeof(unsigned long long));
     c->d_work.alloc(sizeof(unsigned long long));
     *out = c.release();
@@ -1286,8 +1293,18 @@ int mdx_job_alias_domain(mdx_ctx* c, int src) {
 int mdx_set_dose_precision(mdx_ctx* c, int mode) {
   return guard([&] {
     need(c, "NULL context");
-    need(mode == MDX_DOSE_F64 || mode == MDX_DOSE_F32 || mode == MDX_DOSE_F32C, "unknown dose precision");
+    need(mode == MDX_DOSE_F64 || mode == MDX_DOSE_F32 || mode == MDX_DOSE_F32C || mode == MDX_DOSE_F32C_LDS,
+         "unknown dose precision");
     need(c->J().domain_of < 0, "voxel dose belongs to the job that owns the domain");
+    if (mode == MDX_DOSE_F32C_LDS) {
+      need(!c->J().grids.empty(), "MDX_DOSE_F32C_LDS needs a grid");
+      const size_t static_lds = 24 * 1024;  // the queue kernel's own LDS (tallies, spectra, counters), rounded up
+      const size_t need_b = c->J().grids[0]->voxels() * 8;
+      if (need_b + static_lds > c->lds_optin)
+        throw std::invalid_argument("grid 0 has " + std::to_string(c->J().grids[0]->voxels()) +
+                                    " voxels; MDX_DOSE_F32C_LDS holds at most " +
+                                    std::to_string((c->lds_optin - static_lds) / 8));
+    }
     c->J().dose_mode = mode;
     if (c->device < 0) return;
     set_device_job(c);

@@ -76,6 +76,7 @@ struct mdx_tables_dev {
 #define MDX_DOSE_F64 0   // FP64 atomics
 #define MDX_DOSE_F32 1   // FP32 atomics
 #define MDX_DOSE_F32C 2  // FP32 atomics with the rounding error of each add kept in a second FP32
+#define MDX_DOSE_F32C_LDS 3  // as F32C, the sums of deposits first per block in LDS (one grid that fits)
 
 struct mdx_grid_dev {
   float origin[3];

@@ -91,15 +91,17 @@ __device__ __forceinline__ void compensated_add(float* p, float v) {
 
 // One energy deposit in voxel lin of grid G (physics_engine.cpp:70-75,
 // 160-168) in the grid's accumulation precision.
-__device__ __forceinline__ void voxel_deposit(const mdx_grid_dev& G, uint64_t lin, float dep) {
+__device__ __forceinline__ void voxel_deposit(const mdx_grid_dev& G, uint64_t lin, float dep,
+                                              float* ldose = nullptr) {
   if (G.dose_mode == MDX_DOSE_F32) {
     if (G.dose32) {
       atomicAdd(G.dose32 + 2 * lin, dep);
       atomicAdd(G.dose32 + 2 * lin + 1, dep * dep);
     }
-  } else if (G.dose_mode == MDX_DOSE_F32C) {
+  } else if (G.dose_mode == MDX_DOSE_F32C || G.dose_mode == MDX_DOSE_F32C_LDS) {
     if (G.dose32) {
-      compensated_add(G.dose32 + 4 * lin, dep);
+      if (ldose) compensated_add(ldose + 2 * lin, dep);  // this block's sum, in LDS
+      else compensated_add(G.dose32 + 4 * lin, dep);
       const float q = __fmul_rn(dep, dep);  // dep^2 = q + ql exactly (never contracted into an fma)
       const float ql = __builtin_fmaf(dep, dep, -q);
       compensated_add(G.dose32 + 4 * lin + 2, q);
@@ -614,11 +616,17 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
                                                      const uint64_t* __restrict__ prefix, int n_jobs, uint64_t total,
                                                      unsigned long long* __restrict__ work,
                                                      unsigned long long* __restrict__ counters, int refill_min,
-                                                     int interact_min) {
+                                                     int interact_min, uint32_t lds_voxels) {
   __shared__ double acc[MULTI ? 1 : MDX_MAX_TALLIES * kAccPerTally];
   __shared__ double lhist[MULTI ? 1 : kLdsHist];
   __shared__ unsigned long long cnt[MDX_NCOUNTER];
+  extern __shared__ float ldose_mem[];  // MDX_DOSE_F32C_LDS: (sum, error) per voxel of grid 0
   int hoff[MDX_MAX_TALLIES];
+  float* ldose = nullptr;
+  if (!MULTI && lds_voxels > 0) {
+    ldose = ldose_mem;
+    for (uint32_t i = threadIdx.x; i < 2 * lds_voxels; i += blockDim.x) ldose[i] = 0.0f;
+  }
   if (!MULTI) {
     hist_layout(jobs[0].TL, hoff);
     const int ntl = jobs[0].TL->n;
@@ -825,7 +833,7 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
         if (dep != 0.0f) {
           ++c_dep;
           if (qg >= 0) {
-            voxel_deposit(D->grids[qg], qlin, dep);
+            voxel_deposit(D->grids[qg], qlin, dep, qg == 0 ? ldose : nullptr);
           } else if (D->bg_dose) {
             atomicAdd(D->bg_dose, (double)dep);
             atomicAdd(D->bg_dose + 1, (double)dep * (double)dep);
@@ -860,6 +868,14 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
       }
     }
     flush_hist(jobs[0].TL, hoff, lhist);
+    if (ldose) {  // the block's sums of grid 0 into the global compensated sums
+      float* g = jobs[0].D->grids[0].dose32;
+      for (uint32_t i = threadIdx.x; i < lds_voxels; i += blockDim.x) {
+        const float hi = ldose[2 * i], lo = ldose[2 * i + 1];
+        if (hi != 0.0f) compensated_add(g + 4 * i, hi);
+        if (lo != 0.0f) atomicAdd(g + 4 * i + 1, lo);
+      }
+    }
   }
   if (threadIdx.x < MDX_NCOUNTER && cnt[threadIdx.x]) atomicAdd(counters + threadIdx.x, cnt[threadIdx.x]);
 }
@@ -869,8 +885,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MDX
 k_transport_queue(const mdx_tables_dev* __restrict__ T, const mdx_job_dev* __restrict__ jobs,
                   const uint64_t* __restrict__ prefix, int n_jobs, uint64_t total,
                   unsigned long long* __restrict__ work, unsigned long long* __restrict__ counters,
-                  int refill_min, int interact_min) {
-  transport_queue_body<MULTI>(T, jobs, prefix, n_jobs, total, work, counters, refill_min, interact_min);
+                  int refill_min, int interact_min, uint32_t lds_voxels) {
+  transport_queue_body<MULTI>(T, jobs, prefix, n_jobs, total, work, counters, refill_min, interact_min,
+                              lds_voxels);
 }
 
 // Graph form: job count and total come from the round buffer's header
@@ -885,7 +902,7 @@ k_transport_queue_round(const mdx_tables_dev* __restrict__ T, const char* __rest
   const uint64_t* prefix = hdr + 2;
   const mdx_job_dev* jobs = reinterpret_cast<const mdx_job_dev*>(round + 2 * sizeof(uint64_t) + 1025 * sizeof(uint64_t));
   if (n_jobs <= 0 || total == 0) return;
-  transport_queue_body<true>(T, jobs, prefix, n_jobs, total, work, counters, refill_min, interact_min);
+  transport_queue_body<true>(T, jobs, prefix, n_jobs, total, work, counters, refill_min, interact_min, 0u);
 }
 
 // ------------------------------------------------------------------ RITA table build (FP64)
@@ -1143,18 +1160,25 @@ hipError_t transport_plain(const mdx_tables_dev* T, const mdx_domain_dev* D, con
 
 hipError_t transport_queue(const mdx_tables_dev* T, const mdx_job_dev* jobs, const uint64_t* prefix, int n_jobs,
                            uint64_t total, unsigned long long* work, unsigned long long* counters, int grid_blocks,
-                           int refill_min, int interact_min, hipStream_t st) {
+                           int refill_min, int interact_min, uint32_t lds_voxels, hipStream_t st) {
   unsigned blocks = (unsigned)grid_blocks;
   uint64_t need = (total + kBlock - 1) / kBlock;
   if (need < blocks) blocks = (unsigned)(need ? need : 1);
   refill_min = refill_min < 1 ? 1 : (refill_min > 64 ? 64 : refill_min);
   interact_min = interact_min < 1 ? 1 : (interact_min > 64 ? 64 : interact_min);
-  if (n_jobs == 1)
-    hipLaunchKernelGGL(k_transport_queue<false>, dim3(blocks), dim3(kBlock), 0, st, T, jobs, prefix, n_jobs, total,
-                       work, counters, refill_min, interact_min);
-  else
+  if (n_jobs == 1) {
+    const size_t lds = (size_t)lds_voxels * 2 * sizeof(float);
+    if (lds > 65536) {
+      hipError_t e = hipFuncSetAttribute((const void*)k_transport_queue<false>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_transport_queue<false>, dim3(blocks), dim3(kBlock), lds, st, T, jobs, prefix, n_jobs, total,
+                       work, counters, refill_min, interact_min, lds_voxels);
+  } else {
     hipLaunchKernelGGL(k_transport_queue<true>, dim3(blocks), dim3(kBlock), 0, st, T, jobs, prefix, n_jobs, total,
-                       work, counters, refill_min, interact_min);
+                       work, counters, refill_min, interact_min, 0u);
+  }
   return hipGetLastError();
 }
 

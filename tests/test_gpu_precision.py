@@ -19,7 +19,7 @@ def test_dose_precision_modes(kernel):
     scenes.setup_slab(ctx)
     n = 2_000_000
     out = {}
-    for mode in (M.DOSE_F64, M.DOSE_F32, M.DOSE_F32C):
+    for mode in (M.DOSE_F64, M.DOSE_F32, M.DOSE_F32C, M.DOSE_F32C_LDS):
         ctx.set_dose_precision(mode)
         ctx.reset_tallies()
         ctx.run(n // 2, seed=5, first=0, kernel=kernel)
@@ -28,8 +28,11 @@ def test_dose_precision_modes(kernel):
     s64, q64, c64 = out[M.DOSE_F64]
     hot = c64 >= 20
     assert hot.sum() > 40
-    for mode in (M.DOSE_F32, M.DOSE_F32C):
+    for mode in (M.DOSE_F32, M.DOSE_F32C, M.DOSE_F32C_LDS):
         assert np.array_equal(out[mode][2], c64)  # the same deposits
+    # block sums in LDS (queue kernel; the plain kernel takes the global form)
+    e32l = np.abs(out[M.DOSE_F32C_LDS][0][hot] / s64[hot] - 1)
+    assert e32l.max() < 1e-9, e32l.max()
     e32 = np.abs(out[M.DOSE_F32][0][hot] / s64[hot] - 1)
     q32 = np.abs(out[M.DOSE_F32][1][hot] / q64[hot] - 1)
     e32c = np.abs(out[M.DOSE_F32C][0][hot] / s64[hot] - 1)
@@ -43,4 +46,15 @@ def test_dose_precision_modes(kernel):
     assert e32.max() > 10 * max(e32c.max(), 1e-12) and q32.max() > 10 * q32c.max()
     with pytest.raises(M.MdxError):
         ctx.voxel_dose_f32(0, n)  # FP32 (mean, error) reads need FP64 sums
+    ctx.close()
+
+
+def test_lds_dose_rejects_large_grids():
+    ctx = M.Context(0)
+    ctx.build_tables(list(scenes.C1_MATERIALS))
+    ctx.set_domain((10.0, 10.0, 10.0), 3)
+    ctx.add_grid(np.full((40, 40, 40), 48, np.uint8), (0.25, 0.25, 0.25), (0, 0, 0))
+    ctx.enable_voxel_dose(True)
+    with pytest.raises(M.MdxError):
+        ctx.set_dose_precision(M.DOSE_F32C_LDS)  # 64 000 voxels do not fit in one block's LDS
     ctx.close()
