@@ -43,7 +43,8 @@ EXPORTS = [
     "mdx_tables_device_bytes", "mdx_run_jobs", "mdx_set_queue_params", "mdx_get_queue_params",
     "mdx_autotune", "mdx_job_alias_domain", "mdx_set_graphs",
     "mdx_run_jobs_async", "mdx_wait", "mdx_mem_info", "mdx_host_alloc", "mdx_host_free",
-    "mdx_set_flights", "mdx_set_rita_builder", "mdx_set_dose_precision", "mdx_get_voxel_dose_f32", "mdx_stage_voxel_dose_f32", "mdx_copy_probe", "mdx_air_muen", "mdx_set_air_muen_table",
+    "mdx_set_flights", "mdx_set_rita_builder", "mdx_set_dose_precision", "mdx_get_voxel_dose_f32", "mdx_stage_voxel_dose_f32",
+    "mdx_read_staged_dose_f32_async", "mdx_read_done", "mdx_copy_probe", "mdx_air_muen", "mdx_set_air_muen_table",
 ]
 
 
@@ -483,6 +484,20 @@ class Context:
     def set_air_muen_table(self, table):
         """0: NIST grid (default), 1: TG-195 0.5 keV grid."""
         self._call(self.lib.mdx_set_air_muen_table, self._h, ctypes.c_int(table))
+
+    def read_staged_dose_f32_async(self, out):
+        """Starts copying the selected job's staged FP32 dose into out (a
+        float32 array of 2 * voxels, pinned for a copy that does not block)."""
+        assert out.dtype == np.float32 and out.flags.c_contiguous
+        self._call(self.lib.mdx_read_staged_dose_f32_async, self._h, _ptr(out, ctypes.c_float))
+
+    def read_done(self, job, wait=False):
+        """True once the copy started by read_staged_dose_f32_async for job
+        has finished (wait=True blocks until then)."""
+        d = ctypes.c_int()
+        self._call(self.lib.mdx_read_done, self._h, ctypes.c_int(job), ctypes.c_int(1 if wait else 0),
+                   ctypes.byref(d))
+        return bool(d.value)
 
     def mem_info(self):
         f, t = ctypes.c_uint64(), ctypes.c_uint64()

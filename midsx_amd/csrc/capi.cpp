@@ -192,11 +192,15 @@ struct Job {
   hipEvent_t ev_stage = nullptr;
   int staged_grid = -1;
   uint64_t staged_n = 0;
+  hipEvent_t ev_copy = nullptr;  // end of the staged dose's copy to the host (mdx_read_staged_dose_f32_async)
+  bool copy_pending = false;
   Job() = default;
   Job(const Job&) = delete;
   Job& operator=(const Job&) = delete;
   ~Job() {
+    if (copy_pending && ev_copy) (void)hipEventSynchronize(ev_copy);
     if (ev_stage) (void)hipEventDestroy(ev_stage);
+    if (ev_copy) (void)hipEventDestroy(ev_copy);
   }
   size_t device_bytes() const {
     size_t b = d_bg.bytes + d_domain.bytes + d_cdf.bytes + d_energies.bytes + d_source.bytes + d_tallies.bytes +
@@ -589,7 +593,8 @@ void read_stats(mdx_ctx* c, mdx_stats* stats) {
   if (!stats) return;
   std::memset(stats, 0, sizeof *stats);
   unsigned long long h[MDX_NCOUNTER];
-  hip_check(hipMemcpy(h, c->d_counters.p, sizeof h, hipMemcpyDeviceToHost), "counters");
+  hip_check(hipMemcpyAsync(h, c->d_counters.p, sizeof h, hipMemcpyDeviceToHost, c->stream), "counters");
+  hip_check(hipStreamSynchronize(c->stream), "counters");
   for (int i = 0; i < MDX_NCOUNTER && i < MDX_STAT_NCOUNTERS; ++i) stats->counters[i] = h[i];
   float ms = 0;
   hip_check(hipEventElapsedTime(&ms, c->ev0, c->ev1), "hipEventElapsedTime");
@@ -1452,6 +1457,15 @@ int mdx_get_tally(mdx_ctx* c, int i, double* acc, double* hist, double* image) {
     if (i < 0 || i >= (int)c->J().tallies.size()) throw std::out_of_range("tally index out of range");
     set_device_nowait(c);
     TallyHost& h = *c->J().tallies[i];
+    // beside a round on the I/O stream; otherwise on the transport stream, so
+    // as not to queue behind dose copies in flight on the I/O stream
+    hipStream_t keep = c->io_stream;
+    if (!c->inflight) c->io_stream = c->stream;
+    struct Restore {
+      mdx_ctx* c;
+      hipStream_t s;
+      ~Restore() { c->io_stream = s; }
+    } restore{c, keep};
     if (acc) read_back(c, acc, h.acc.p, h.acc.bytes, "read tally");
     if (hist && h.hist.p) read_back(c, hist, h.hist.p, h.hist.bytes, "read hist");
     if (image && h.image.p) read_back(c, image, h.image.p, h.image.bytes, "read image");
@@ -1510,6 +1524,46 @@ int mdx_get_voxel_dose_f32(mdx_ctx* c, int grid, uint64_t n_histories, float* ou
                                    c->io_stream),
               "dose f32");
     read_back(c, out, c->d_dose32.p, nv * 8, "read dose f32");
+  });
+}
+
+int mdx_read_staged_dose_f32_async(mdx_ctx* c, float* out) {
+  return guard([&] {
+    need(c && out, "NULL argument");
+    Job& D = c->DJ();
+    need(D.staged_grid >= 0 && D.d_stage.p, "no staged dose (mdx_stage_voxel_dose_f32)");
+    need(!D.copy_pending, "a copy of this job's dose is in flight");
+    set_device_nowait(c);
+    if (!D.ev_copy) hip_check(hipEventCreateWithFlags(&D.ev_copy, hipEventDisableTiming), "hipEventCreate");
+    hip_check(hipStreamWaitEvent(c->io_stream, D.ev_stage, 0), "wait staged dose");
+    hip_check(hipMemcpyAsync(out, D.d_stage.p, D.d_stage.bytes, hipMemcpyDeviceToHost, c->io_stream), "read dose f32");
+    hip_check(hipEventRecord(D.ev_copy, c->io_stream), "hipEventRecord");
+    D.copy_pending = true;
+  });
+}
+
+int mdx_read_done(mdx_ctx* c, int job, int wait, int* done) {
+  return guard([&] {
+    need(c && done, "NULL argument");
+    Job& J = job_at(c, job);
+    Job& D = J.domain_of >= 0 ? job_at(c, J.domain_of) : J;
+    *done = 1;
+    if (!D.copy_pending) return;
+    set_device_nowait(c);
+    if (wait) {
+      hip_check(hipEventSynchronize(D.ev_copy), "read dose f32");
+    } else {
+      const hipError_t e = hipEventQuery(D.ev_copy);
+      if (e == hipErrorNotReady) {
+        (void)hipGetLastError();
+        *done = 0;
+        return;
+      }
+      hip_check(e, "read dose f32");
+    }
+    D.copy_pending = false;
+    D.staged_grid = -1;
+    D.d_stage.release();
   });
 }
 

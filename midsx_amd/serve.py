@@ -79,7 +79,7 @@ def read_dose(ctx, job, bufs, dose):
 
 
 class _Active:
-    __slots__ = ("job", "res", "slot", "done", "bytes")
+    __slots__ = ("job", "res", "slot", "done", "bytes", "image", "buf")
 
     def __init__(self, job, res, slot, nbytes):
         self.job = job
@@ -87,6 +87,8 @@ class _Active:
         self.slot = slot
         self.done = 0
         self.bytes = nbytes
+        self.image = None
+        self.buf = None
 
 
 class Server:
@@ -99,10 +101,12 @@ class Server:
     than a job has left).  phantoms: callable job -> uint8 grid (default:
     `scenes.dose_phantom`), so that a client can hand over its own data.
     dose: format of the dose read back ("f32" or "f64", see `read_dose`);
-    on_result(job, a, b, image) receives it (the buffers are reused)."""
+    on_result(job, a, b, image) receives it (the buffers are reused).
+    admit_slice: seconds of job set-up per round at most while others run
+    (the next round starts only when the loop gets back to it)."""
 
     def __init__(self, ctx, budget_bytes=None, round_histories=30_000_000, min_slice=100_000, max_active=512,
-                 graphs=True, phantoms=None, on_result=None, dose="f32"):
+                 graphs=True, phantoms=None, on_result=None, dose="f32", admit_slice=0.01):
         self.ctx = ctx
         if budget_bytes is None:
             free, _ = ctx.mem_info()
@@ -115,13 +119,24 @@ class Server:
         self.phantoms = phantoms or (lambda j: scenes.dose_phantom(j["shape"], j["voxel_cm"], j["phantom_seed"]))
         self.on_result = on_result
         self.dose = dose
+        self.admit_slice = float(admit_slice)
         self._buf = {}
+        self._free = {}  # pinned float32 buffers by size, for copies in flight
         self.rounds = 0
         self.kernel_ms = 0.0
         self.counters = {}
         self.host_s = dict(admit=0.0, read=0.0, wait=0.0, release=0.0, launch=0.0)
 
     # ------------------------------------------------------------------ helpers
+    def _pinned(self, n):
+        free = self._free.get(n)
+        if free:
+            return free.pop()
+        try:
+            return self.ctx.pinned(n, np.float32)
+        except Exception:
+            return np.empty(n, np.float32)
+
     def _admit(self, job, res, now):
         ctx = self.ctx
         slot = ctx.job_create()
@@ -173,9 +188,10 @@ class Server:
         active = []
         resident = 0
         to_read = []  # finished in the last round, results not yet copied
+        copying = []  # FP32 dose copies in flight (started beside a round)
         t0 = time.perf_counter()
         clock = lambda: time.perf_counter() - t0
-        while order or waiting or active or to_read:
+        while order or waiting or active or to_read or copying:
             now = clock()
             while order and arrivals[order[0]] <= now:
                 waiting.append(order.popleft())
@@ -190,11 +206,18 @@ class Server:
                     a.done += int(t_)
             t1 = clock()
             self.host_s["launch"] += t1 - t
-            # beside the round in flight: results of the jobs finished in the
-            # round before, then the set-up of newly admitted jobs
+            # beside the round in flight: the results of the jobs finished in
+            # the round before (FP32: the copies are started and left running),
+            # then the set-up of newly admitted jobs
             for a in to_read:
-                self._read(a)
-                a.res.finished = clock()
+                if self.dose == "f32":
+                    ctx.job_select(a.slot)
+                    a.buf = self._pinned(2 * int(np.prod(a.job["shape"])))
+                    ctx.read_staged_dose_f32_async(a.buf)
+                    copying.append(a)
+                else:
+                    self._read(a)
+                    a.res.finished = clock()
             t2 = clock()
             self.host_s["read"] += t2 - t1
             admitted = []
@@ -202,9 +225,11 @@ class Server:
             while waiting and len(active) + len(admitted) < self.max_active:
                 if not launched and queued >= self.round_histories:
                     break  # start the first round; admit the rest beside it
+                if (active or admitted) and clock() - t2 > self.admit_slice:
+                    break  # leave the rest for the next rounds: the device waits for this loop
                 i = waiting[0]
                 need = scenes.dose_job_bytes(jobs[i])
-                if resident + need > self.budget and (active or to_read or admitted):
+                if resident + need > self.budget and (active or to_read or admitted or copying):
                     break
                 waiting.popleft()
                 a = self._admit(jobs[i], results[i], clock())
@@ -222,17 +247,37 @@ class Server:
                     if k != "kernel_ms":
                         self.counters[k] = self.counters.get(k, 0) + v
             t = clock()
-            for a in to_read:
-                ctx.job_release(a.slot)
-                resident -= a.bytes
+            # finished copies: results out, device memory freed
+            block = not launched and not admitted
+            still = []
+            for a in copying:
+                if ctx.read_done(a.slot, wait=block):
+                    buf = a.buf
+                    if self.on_result:
+                        a.res.summary = self.on_result(a.job, buf[0::2], buf[1::2], a.image)
+                    a.res.histories = a.done
+                    a.res.finished = clock()
+                    self._free.setdefault(buf.size, []).append(buf)
+                    a.buf = None
+                    ctx.job_release(a.slot)
+                    resident -= a.bytes
+                    block = False
+                else:
+                    still.append(a)
+            copying = still
+            if self.dose != "f32":
+                for a in to_read:
+                    ctx.job_release(a.slot)
+                    resident -= a.bytes
             self.host_s["release"] += clock() - t
-            idle = not launched and not to_read and not admitted
+            idle = not launched and not to_read and not admitted and not copying
             to_read = [a for a in active if a.done >= a.job["histories"]]
             active = [a for a in active if a.done < a.job["histories"]] + admitted
             if self.dose == "f32":
                 for a in to_read:  # conversion queued ahead of the next round; the copy runs beside it
                     ctx.job_select(a.slot)
                     ctx.stage_voxel_dose_f32(0, a.job["histories"])
+                    a.image = ctx.tally(0)["image"]
             if idle and order:
                 time.sleep(max(0.0, min(arrivals[order[0]] - clock(), 0.05)))
         self.makespan = clock()

@@ -112,3 +112,24 @@ def test_dose_f32_read_matches_f64(ctx):
     mean2, rel2 = ctx.voxel_dose_f32(0, n)
     assert np.array_equal(mean2, mean) and np.array_equal(rel2, rel)
     ctx.job_release(slot)
+
+
+def test_served_f32_reads_match_f64(ctx):
+    """The FP32 (mean, relative error) results, converted on the device and
+    copied beside later rounds, equal the FP64 sums read synchronously."""
+    jobs = scenes.draw_dose_jobs(6, 8, shapes=(16, 24), h_lo=5e4, h_hi=2e5)
+    budget = 2 * max(scenes.dose_job_bytes(j) for j in jobs)
+    a = Server(ctx, budget_bytes=budget, round_histories=150_000, min_slice=20_000, on_result=keep_arrays,
+               dose="f32").run(jobs)
+    b = Server(ctx, budget_bytes=budget, round_histories=150_000, min_slice=20_000, on_result=keep_arrays,
+               dose="f64").run(jobs)
+    for ra, rb, j in zip(a, b, jobs):
+        n = j["histories"]
+        mean, rel, im_a = ra.summary
+        s, s2, im_b = rb.summary
+        m = s / n
+        r = np.where(m > 0, np.sqrt(np.maximum(s2 / n - m * m, 0) / (n - 1)) / np.where(m > 0, m, 1), 0)
+        assert np.count_nonzero(m) > 100
+        assert np.allclose(mean, m, rtol=1e-6, atol=0)
+        assert np.allclose(rel, r, rtol=1e-4, atol=1e-6)
+        assert np.allclose(im_a, im_b, rtol=1e-12, atol=1e-9)

@@ -77,7 +77,18 @@ class FakeCtx:
         return np.ones(n), np.ones(n), None
 
     def stage_voxel_dose_f32(self, g, n_histories):
-        pass
+        self.slots[self.cur]["staged"] = True
+
+    def read_staged_dose_f32_async(self, out):
+        assert self.slots[self.cur].get("staged")
+        self.slots[self.cur]["copy"] = 2  # done after two polls
+
+    def read_done(self, job, wait=False):
+        s = self.slots[job]
+        if wait:
+            s["copy"] = 0
+        s["copy"] = max(0, s.get("copy", 0) - 1)
+        return s["copy"] == 0
 
     def voxel_dose_f32(self, g, n_histories, out=None):
         n = int(np.prod(self.slots[self.cur]["shape"]))
