@@ -106,6 +106,44 @@ struct StepRng {
   }
 };
 
+// ------------------------------------------------------------------ pointers
+// A pointer read from memory is a generic one, for which the compiler emits
+// flat memory instructions (address-space checks, 64-bit VGPR addresses, and
+// waits that also cover LDS traffic).  Every table, grid and tally buffer
+// that the kernels reach through such a pointer is device global memory, so
+// their loads and atomics go through address space 1.
+#define MDX_GLOBAL __attribute__((address_space(1)))
+
+template <class T>
+__device__ __forceinline__ T gld(const T* p) {
+  return *(const MDX_GLOBAL T*)p;
+}
+
+typedef float mdx_v4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ mdx_f4 gld(const mdx_f4* p) {
+  const mdx_v4 a = *(const MDX_GLOBAL mdx_v4*)p;
+  mdx_f4 r;
+  r.x = a.x; r.y = a.y; r.z = a.z; r.w = a.w;
+  return r;
+}
+
+typedef float mdx_v2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ mdx_node gld(const mdx_node* p) {  // the padding is not read
+  const mdx_v4 a = *(const MDX_GLOBAL mdx_v4*)p;
+  const mdx_v2 b = *(const MDX_GLOBAL mdx_v2*)((const MDX_GLOBAL float*)p + 4);
+  mdx_node r;
+  r.a_hi = a.x; r.a_lo = a.y; r.c1 = a.z; r.c2 = a.w;
+  r.c3 = b.x; r.x = b.y; r.pad0 = 0.0f; r.pad1 = 0.0f;
+  return r;
+}
+
+template <class T>
+__device__ __forceinline__ T gadd(T* p, T v) {  // atomicAdd on global memory; returns the old value
+  return __hip_atomic_fetch_add((MDX_GLOBAL T*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // ------------------------------------------------------------------ curves
 // ln(X / xk) for X, xk > 0, with the rounding error of the quotient folded
 // back in (the quotient's exact residual comes from one fma).
@@ -115,38 +153,77 @@ __device__ __forceinline__ float log_ratio(float X, float xk) {
   return logf(r) + e / (r * xk);
 }
 
-__device__ __forceinline__ int curve_bracket(const mdx_tables_dev* T, const mdx_curve& c, float X) {
-  const float* xs = T->xs + c.off;
+// A curve lookup is two dependent loads: the hash bin of ln X gives the
+// largest node index k with x_k at or below the bin's lower edge, clamped to
+// [0, n-2], and the node pair (k, k+1) then brackets X unless the bin holds
+// further nodes, in which case a short walk over the abscissae follows.  The
+// result is the largest k in [0, n-2] with x_k <= X (0 below x_0), the same
+// index as upper_bound - 1 clamped to the last span.
+struct CurveAt {
+  int k;
+  mdx_node r;  // node k
+  float xn;    // abscissa of node k + 1
+};
+
+__device__ __forceinline__ int curve_hash(const mdx_tables_dev* T, const mdx_curve& c, float X) {
   int b = (int)((logf(X) - c.lnx0) * c.inv_dlnx);
   b = b < 0 ? 0 : (b >= c.nh ? c.nh - 1 : b);
-  int k = T->hash[c.hoff + b];
-  while (k > 0 && xs[k] > X) --k;
-  while (k + 1 < c.n && xs[k + 1] <= X) ++k;
-  return k;
+  const int k = gld(T->hash + c.hoff + b);
+  return k > c.n - 2 ? c.n - 2 : k;
 }
 
-// Evaluates a packed curve (see mdx_types.h).  Linear curves clamp outside
-// their node range (interpolators.cpp:48-54); spline curves extrapolate with
-// their end spans (Eigen Spline::Span).
-__device__ __forceinline__ float curve_eval(const mdx_tables_dev* T, const mdx_curve& c, float X) {
-  const float* xs = T->xs + c.off;
+__device__ __forceinline__ CurveAt curve_nodes(const mdx_tables_dev* T, const mdx_curve& c, int k) {
   const mdx_node* nd = T->nodes + c.off;
-  int k;
+  return CurveAt{k, gld(nd + k), gld(&nd[k + 1].x)};
+}
+
+__device__ __forceinline__ void curve_fix(const mdx_tables_dev* T, const mdx_curve& c, float X, CurveAt& a) {
+  const int kmax = c.n - 2;
+  if ((X < a.r.x && a.k > 0) || (X >= a.xn && a.k < kmax)) {
+    const float* xs = T->xs + c.off;
+    int k = a.k;
+    while (k > 0 && gld(xs + k) > X) --k;
+    while (k < kmax && gld(xs + k + 1) <= X) ++k;
+    a = curve_nodes(T, c, k);
+  }
+}
+
+// Linear curves clamp outside their node range (interpolators.cpp:48-54);
+// spline curves extrapolate with their end spans (Eigen Spline::Span).
+__device__ __forceinline__ float curve_finish(const mdx_tables_dev* T, const mdx_curve& c, float X, CurveAt a) {
   float t;
   if (c.kind == 0) {
-    if (X <= xs[0]) return nd[0].a_hi + nd[0].a_lo;
-    if (X >= xs[c.n - 1]) return nd[c.n - 1].a_hi + nd[c.n - 1].a_lo;
-    k = curve_bracket(T, c, X);
-    t = log_ratio(X, xs[k]);
+    if (X <= a.r.x) return a.r.a_hi + a.r.a_lo;  // X <= x_0 (or exactly on node k)
+    if (X >= a.xn) {                             // X >= x_{n-1}
+      const mdx_node e = gld(T->nodes + c.off + a.k + 1);
+      return e.a_hi + e.a_lo;
+    }
+    t = log_ratio(X, a.r.x);
   } else {
-    k = curve_bracket(T, c, X);
-    k = k > c.n - 2 ? c.n - 2 : k;
-    t = X >= xs[k] ? log_ratio(X, xs[k]) : -log_ratio(xs[k], X);
+    t = X >= a.r.x ? log_ratio(X, a.r.x) : -log_ratio(a.r.x, X);
   }
-  const mdx_node r = nd[k];
-  float a = t * (r.c1 + t * (r.c2 + t * r.c3));
-  float em1 = expm1f(a);
+  const mdx_node& r = a.r;
+  const float p = t * (r.c1 + t * (r.c2 + t * r.c3));
+  const float em1 = expm1f(p);
   return __builtin_fmaf(r.a_hi, em1, __builtin_fmaf(r.a_lo, em1, r.a_lo)) + r.a_hi;
+}
+
+// Evaluates a packed curve (see mdx_types.h).
+__device__ __forceinline__ float curve_eval(const mdx_tables_dev* T, const mdx_curve& c, float X) {
+  CurveAt a = curve_nodes(T, c, curve_hash(T, c, X));
+  curve_fix(T, c, X, a);
+  return curve_finish(T, c, X, a);
+}
+
+// Two curves at once, their loads issued side by side.
+__device__ __forceinline__ void curve_eval2(const mdx_tables_dev* T, const mdx_curve& c1, float X1,
+                                            const mdx_curve& c2, float X2, float& v1, float& v2) {
+  const int k1 = curve_hash(T, c1, X1), k2 = curve_hash(T, c2, X2);
+  CurveAt a1 = curve_nodes(T, c1, k1), a2 = curve_nodes(T, c2, k2);
+  curve_fix(T, c1, X1, a1);
+  curve_fix(T, c2, X2, a2);
+  v1 = curve_finish(T, c1, X1, a1);
+  v2 = curve_finish(T, c2, X2, a2);
 }
 
 // ------------------------------------------------------------------ geometry
@@ -239,12 +316,12 @@ __device__ __forceinline__ float rita_row(const mdx_tables_dev* T, int row0, int
   int lo = 0, hi = m;
   while (lo < hi) {
     int mid = (lo + hi) >> 1;
-    if (R[row0 + mid].y < y) lo = mid + 1; else hi = mid;
+    if (gld(&R[row0 + mid].y) < y) lo = mid + 1; else hi = mid;
   }
   int i = lo - 1;
   if (i == m - 1) return 2.0f;
   if (i < 0) return 0.0f;
-  mdx_f4 a = R[row0 + i], b = R[row0 + i + 1];
+  const mdx_f4 a = gld(R + row0 + i), b = gld(R + row0 + i + 1);
   float nu = y - a.y;
   float d = b.y - a.y;
   return a.x + (1.0f + a.z + a.w) * d * nu / (d * d + a.z * d * nu + a.w * nu * nu) * (b.x - a.x);
@@ -258,14 +335,15 @@ __device__ __forceinline__ float rita_sample(const mdx_tables_dev* T, const mdx_
   int lo = 0, hi = M.rita_ne;
   while (lo < hi) {
     int mid = (lo + hi) >> 1;
-    if (Eg[mid] < E) lo = mid + 1; else hi = mid;
+    if (gld(Eg + mid) < E) lo = mid + 1; else hi = mid;
   }
   int il = lo - 1;
   il = il < 0 ? 0 : (il > M.rita_ne - 2 ? M.rita_ne - 2 : il);
   const int* ro = T->rita_row_off + M.rita_roff;
-  float wl = rita_row(T, ro[il], ro[il + 1], y);
-  float wu = rita_row(T, ro[il + 1], ro[il + 2], y);
-  float f = log_ratio(E, Eg[il]) * T->rita_inv_dlnE[M.rita_eoff + il];
+  const int r0 = gld(ro + il), r1 = gld(ro + il + 1), r2 = gld(ro + il + 2);
+  float wl = rita_row(T, r0, r1, y);
+  float wu = rita_row(T, r1, r2, y);
+  float f = log_ratio(E, gld(Eg + il)) * gld(T->rita_inv_dlnE + M.rita_eoff + il);
   float w = (wl > 0.0f && wu > 0.0f) ? wl * expf(f * log_ratio(wu, wl)) : wl;
   return w - 1.0f;
 }

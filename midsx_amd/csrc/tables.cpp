@@ -497,6 +497,8 @@ static void split(double a, float& hi, float& lo) {
 }
 
 static void build_hash(Packed& p, mdx_curve& c, double lo, double hi) {
+  // the device reads the node pair (k, k + 1) of every lookup
+  if (c.n < 2) throw std::runtime_error("a packed curve needs at least two nodes");
   const float* xs = p.xs.data() + c.off;
   lo = std::max(lo, (double)xs[0]);
   hi = std::min(hi, (double)xs[c.n - 1]);

@@ -70,7 +70,7 @@ __device__ void flush_hist(const mdx_tallies_dev* TL, const int* off, double* lh
     const mdx_tally_dev& T = TL->t[t];
     for (int i = threadIdx.x; i < MDX_NBUCKET * T.hist_bins; i += blockDim.x) {
       double v = lhist[off[t] + i];
-      if (v != 0.0) atomicAdd(T.hist + i, v);
+      if (v != 0.0) gadd(T.hist + i, v);
     }
   }
 }
@@ -81,12 +81,17 @@ __device__ void flush_hist(const mdx_tallies_dev* TL, const int* off, double* lh
 // value, so the stored sum is fl(old + v) and TwoSum gives the exact error,
 // which goes to p[1].  hi + lo then carries the sum to the precision of the
 // error sum lo, itself an FP32 sum of values below half an ulp of hi.
+// GLOBAL: p is in global memory (else in LDS).
+template <bool GLOBAL>
 __device__ __forceinline__ void compensated_add(float* p, float v) {
-  const float old = atomicAdd(p, v);
+  const float old = GLOBAL ? gadd(p, v) : atomicAdd(p, v);
   const float s = __fadd_rn(old, v);
   const float bb = __fsub_rn(s, old);
   const float err = __fadd_rn(__fsub_rn(old, __fsub_rn(s, bb)), __fsub_rn(v, bb));
-  if (err != 0.0f) atomicAdd(p + 1, err);
+  if (err != 0.0f) {
+    if (GLOBAL) gadd(p + 1, err);
+    else atomicAdd(p + 1, err);
+  }
 }
 
 // One energy deposit in voxel lin of grid G (physics_engine.cpp:70-75,
@@ -95,23 +100,23 @@ __device__ __forceinline__ void voxel_deposit(const mdx_grid_dev& G, uint64_t li
                                               float* ldose = nullptr) {
   if (G.dose_mode == MDX_DOSE_F32) {
     if (G.dose32) {
-      atomicAdd(G.dose32 + 2 * lin, dep);
-      atomicAdd(G.dose32 + 2 * lin + 1, dep * dep);
+      gadd(G.dose32 + 2 * lin, dep);
+      gadd(G.dose32 + 2 * lin + 1, dep * dep);
     }
   } else if (G.dose_mode == MDX_DOSE_F32C || G.dose_mode == MDX_DOSE_F32C_LDS) {
     if (G.dose32) {
-      if (ldose) compensated_add(ldose + 2 * lin, dep);  // this block's sum, in LDS
-      else compensated_add(G.dose32 + 4 * lin, dep);
+      if (ldose) compensated_add<false>(ldose + 2 * lin, dep);  // this block's sum, in LDS
+      else compensated_add<true>(G.dose32 + 4 * lin, dep);
       const float q = __fmul_rn(dep, dep);  // dep^2 = q + ql exactly (never contracted into an fma)
       const float ql = __builtin_fmaf(dep, dep, -q);
-      compensated_add(G.dose32 + 4 * lin + 2, q);
-      if (ql != 0.0f) atomicAdd(G.dose32 + 4 * lin + 3, ql);
+      compensated_add<true>(G.dose32 + 4 * lin + 2, q);
+      if (ql != 0.0f) gadd(G.dose32 + 4 * lin + 3, ql);
     }
   } else if (G.dose) {
-    atomicAdd(G.dose + 2 * lin, (double)dep);
-    atomicAdd(G.dose + 2 * lin + 1, (double)dep * (double)dep);
+    gadd(G.dose + 2 * lin, (double)dep);
+    gadd(G.dose + 2 * lin + 1, (double)dep * (double)dep);
   }
-  if (G.dose_n) atomicAdd(G.dose_n + lin, 1u);
+  if (G.dose_n) gadd(G.dose_n + lin, 1u);
 }
 
 __device__ void process_tallies(const mdx_tallies_dev* TL, double* acc, const Step& r, uint32_t& surf_hits,
@@ -169,9 +174,9 @@ __device__ void process_tallies(const mdx_tallies_dev* TL, double* acc, const St
         px = px >= T.nx ? T.nx - 1 : (px < 0 ? 0 : px);
         py = py >= T.ny ? T.ny - 1 : (py < 0 ? 0 : py);
         double* im = T.image + 3 * ((size_t)py * T.nx + px);
-        atomicAdd(im, (double)r.E);
-        atomicAdd(im + 1, (double)r.E * (double)r.E);
-        atomicAdd(im + 2, 1.0);
+        gadd(im, (double)r.E);
+        gadd(im + 1, (double)r.E * (double)r.E);
+        gadd(im + 2, 1.0);
       }
       if (r.E != 0.0f) lds_event(acc, t, MDX_SQ_INCIDENT_ENERGY, b, (double)r.E);
       lds_event(acc, t, MDX_SQ_ENTRANCE_COSINE, b, (double)cosv);
@@ -181,7 +186,7 @@ __device__ void process_tallies(const mdx_tallies_dev* TL, double* acc, const St
         if (fb >= 0.0f && fb < (float)T.hist_bins) {
           const int bin = b * T.hist_bins + (int)fb;
           if (hoff[t] >= 0) atomicAdd(lhist + hoff[t] + bin, 1.0);
-          else atomicAdd(T.hist + bin, 1.0);
+          else gadd(T.hist + bin, 1.0);
         }
       }
     }
@@ -233,10 +238,10 @@ __device__ void generate(const mdx_source_dev* S, Philox& rng, float& px, float&
     int lo = 0, hi = S->n_bins;
     while (lo < hi) {
       int mid = (lo + hi) >> 1;
-      if (S->cdf[mid] < u) lo = mid + 1; else hi = mid;
+      if (gld(S->cdf + mid) < u) lo = mid + 1; else hi = mid;
     }
     lo = lo >= S->n_bins ? S->n_bins - 1 : lo;
-    E = S->energies[lo];
+    E = gld(S->energies + lo);
   } else {
     E = S->energy;
   }
@@ -254,26 +259,34 @@ __device__ float compton(const mdx_tables_dev* T, const mdx_material_dev& M, Phi
   const float k = E * kInvMc2;
   const float r1p = (1.0f + 2.0f * k) / (9.0f + 2.0f * k);
   const float s_max = curve_eval(T, M.sf, kAlphaSqrt2 * k);
-  float chi = 1.0f;
-  for (uint32_t it = 0;; ++it) {
-    ++trials;
-    float r1 = rng.uniform(), r2 = rng.uniform(), r3 = rng.uniform();
-    if (r1 <= r1p) {
-      chi = 1.0f + 2.0f * k * r2;
-      if (!(r3 <= 4.0f * (1.0f / chi - 1.0f / (chi * chi))) && it < kMaxKahnTrials) continue;
-    } else {
-      chi = (1.0f + 2.0f * k) / (1.0f + 2.0f * k * r2);
-      float q = 1.0f / k - chi / k + 1.0f;
-      if (!(2.0f * r3 <= q * q + 1.0f / chi) && it < kMaxKahnTrials) continue;
+  // Trial it draws (r1, r2, r3) for Kahn's test and, once that passes (or
+  // the trial cap is reached), r4 for the S(x)/S_max test.  The Kahn draws
+  // loop on their own so that a wave runs the table lookup of the S-test
+  // once per round of S-tests rather than once per Kahn trial.
+  uint32_t it = 0;
+  for (;;) {
+    float chi;
+    for (;; ++it) {
+      ++trials;
+      const float r1 = rng.uniform(), r2 = rng.uniform(), r3 = rng.uniform();
+      if (r1 <= r1p) {
+        chi = 1.0f + 2.0f * k * r2;
+        if (r3 <= 4.0f * (1.0f / chi - 1.0f / (chi * chi)) || it >= kMaxKahnTrials) break;
+      } else {
+        chi = (1.0f + 2.0f * k) / (1.0f + 2.0f * k * r2);
+        const float q = 1.0f / k - chi / k + 1.0f;
+        if (2.0f * r3 <= q * q + 1.0f / chi || it >= kMaxKahnTrials) break;
+      }
     }
     ++s_tests;
-    float one_minus_mu = (chi - 1.0f) / k;  // 1 - mu, mu = 1 + 1/k - 1/k'
-    float x = kAlpha * k * sqrtf(one_minus_mu);
-    float sx = curve_eval(T, M.sf, x);
+    const float one_minus_mu = (chi - 1.0f) / k;  // 1 - mu, mu = 1 + 1/k - 1/k'
+    const float x = kAlpha * k * sqrtf(one_minus_mu);
+    const float sx = curve_eval(T, M.sf, x);
     if (rng.uniform() <= sx / s_max || it >= kMaxKahnTrials) {
       mu = 1.0f - one_minus_mu;
       return E / chi;
     }
+    ++it;
   }
 }
 
@@ -328,7 +341,7 @@ k_transport_plain(const mdx_tables_dev* __restrict__ T, const mdx_domain_dev* __
       }
       uint64_t lin = 0;
       int g = voxel_lookup(D, px, py, pz, lin);
-      int slot = g < 0 ? D->bg_slot : (int)T->id_to_slot[D->grids[g].ids[lin]];
+      int slot = g < 0 ? D->bg_slot : (int)T->id_to_slot[gld(D->grids[g].ids + lin)];
       const mdx_material_dev& M = T->mat[slot];
       if (slot != cslot) {
         stot = curve_eval(T, M.total, E);
@@ -339,8 +352,8 @@ k_transport_plain(const mdx_tables_dev* __restrict__ T, const mdx_domain_dev* __
       if (!(rng.uniform() < 1.0f - stot / smax)) {
         ++c_real;
         r.interact = true;
-        float sc = curve_eval(T, M.coh, E);
-        float si = curve_eval(T, M.incoh, E);
+        float sc, si;
+        curve_eval2(T, M.coh, E, M.incoh, E, sc, si);
         int type = select_type(sc, si, stot, rng.uniform());
         float dep = 0.0f;
         if (type == 1) {  // photon_interactions.cpp:50-60
@@ -371,9 +384,9 @@ k_transport_plain(const mdx_tables_dev* __restrict__ T, const mdx_domain_dev* __
           if (g >= 0) {
             voxel_deposit(D->grids[g], lin, dep);
           } else if (D->bg_dose) {
-            atomicAdd(D->bg_dose, (double)dep);
-            atomicAdd(D->bg_dose + 1, (double)dep * (double)dep);
-            atomicAdd(D->bg_dose + 2, 1.0);
+            gadd(D->bg_dose, (double)dep);
+            gadd(D->bg_dose + 1, (double)dep * (double)dep);
+            gadd(D->bg_dose + 2, 1.0);
           }
         }
       }
@@ -397,7 +410,7 @@ k_transport_plain(const mdx_tables_dev* __restrict__ T, const mdx_domain_dev* __
     double v = acc[i];
     if (v != 0.0) {
       int t = i / kAccPerTally;
-      atomicAdd(TL->t[t].acc + (i - t * kAccPerTally), v);
+      gadd(TL->t[t].acc + (i - t * kAccPerTally), v);
     }
   }
   flush_hist(TL, hoff, lhist);
@@ -435,9 +448,9 @@ __device__ __forceinline__ void event_global(double* acc, int q, int b, double v
       s2 += __shfl(v2, src);
     }
     if (lane == leader) {
-      atomicAdd(p, (double)__popcll(same));
-      atomicAdd(p + 1, s);
-      atomicAdd(p + 2, s2);
+      gadd(p, (double)__popcll(same));
+      gadd(p + 1, s);
+      gadd(p + 2, s2);
     }
   }
 }
@@ -454,7 +467,7 @@ __device__ __forceinline__ void wave_add(double* p, double v) {
     todo &= ~same;
     double s = 0.0;
     for (unsigned long long m = same; m; m &= m - 1) s += __shfl(v, __ffsll((long long)m) - 1);
-    if (lane == leader) atomicAdd(p, s);
+    if (lane == leader) gadd(p, s);
   }
 }
 
@@ -518,9 +531,9 @@ __device__ void tallies_step(const mdx_tallies_dev* TL, double* acc, const Step&
         px = px >= T.nx ? T.nx - 1 : (px < 0 ? 0 : px);
         py = py >= T.ny ? T.ny - 1 : (py < 0 ? 0 : py);
         double* im = T.image + 3 * ((size_t)py * T.nx + px);
-        atomicAdd(im, (double)r.E);
-        atomicAdd(im + 1, (double)r.E * (double)r.E);
-        atomicAdd(im + 2, 1.0);
+        gadd(im, (double)r.E);
+        gadd(im + 1, (double)r.E * (double)r.E);
+        gadd(im + 2, 1.0);
       }
       if (r.E != 0.0f) event_global(T.acc, MDX_SQ_INCIDENT_ENERGY, b, (double)r.E);
       event_global(T.acc, MDX_SQ_ENTRANCE_COSINE, b, (double)cosv);
@@ -708,21 +721,44 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
     if (__ballot(alive) == 0ull) break;
 
     // ---- one Woodcock step for every lane that can step (physics_engine.cpp:38-66)
+    // Written so that the costly parts (a cross-section evaluation, a tally
+    // replay, the next tally event) each have one site that all lanes
+    // needing it run together, whatever branch brought them there.
     if (alive && !pend) {
       ++c_step;
       ++nstep;
       float u1, u2;
       srng.next(u1, u2);
       const float x0 = px, y0 = py, z0 = pz;
-      if (fly) {
+      bool escaped = false;
+      int g = -1;
+      uint64_t lin = 0;
+      float s = 0.0f;
+      int slot = D->bg_slot;  // free flights: the background's cross section
+      if (!fly) {
+        s = -__log2f(u1) * step_scale;
+        px = px + s * dx;
+        py = py + s * dy;
+        pz = pz + s * dz;
+        escaped = !in_domain(D, px, py, pz);
+        if (!escaped) {
+          g = voxel_lookup(D, px, py, pz, lin);
+          slot = g < 0 ? D->bg_slot : (int)T->id_to_slot[gld(D->grids[g].ids + lin)];
+        }
+      }
+      if (!escaped && slot != cslot) {
+        stot = curve_eval(T, T->mat[slot].total, E);
+        thr = 1.0f - stot / smax;
+        cslot = slot;
+      }
+      bool replay = escaped;  // replay this step's segment for the tallies
+      float seg = s;
+      if (escaped) {
+        ++c_esc;
+        alive = false;
+      } else if (fly) {
         // background free flight: distance at the background's own cross
         // section, cut at the next grid face or the domain's boundary
-        const int bgs = D->bg_slot;
-        if (cslot != bgs) {
-          stot = curve_eval(T, T->mat[bgs].total, E);
-          thr = 1.0f - stot / smax;
-          cslot = bgs;
-        }
         const float si = -__log2f(u1) * (0.69314718055994531f / stot);
         float dgrid, dexit;
         flight_bounds(D, x0, y0, z0, dx, dy, dz, dgrid, dexit);
@@ -740,57 +776,33 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
           pz = z0 + dgrid * dz;
           fly = false;
           tfl -= dgrid;
-          if (tfl <= 0.0f) {
-            Step r{x0, y0, z0, dx, dy, dz, dgrid, E, nc, ni, 0, 0, 0.0f, false};
-            tallies_step<MULTI>(TL, acc, r, c_surf, lhist, hoff);
-            tfl = next_tally_event(TL, px, py, pz, dx, dy, dz);
-          }
+          replay = tfl <= 0.0f;
+          seg = dgrid;
           if (nstep >= kMaxSteps) alive = false;
         } else {  // out of the domain, overshooting by a majorant step
-          const float s = dexit - __log2f(u2) * step_scale;
-          px = x0 + s * dx;
-          py = y0 + s * dy;
-          pz = z0 + s * dz;
+          seg = dexit - __log2f(u2) * step_scale;
+          px = x0 + seg * dx;
+          py = y0 + seg * dy;
+          pz = z0 + seg * dz;
           ++c_esc;
-          Step r{x0, y0, z0, dx, dy, dz, s, E, nc, ni, 0, 0, 0.0f, false};
-          tallies_step<MULTI>(TL, acc, r, c_surf, lhist, hoff);
+          replay = true;
           alive = false;
         }
+      } else if (!(u2 < thr)) {  // real interaction: park it for the interaction phase
+        pend = true;
+        qx = x0; qy = y0; qz = z0; qs = s;
+        qg = g;
+        qlin = lin;
       } else {
-        const float s = -__log2f(u1) * step_scale;
-        px = px + s * dx;
-        py = py + s * dy;
-        pz = pz + s * dz;
-        if (!in_domain(D, px, py, pz)) {
-          ++c_esc;
-          Step r{x0, y0, z0, dx, dy, dz, s, E, nc, ni, 0, 0, 0.0f, false};
-          tallies_step<MULTI>(TL, acc, r, c_surf, lhist, hoff);
-          alive = false;
-        } else {
-          uint64_t lin = 0;
-          const int g = voxel_lookup(D, px, py, pz, lin);
-          const int slot = g < 0 ? D->bg_slot : (int)T->id_to_slot[D->grids[g].ids[lin]];
-          if (slot != cslot) {
-            stot = curve_eval(T, T->mat[slot].total, E);
-            thr = 1.0f - stot / smax;
-            cslot = slot;
-          }
-          if (!(u2 < thr)) {  // real interaction: park it for the interaction phase
-            pend = true;
-            qx = x0; qy = y0; qz = z0; qs = s;
-            qg = g;
-            qlin = lin;
-          } else {
-            tfl -= s;
-            if (tfl <= 0.0f) {  // the step may reach a tally: replay it
-              Step r{x0, y0, z0, dx, dy, dz, s, E, nc, ni, 0, 0, 0.0f, false};
-              tallies_step<MULTI>(TL, acc, r, c_surf, lhist, hoff);
-              tfl = next_tally_event(TL, px, py, pz, dx, dy, dz);
-            }
-            if (nstep >= kMaxSteps) alive = false;
-            fly = flights && g < 0;
-          }
-        }
+        tfl -= s;
+        replay = tfl <= 0.0f;  // the step may reach a tally
+        if (nstep >= kMaxSteps) alive = false;
+        fly = flights && g < 0;
+      }
+      if (replay) {
+        Step r{x0, y0, z0, dx, dy, dz, seg, E, nc, ni, 0, 0, 0.0f, false};
+        tallies_step<MULTI>(TL, acc, r, c_surf, lhist, hoff);
+        if (alive) tfl = next_tally_event(TL, px, py, pz, dx, dy, dz);
       }
     }
 
@@ -802,8 +814,8 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
         ++c_real;
         const mdx_material_dev& M = T->mat[cslot];
         Step r{qx, qy, qz, dx, dy, dz, qs, E, nc, ni, 0, 0, 0.0f, true};
-        const float sc = curve_eval(T, M.coh, E);
-        const float si = curve_eval(T, M.incoh, E);
+        float sc, si;
+        curve_eval2(T, M.coh, E, M.incoh, E, sc, si);
         const int type = select_type(sc, si, stot, rng.uniform());
         float dep = 0.0f;
         if (type == 1) {
@@ -835,9 +847,9 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
           if (qg >= 0) {
             voxel_deposit(D->grids[qg], qlin, dep, qg == 0 ? ldose : nullptr);
           } else if (D->bg_dose) {
-            atomicAdd(D->bg_dose, (double)dep);
-            atomicAdd(D->bg_dose + 1, (double)dep * (double)dep);
-            atomicAdd(D->bg_dose + 2, 1.0);
+            gadd(D->bg_dose, (double)dep);
+            gadd(D->bg_dose + 1, (double)dep * (double)dep);
+            gadd(D->bg_dose + 2, 1.0);
           }
         }
         tallies_step<MULTI>(TL, acc, r, c_surf, lhist, hoff);
@@ -864,7 +876,7 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
       double v = acc[i];
       if (v != 0.0) {
         int t = i / kAccPerTally;
-        atomicAdd(jobs[0].TL->t[t].acc + (i - t * kAccPerTally), v);
+        gadd(jobs[0].TL->t[t].acc + (i - t * kAccPerTally), v);
       }
     }
     flush_hist(jobs[0].TL, hoff, lhist);
@@ -872,8 +884,8 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
       float* g = jobs[0].D->grids[0].dose32;
       for (uint32_t i = threadIdx.x; i < lds_voxels; i += blockDim.x) {
         const float hi = ldose[2 * i], lo = ldose[2 * i + 1];
-        if (hi != 0.0f) compensated_add(g + 4 * i, hi);
-        if (lo != 0.0f) atomicAdd(g + 4 * i + 1, lo);
+        if (hi != 0.0f) compensated_add<true>(g + 4 * i, hi);
+        if (lo != 0.0f) gadd(g + 4 * i + 1, lo);
       }
     }
   }
