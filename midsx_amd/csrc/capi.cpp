@@ -1303,7 +1303,7 @@ int mdx_set_dose_precision(mdx_ctx* c, int mode) {
     need(c->J().domain_of < 0, "voxel dose belongs to the job that owns the domain");
     if (mode == MDX_DOSE_F32C_LDS) {
       need(!c->J().grids.empty(), "MDX_DOSE_F32C_LDS needs a grid");
-      const size_t static_lds = 24 * 1024;  // the queue kernel's own LDS (tallies, spectra, counters), rounded up
+      const size_t static_lds = mdx_launch::transport_queue_static_lds(c->device >= 0);
       const size_t need_b = c->J().grids[0]->voxels() * 8;
       if (need_b + static_lds > c->lds_optin)
         throw std::invalid_argument("grid 0 has " + std::to_string(c->J().grids[0]->voxels()) +

@@ -16,6 +16,9 @@ hipError_t transport_plain(const mdx_tables_dev* T, const mdx_domain_dev* D, con
 hipError_t transport_queue(const mdx_tables_dev* T, const mdx_job_dev* jobs, const uint64_t* prefix, int n_jobs,
                            uint64_t total, unsigned long long* work, unsigned long long* counters, int grid_blocks,
                            int refill_min, int interact_min, uint32_t lds_voxels, hipStream_t st);
+// Static LDS of the one-job queue kernel (from the runtime when on_device,
+// else an upper bound); MDX_DOSE_F32C_LDS adds 8 B per voxel of grid 0.
+size_t transport_queue_static_lds(bool on_device);
 // Graph form of the multi-job queue kernel: reads the job count, total,
 // prefix and job records from a device round buffer.
 hipError_t transport_queue_round(const mdx_tables_dev* T, const char* round, unsigned long long* work,

@@ -623,6 +623,24 @@ __device__ __forceinline__ void flight_bounds(const mdx_domain_dev* D, float x, 
 #define MDX_QUEUE_WAVES_PER_EU 4
 #endif
 
+static_assert(sizeof(mdx_material_dev) % 4 == 0 && sizeof(mdx_domain_dev) % 4 == 0 &&
+                  sizeof(mdx_source_dev) % 4 == 0 && sizeof(mdx_tallies_dev) % 4 == 0,
+              "descriptors are copied to LDS in 4-byte words");
+// An upper bound of the one-job queue kernel's static LDS (its shared arrays
+// plus 1 KB for alignment), for hosts that cannot ask the runtime.
+constexpr size_t kQueueStaticLdsBound =
+    sizeof(double) * (MDX_MAX_TALLIES * kAccPerTally + kLdsHist) + sizeof(unsigned long long) * MDX_NCOUNTER +
+    sizeof(mdx_material_dev) * MDX_MAX_MATERIALS + 256 + sizeof(mdx_domain_dev) + sizeof(mdx_source_dev) +
+    sizeof(mdx_tallies_dev) + 1024;
+
+// Copies a descriptor (a multiple of 4 bytes) from global memory to LDS with
+// the whole block; the caller synchronises the block before reading it.
+__device__ __forceinline__ void block_copy(void* dst, const void* src, int bytes) {
+  unsigned char* d = (unsigned char*)dst;
+  const unsigned char* s = (const unsigned char*)src;
+  for (int i = threadIdx.x; i < bytes / 4; i += blockDim.x) __builtin_memcpy(d + 4 * i, s + 4 * i, 4);
+}
+
 template <bool MULTI>
 __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __restrict__ T,
                                                      const mdx_job_dev* __restrict__ jobs,
@@ -633,6 +651,16 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
   __shared__ double acc[MULTI ? 1 : MDX_MAX_TALLIES * kAccPerTally];
   __shared__ double lhist[MULTI ? 1 : kLdsHist];
   __shared__ unsigned long long cnt[MDX_NCOUNTER];
+  // Per-block copies of what every step and interaction reads: the material
+  // descriptors and the ID -> slot map (all launches), and for a one-job
+  // launch the job's domain, source and tally descriptors.  Divergent reads
+  // of these then cost an LDS access instead of a vector-memory round trip,
+  // and uniform ones are not re-read from memory after every atomic.
+  __shared__ mdx_material_dev lmat[MDX_MAX_MATERIALS];
+  __shared__ uint8_t lslot[256];
+  __shared__ mdx_domain_dev lD;
+  __shared__ mdx_source_dev lS;
+  __shared__ mdx_tallies_dev lTL;
   extern __shared__ float ldose_mem[];  // MDX_DOSE_F32C_LDS: (sum, error) per voxel of grid 0
   int hoff[MDX_MAX_TALLIES];
   float* ldose = nullptr;
@@ -640,7 +668,12 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
     ldose = ldose_mem;
     for (uint32_t i = threadIdx.x; i < 2 * lds_voxels; i += blockDim.x) ldose[i] = 0.0f;
   }
+  block_copy(lmat, T->mat, T->n_mat * (int)sizeof(mdx_material_dev));
+  block_copy(lslot, T->id_to_slot, 256);
   if (!MULTI) {
+    block_copy(&lD, jobs[0].D, (int)sizeof(mdx_domain_dev));
+    block_copy(&lS, jobs[0].S, (int)sizeof(mdx_source_dev));
+    block_copy(&lTL, jobs[0].TL, (int)sizeof(mdx_tallies_dev));
     hist_layout(jobs[0].TL, hoff);
     const int ntl = jobs[0].TL->n;
     for (int i = threadIdx.x; i < ntl * kAccPerTally; i += blockDim.x) acc[i] = 0.0;
@@ -655,9 +688,9 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
   // lane state: alive = carries a photon; pend = its last step ended in a
   // real interaction that waits for the wave's interaction phase
   bool alive = false, pend = false, exhausted = false;
-  const mdx_domain_dev* D = jobs[0].D;
-  const mdx_source_dev* S = jobs[0].S;
-  const mdx_tallies_dev* TL = jobs[0].TL;
+  const mdx_domain_dev* D = MULTI ? jobs[0].D : &lD;
+  const mdx_source_dev* S = MULTI ? jobs[0].S : &lS;
+  const mdx_tallies_dev* TL = MULTI ? jobs[0].TL : &lTL;
   Philox rng;     // source and interaction sampling
   StepRng srng;   // Woodcock steps
   float px = 0, py = 0, pz = 0, dx = 0, dy = 0, dz = 1, E = 1;
@@ -674,6 +707,16 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
   // free flights: the job allows them / the photon sits in the background
   bool flights = false, fly = false;
   for (;;) {
+    if (!MULTI) {
+      // The descriptors are re-read from LDS in every iteration: an offset
+      // the compiler cannot see through keeps it from holding all of them in
+      // registers across the loop, which would spill.
+      int o = 0;
+      asm volatile("" : "+s"(o));
+      D = &lD + o;
+      S = &lS + o;
+      TL = &lTL + o;
+    }
     // ---- refill: claim histories for free lanes, in batches
     const bool free_lane = !alive && !exhausted;
     const unsigned long long fmask = __ballot(free_lane);
@@ -743,11 +786,11 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
         escaped = !in_domain(D, px, py, pz);
         if (!escaped) {
           g = voxel_lookup(D, px, py, pz, lin);
-          slot = g < 0 ? D->bg_slot : (int)T->id_to_slot[gld(D->grids[g].ids + lin)];
+          slot = g < 0 ? D->bg_slot : (int)lslot[gld(D->grids[g].ids + lin)];
         }
       }
       if (!escaped && slot != cslot) {
-        stot = curve_eval(T, T->mat[slot].total, E);
+        stot = curve_eval(T, lmat[slot].total, E);
         thr = 1.0f - stot / smax;
         cslot = slot;
       }
@@ -812,7 +855,7 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
       if (pend) {
         pend = false;
         ++c_real;
-        const mdx_material_dev& M = T->mat[cslot];
+        const mdx_material_dev& M = lmat[cslot];
         Step r{qx, qy, qz, dx, dy, dz, qs, E, nc, ni, 0, 0, 0.0f, true};
         float sc, si;
         curve_eval2(T, M.coh, E, M.incoh, E, sc, si);
@@ -1180,7 +1223,7 @@ hipError_t transport_queue(const mdx_tables_dev* T, const mdx_job_dev* jobs, con
   interact_min = interact_min < 1 ? 1 : (interact_min > 64 ? 64 : interact_min);
   if (n_jobs == 1) {
     const size_t lds = (size_t)lds_voxels * 2 * sizeof(float);
-    if (lds > 65536) {
+    if (lds > 0) {
       hipError_t e = hipFuncSetAttribute((const void*)k_transport_queue<false>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       if (e != hipSuccess) return e;
@@ -1192,6 +1235,14 @@ hipError_t transport_queue(const mdx_tables_dev* T, const mdx_job_dev* jobs, con
                        work, counters, refill_min, interact_min, 0u);
   }
   return hipGetLastError();
+}
+
+size_t transport_queue_static_lds(bool on_device) {
+  if (on_device) {
+    hipFuncAttributes a;
+    if (hipFuncGetAttributes(&a, (const void*)k_transport_queue<false>) == hipSuccess) return a.sharedSizeBytes;
+  }
+  return kQueueStaticLdsBound;
 }
 
 hipError_t transport_queue_round(const mdx_tables_dev* T, const char* round, unsigned long long* work,
