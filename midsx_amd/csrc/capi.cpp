@@ -247,7 +247,7 @@ struct mdx_ctx {
   int queue_blocks_per_cu = 8;
   int refill_min = 8;
   int interact_min = 16;
-  DBuf d_work;  // queue kernel: work counter
+  DBuf d_work;  // queue kernel: histories claimed, one counter per job of a launch
   DBuf d_jobs, d_prefix;
   DBuf d_dose32;  // staging of mdx_get_voxel_dose_f32 (on the I/O stream)
 
@@ -668,7 +668,7 @@ void build_graph(mdx_ctx* c) {
   hip_check(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "begin capture");
   hip_check(hipMemcpyAsync(d, c->h_round, kRoundBytes, hipMemcpyHostToDevice, c->stream), "capture copy");
   hip_check(mdx_launch::zero_u64(c->d_counters.as<unsigned long long>(), MDX_NCOUNTER, c->stream), "capture zero");
-  hip_check(mdx_launch::zero_u64(c->d_work.as<unsigned long long>(), 1, c->stream), "capture zero");
+  hip_check(mdx_launch::zero_u64(c->d_work.as<unsigned long long>(), kRoundMax, c->stream), "capture zero");
   hip_check(mdx_launch::transport_queue_round(c->d_tables.as<mdx_tables_dev>(), d, c->d_work.as<unsigned long long>(),
                                               c->d_counters.as<unsigned long long>(),
                                               c->num_cu * c->queue_blocks_per_cu, c->refill_min, c->interact_min,
@@ -726,7 +726,8 @@ void run_jobs(mdx_ctx* c, int n_jobs, const int* ids, const uint64_t* first, con
   c->d_jobs.upload(jd.data(), jd.size(), c->stream);
   c->d_prefix.upload(prefix.data(), prefix.size(), c->stream);
   hip_check(mdx_launch::zero_u64(cnt, MDX_NCOUNTER, c->stream), "zero counters");
-  hip_check(mdx_launch::zero_u64(c->d_work.as<unsigned long long>(), 1, c->stream), "zero work");
+  if (c->d_work.bytes < (size_t)n_jobs * sizeof(unsigned long long)) c->d_work.alloc(n_jobs * sizeof(unsigned long long));
+  hip_check(mdx_launch::zero_u64(c->d_work.as<unsigned long long>(), std::max(n_jobs, 1), c->stream), "zero work");
   hip_check(hipEventRecord(c->ev0, c->stream), "hipEventRecord");
   if (total > 0)
     hip_check(mdx_launch::transport_queue(c->d_tables.as<mdx_tables_dev>(), c->d_jobs.as<mdx_job_dev>(),
@@ -776,7 +777,7 @@ int mdx_create(int device, mdx_ctx** out) {
     c->pool.limit = prop.totalGlobalMem / 4;
     c->lds_optin = std::max(prop.sharedMemPerBlockOptin, prop.sharedMemPerBlock);
     c->d_counters.alloc(MDX_NCOUNTER * sizeof(unsigned long long));
-    c->d_work.alloc(sizeof(unsigned long long));
+    c->d_work.alloc(kRoundMax * sizeof(unsigned long long));
     *out = c.release();
   });
 }

@@ -9,10 +9,11 @@ namespace mdx_launch {
 hipError_t transport_plain(const mdx_tables_dev* T, const mdx_domain_dev* D, const mdx_source_dev* S,
                            const mdx_tallies_dev* TL, uint64_t first, uint64_t n, uint64_t seed, uint32_t job,
                            unsigned long long* counters, int grid_blocks, hipStream_t st);
-// Queue kernel over the concatenated history slices of n_jobs jobs; *work must
-// be zero at launch.  A wave claims new histories once refill_min of its lanes
-// are free and runs its parked real interactions once interact_min lanes have
-// one (or when nothing else can proceed).
+// Queue kernel over the history slices of n_jobs jobs; work[0 .. n_jobs)
+// (histories claimed per job) must be zero at launch.  A wave claims new
+// histories once refill_min of its lanes are free and runs its parked real
+// interactions once interact_min lanes have one (or when nothing else can
+// proceed).
 hipError_t transport_queue(const mdx_tables_dev* T, const mdx_job_dev* jobs, const uint64_t* prefix, int n_jobs,
                            uint64_t total, unsigned long long* work, unsigned long long* counters, int grid_blocks,
                            int refill_min, int interact_min, uint32_t lds_voxels, hipStream_t st);

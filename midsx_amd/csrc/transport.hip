@@ -641,6 +641,39 @@ __device__ __forceinline__ void block_copy(void* dst, const void* src, int bytes
   for (int i = threadIdx.x; i < bytes / 4; i += blockDim.x) __builtin_memcpy(d + 4 * i, s + 4 * i, 4);
 }
 
+// The same with one (converged) wave; its lanes read the copy afterwards.
+__device__ __forceinline__ void wave_copy(void* dst, const void* src, int bytes) {
+  unsigned char* d = (unsigned char*)dst;
+  const unsigned char* s = (const unsigned char*)src;
+  for (int i = threadIdx.x & 63; i < bytes / 4; i += 64) __builtin_memcpy(d + 4 * i, s + 4 * i, 4);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// A job's scene descriptors as the kernel keeps them in LDS.
+struct JobDesc {
+  mdx_domain_dev D;
+  mdx_source_dev S;
+  mdx_tallies_dev TL;
+};
+
+__device__ __forceinline__ void load_job(JobDesc* dst, const mdx_job_dev& j) {
+  wave_copy(&dst->D, j.D, (int)sizeof(mdx_domain_dev));
+  wave_copy(&dst->S, j.S, (int)sizeof(mdx_source_dev));
+  wave_copy(&dst->TL, j.TL, (int)sizeof(mdx_tallies_dev));
+}
+
+constexpr int kWaves = kBlock / 64;
+
+// Work distribution.  One job: lanes claim histories from one counter.
+// Several jobs: each wave works on one job at a time, claiming from that
+// job's counter (work[j]), so that the scene a lane reads is the same for the
+// whole wave; the wave starts on the job its index falls in when waves are
+// spread over the jobs in proportion to their histories, and when its job
+// runs out it moves to the next job with histories left.  Its previous job's
+// photons finish meanwhile: a wave keeps two job slots, and waits for one to
+// empty before it takes a third job.
 template <bool MULTI>
 __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __restrict__ T,
                                                      const mdx_job_dev* __restrict__ jobs,
@@ -652,15 +685,14 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
   __shared__ double lhist[MULTI ? 1 : kLdsHist];
   __shared__ unsigned long long cnt[MDX_NCOUNTER];
   // Per-block copies of what every step and interaction reads: the material
-  // descriptors and the ID -> slot map (all launches), and for a one-job
-  // launch the job's domain, source and tally descriptors.  Divergent reads
-  // of these then cost an LDS access instead of a vector-memory round trip,
-  // and uniform ones are not re-read from memory after every atomic.
+  // descriptors and the ID -> slot map, and the scene descriptors of the
+  // job(s) being run (one for the block in a one-job launch, two per wave
+  // otherwise).  Divergent reads of these then cost an LDS access instead of a
+  // vector-memory round trip, and uniform ones are not re-read from memory
+  // after every atomic.
   __shared__ mdx_material_dev lmat[MDX_MAX_MATERIALS];
   __shared__ uint8_t lslot[256];
-  __shared__ mdx_domain_dev lD;
-  __shared__ mdx_source_dev lS;
-  __shared__ mdx_tallies_dev lTL;
+  __shared__ JobDesc desc[MULTI ? 2 * kWaves : 1];
   extern __shared__ float ldose_mem[];  // MDX_DOSE_F32C_LDS: (sum, error) per voxel of grid 0
   int hoff[MDX_MAX_TALLIES];
   float* ldose = nullptr;
@@ -671,9 +703,9 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
   block_copy(lmat, T->mat, T->n_mat * (int)sizeof(mdx_material_dev));
   block_copy(lslot, T->id_to_slot, 256);
   if (!MULTI) {
-    block_copy(&lD, jobs[0].D, (int)sizeof(mdx_domain_dev));
-    block_copy(&lS, jobs[0].S, (int)sizeof(mdx_source_dev));
-    block_copy(&lTL, jobs[0].TL, (int)sizeof(mdx_tallies_dev));
+    block_copy(&desc[0].D, jobs[0].D, (int)sizeof(mdx_domain_dev));
+    block_copy(&desc[0].S, jobs[0].S, (int)sizeof(mdx_source_dev));
+    block_copy(&desc[0].TL, jobs[0].TL, (int)sizeof(mdx_tallies_dev));
     hist_layout(jobs[0].TL, hoff);
     const int ntl = jobs[0].TL->n;
     for (int i = threadIdx.x; i < ntl * kAccPerTally; i += blockDim.x) acc[i] = 0.0;
@@ -683,14 +715,33 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
   __syncthreads();
 
   const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
   uint32_t c_hist = 0, c_step = 0, c_real = 0, c_ray = 0, c_comp = 0, c_photo = 0, c_trial = 0, c_stest = 0,
            c_dep = 0, c_surf = 0, c_esc = 0;
   // lane state: alive = carries a photon; pend = its last step ended in a
   // real interaction that waits for the wave's interaction phase
   bool alive = false, pend = false, exhausted = false;
-  const mdx_domain_dev* D = MULTI ? jobs[0].D : &lD;
-  const mdx_source_dev* S = MULTI ? jobs[0].S : &lS;
-  const mdx_tallies_dev* TL = MULTI ? jobs[0].TL : &lTL;
+  // several jobs (uniform per wave): the current job, the last one that ran
+  // out, the current job's slot, and whether every job has run out; per lane:
+  // the slot of the photon's job
+  int jc = -1, jc_last = -1, cs = 0;
+  bool done_all = false;
+  int ls = 0;
+  if (MULTI) {
+    const uint64_t nw = (uint64_t)gridDim.x * kWaves;
+    const uint64_t gw = (uint64_t)blockIdx.x * kWaves + (uint64_t)wave;
+    const uint64_t h = (uint64_t)(((double)gw + 0.5) / (double)nw * (double)total);
+    int lo = 0, hi = n_jobs - 1;  // last job with prefix[j] <= h
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (prefix[mid] <= h) lo = mid; else hi = mid - 1;
+    }
+    jc = lo;
+    load_job(&desc[2 * wave], jobs[jc]);
+  }
+  const mdx_domain_dev* D = &desc[0].D;
+  const mdx_source_dev* S = &desc[0].S;
+  const mdx_tallies_dev* TL = &desc[0].TL;
   Philox rng;     // source and interaction sampling
   StepRng srng;   // Woodcock steps
   float px = 0, py = 0, pz = 0, dx = 0, dy = 0, dz = 1, E = 1;
@@ -707,61 +758,99 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
   // free flights: the job allows them / the photon sits in the background
   bool flights = false, fly = false;
   for (;;) {
-    if (!MULTI) {
+    {
       // The descriptors are re-read from LDS in every iteration: an offset
-      // the compiler cannot see through keeps it from holding all of them in
+      // the compiler cannot see through keeps it from holding them in
       // registers across the loop, which would spill.
       int o = 0;
       asm volatile("" : "+s"(o));
-      D = &lD + o;
-      S = &lS + o;
-      TL = &lTL + o;
+      const JobDesc* jd = MULTI ? &desc[2 * wave + ls] + o : &desc[0] + o;
+      D = &jd->D;
+      S = &jd->S;
+      TL = &jd->TL;
     }
     // ---- refill: claim histories for free lanes, in batches
     const bool free_lane = !alive && !exhausted;
     const unsigned long long fmask = __ballot(free_lane);
     const bool steppable = __ballot(alive && !pend) != 0ull;
-    if (fmask && (__popcll(fmask) >= refill_min || !steppable)) {
+    if (fmask && (__popcll(fmask) >= refill_min || !steppable) && !done_all) {
       const int leader = __ffsll((long long)fmask) - 1;
-      unsigned long long base = 0;
-      if (lane == leader) base = atomicAdd(work, (unsigned long long)__popcll(fmask));
-      base = __shfl(base, leader);
-      if (free_lane) {
-        const uint64_t g = base + __popcll(fmask & ((1ull << lane) - 1ull));
-        if (g < total) {
-          uint64_t local = g;
-          int j = 0;
-          if (MULTI) {
-            int lo = 0, hi = n_jobs - 1;  // last job with prefix[j] <= g
-            while (lo < hi) {
-              int mid = (lo + hi + 1) >> 1;
-              if (prefix[mid] <= g) lo = mid; else hi = mid - 1;
-            }
-            j = lo;
-            local = g - prefix[j];
-            D = jobs[j].D;
-            S = jobs[j].S;
-            TL = jobs[j].TL;
+      const uint32_t rank = __popcll(fmask & ((1ull << lane) - 1ull));
+      bool fresh = false;  // this lane claimed a history just now
+      if (!MULTI) {
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(work, (unsigned long long)__popcll(fmask));
+        base = __shfl(base, leader);
+        if (free_lane) {
+          if (base + rank < total) {
+            const uint64_t h = jobs[0].first + base + rank;
+            rng.init(jobs[0].seed, jobs[0].stream, h);
+            srng.init(jobs[0].seed, jobs[0].stream, h);
+            flights = jobs[0].flights != 0;
+            fresh = true;
+          } else {
+            exhausted = true;
           }
-          rng.init(jobs[j].seed, jobs[j].stream, jobs[j].first + local);
-          srng.init(jobs[j].seed, jobs[j].stream, jobs[j].first + local);
-          generate(S, rng, px, py, pz, dx, dy, dz, E);
-          ++c_hist;
-          nc = ni = 0;
-          smax = curve_eval(T, T->majorant, E);
-          step_scale = 0.69314718055994531f / smax;
-          cslot = -1;
-          nstep = 0;
-          tfl = next_tally_event(TL, px, py, pz, dx, dy, dz);
-          flights = jobs[j].flights != 0;
-          fly = false;
-          alive = true;
-        } else {
-          exhausted = true;
+        }
+      } else {
+        if (jc < 0) {  // the current job ran out: the next one with histories left, in a free slot
+          const bool used0 = __ballot(alive && ls == 0) != 0ull;
+          const bool used1 = __ballot(alive && ls == 1) != 0ull;
+          const int sl = !used0 ? 0 : (!used1 ? 1 : -1);
+          if (sl >= 0) {  // else wait for a slot to empty
+            int jn = -1;
+            for (int k = 0; k < n_jobs; ++k) {
+              const int j = (jc_last + 1 + k) % n_jobs;
+              if (__hip_atomic_load(work + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < jobs[j].n) {
+                jn = j;
+                break;
+              }
+            }
+            if (jn < 0) {
+              done_all = true;
+            } else {
+              load_job(&desc[2 * wave + sl], jobs[jn]);
+              jc = jn;
+              cs = sl;
+            }
+          }
+        }
+        if (jc >= 0) {
+          const uint64_t nj = jobs[jc].n;
+          unsigned long long base = 0;
+          if (lane == leader) base = atomicAdd(work + jc, (unsigned long long)__popcll(fmask));
+          base = __shfl(base, leader);
+          if (base >= nj) {
+            jc_last = jc;
+            jc = -1;
+          } else if (free_lane && base + rank < nj) {
+            const uint64_t h = jobs[jc].first + base + rank;
+            rng.init(jobs[jc].seed, jobs[jc].stream, h);
+            srng.init(jobs[jc].seed, jobs[jc].stream, h);
+            flights = jobs[jc].flights != 0;
+            ls = cs;
+            const JobDesc* jd = &desc[2 * wave + ls];
+            D = &jd->D;
+            S = &jd->S;
+            TL = &jd->TL;
+            fresh = true;
+          }
         }
       }
+      if (fresh) {
+        generate(S, rng, px, py, pz, dx, dy, dz, E);
+        ++c_hist;
+        nc = ni = 0;
+        smax = curve_eval(T, T->majorant, E);
+        step_scale = 0.69314718055994531f / smax;
+        cslot = -1;
+        nstep = 0;
+        tfl = next_tally_event(TL, px, py, pz, dx, dy, dz);
+        fly = false;
+        alive = true;
+      }
     }
-    if (__ballot(alive) == 0ull) break;
+    if (__ballot(alive) == 0ull && (!MULTI || done_all)) break;
 
     // ---- one Woodcock step for every lane that can step (physics_engine.cpp:38-66)
     // Written so that the costly parts (a cross-section evaluation, a tally
