@@ -319,7 +319,7 @@ class Context:
         out["kernel_ms"] = st.kernel_ms
         return out
 
-    def set_queue_params(self, blocks_per_cu=8, refill_min=8, interact_min=8):
+    def set_queue_params(self, blocks_per_cu=4, refill_min=32, interact_min=32):
         self._call(self.lib.mdx_set_queue_params, self._h, ctypes.c_int(blocks_per_cu), ctypes.c_int(refill_min),
                    ctypes.c_int(interact_min))
 

@@ -226,7 +226,7 @@ struct mdx_ctx {
   std::vector<std::unique_ptr<mdx::MaterialTables>> mats;
   mdx::Table majorant;
   mdx::Packed packed;
-  DBuf d_xs, d_nodes, d_hash, d_rE, d_rinv, d_roff, d_rrows, d_tables;
+  DBuf d_xs, d_nodes, d_hash, d_rE, d_rinv, d_roff, d_rrows, d_rhash, d_rguide, d_tables;
   bool tables_ready = false;
   bool rita_on_device = true;  // RITA tables built by k_rita_build (FP64) instead of on the host
 
@@ -244,9 +244,9 @@ struct mdx_ctx {
 
   DBuf d_counters;
   // queue-kernel launch parameters (mdx_set_queue_params)
-  int queue_blocks_per_cu = 8;
-  int refill_min = 8;
-  int interact_min = 16;
+  int queue_blocks_per_cu = 4;
+  int refill_min = 32;
+  int interact_min = 32;
   DBuf d_work;  // queue kernel: histories claimed, one counter per job of a launch
   DBuf d_jobs, d_prefix;
   DBuf d_dose32;  // staging of mdx_get_voxel_dose_f32 (on the I/O stream)
@@ -503,6 +503,8 @@ void build_and_upload(mdx_ctx* c, const std::vector<int>& ids) {
   c->d_rinv.upload(P.rita_inv_dlnE.data(), P.rita_inv_dlnE.size(), c->stream);
   c->d_roff.upload(P.rita_row_off.data(), P.rita_row_off.size(), c->stream);
   c->d_rrows.upload(P.rita_rows.data(), P.rita_rows.size(), c->stream);
+  c->d_rhash.upload(P.rita_ehash.data(), P.rita_ehash.size(), c->stream);
+  c->d_rguide.upload(P.rita_guide.data(), P.rita_guide.size(), c->stream);
   mdx_tables_dev T{};
   T.xs = c->d_xs.as<float>();
   T.nodes = c->d_nodes.as<mdx_node>();
@@ -511,6 +513,8 @@ void build_and_upload(mdx_ctx* c, const std::vector<int>& ids) {
   T.rita_inv_dlnE = c->d_rinv.as<float>();
   T.rita_row_off = c->d_roff.as<int32_t>();
   T.rita_rows = c->d_rrows.as<mdx_f4>();
+  T.rita_ehash = c->d_rhash.as<int32_t>();
+  T.rita_guide = c->d_rguide.as<uint8_t>();
   T.majorant = P.majorant;
   T.n_mat = (int)P.mats.size();
   for (size_t s = 0; s < P.mats.size(); ++s) T.mat[s] = P.mats[s];
@@ -1242,16 +1246,22 @@ int mdx_autotune(mdx_ctx* c, uint64_t probe_histories, int* blocks_per_cu, int* 
     };
     int b = c->queue_blocks_per_cu, r = c->refill_min, i = c->interact_min;
     double cur = probe(b, r, i);
-    // coordinate search: interaction batch, refill batch, then occupancy
-    for (int v : {8, 12, 16, 24, 32}) {
-      double t = probe(b, r, v);
-      if (t < cur) { cur = t; i = v; }
+    // coordinate search: interaction batch, refill batch, both again, then
+    // occupancy (the kernel's registers allow four waves per SIMD)
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int v : {8, 16, 24, 32, 40, 48}) {
+        if (v == i) continue;
+        double t = probe(b, r, v);
+        if (t < cur) { cur = t; i = v; }
+      }
+      for (int v : {8, 16, 24, 32, 40, 48}) {
+        if (v == r) continue;
+        double t = probe(b, v, i);
+        if (t < cur) { cur = t; r = v; }
+      }
     }
-    for (int v : {2, 4, 8, 16}) {
-      double t = probe(b, v, i);
-      if (t < cur) { cur = t; r = v; }
-    }
-    for (int v : {4, 8, 12, 16}) {
+    for (int v : {3, 4, 8}) {
+      if (v == b) continue;
       double t = probe(v, r, i);
       if (t < cur) { cur = t; b = v; }
     }

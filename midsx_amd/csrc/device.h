@@ -308,43 +308,69 @@ __device__ __forceinline__ void rotate(float& dx, float& dy, float& dz, float mu
 }
 
 // ------------------------------------------------------------------ RITA
-// probability_dist.cpp:90-119 on one energy row, in w = 1 + x.
-__device__ __forceinline__ float rita_row(const mdx_tables_dev* T, int row0, int row1, float y) {
-  const mdx_f4* R = T->rita_rows;
-  int m = row1 - row0;
-  // lower_bound over the row's CDF values, minus one
-  int lo = 0, hi = m;
-  while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    if (gld(&R[row0 + mid].y) < y) lo = mid + 1; else hi = mid;
+// probability_dist.cpp:90-119 on one energy row, in w = 1 + x: the largest
+// node i with y_i < y (lower_bound - 1), then the rational interpolation on
+// [y_i, y_{i+1}].  The row's guide entry for y gives a start index with
+// y_i < y; the node pair (i, i + 1) brackets y unless the guide bin holds
+// further nodes, and a short walk follows then.
+struct RitaAt {
+  int i;
+  mdx_f4 a, b;  // nodes i and i + 1 (clamped to the row)
+};
+
+__device__ __forceinline__ int rita_guide(const mdx_tables_dev* T, int grow, float y) {
+  int j = (int)(y * (float)MDX_RITA_GUIDE);  // exact scaling: the guide's bins are [j / G, (j + 1) / G)
+  j = j < 0 ? 0 : (j > MDX_RITA_GUIDE - 1 ? MDX_RITA_GUIDE - 1 : j);
+  return (int)gld(T->rita_guide + (size_t)grow * MDX_RITA_GUIDE + j) - 1;
+}
+
+__device__ __forceinline__ RitaAt rita_nodes(const mdx_tables_dev* T, int row0, int m, int i) {
+  const mdx_f4* R = T->rita_rows + row0;
+  const int ia = i < 0 ? 0 : i, ib = i + 1 > m - 1 ? m - 1 : i + 1;
+  return RitaAt{i, gld(R + ia), gld(R + ib)};
+}
+
+__device__ __forceinline__ float rita_finish(const mdx_tables_dev* T, int row0, int m, float y, RitaAt r) {
+  if (r.i + 1 < m && r.b.y < y) {  // the guide bin holds more nodes below y
+    const mdx_f4* R = T->rita_rows + row0;
+    int i = r.i + 1;
+    while (i + 1 < m && gld(&R[i + 1].y) < y) ++i;
+    r = rita_nodes(T, row0, m, i);
   }
-  int i = lo - 1;
-  if (i == m - 1) return 2.0f;
-  if (i < 0) return 0.0f;
-  const mdx_f4 a = gld(R + row0 + i), b = gld(R + row0 + i + 1);
-  float nu = y - a.y;
-  float d = b.y - a.y;
+  if (r.i == m - 1) return 2.0f;
+  if (r.i < 0) return 0.0f;
+  const mdx_f4& a = r.a;
+  const mdx_f4& b = r.b;
+  const float nu = y - a.y;
+  const float d = b.y - a.y;
   return a.x + (1.0f + a.z + a.w) * d * nu / (d * d + a.z * d * nu + a.w * nu * nu) * (b.x - a.x);
 }
 
 // probability_dist.cpp:73-88: energy bracket by lower_bound - 1 on the
-// incoherent-CS grid, x at both energies from the same y, then log-log
-// interpolation of (x + 1) in E.
+// incoherent-CS grid (the largest i with E_i < E, clamped to the last
+// interval; the energy hash gives the start), x at both energies from the
+// same y, then log-log interpolation of (x + 1) in E.
 __device__ __forceinline__ float rita_sample(const mdx_tables_dev* T, const mdx_material_dev& M, float E, float y) {
   const float* Eg = T->rita_E + M.rita_eoff;
-  int lo = 0, hi = M.rita_ne;
-  while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    if (gld(Eg + mid) < E) lo = mid + 1; else hi = mid;
+  const int ne = M.rita_ne;
+  int b = (int)((logf(E) - M.rita_lnE0) * M.rita_hinv);
+  b = b < 0 ? 0 : (b >= M.rita_nh ? M.rita_nh - 1 : b);
+  int i = gld(T->rita_ehash + M.rita_hoff + b);
+  float e0 = i >= 0 ? gld(Eg + i) : 0.0f;
+  float e1 = i + 1 < ne ? gld(Eg + i + 1) : INFINITY;
+  if (!(e0 < E || i < 0) || !(e1 >= E)) {  // the bin holds more energies (or E sits on its edge)
+    while (i >= 0 && gld(Eg + i) >= E) --i;
+    while (i + 1 < ne && gld(Eg + i + 1) < E) ++i;
   }
-  int il = lo - 1;
-  il = il < 0 ? 0 : (il > M.rita_ne - 2 ? M.rita_ne - 2 : il);
+  const int il = i < 0 ? 0 : (i > ne - 2 ? ne - 2 : i);
   const int* ro = T->rita_row_off + M.rita_roff;
   const int r0 = gld(ro + il), r1 = gld(ro + il + 1), r2 = gld(ro + il + 2);
-  float wl = rita_row(T, r0, r1, y);
-  float wu = rita_row(T, r1, r2, y);
-  float f = log_ratio(E, gld(Eg + il)) * gld(T->rita_inv_dlnE + M.rita_eoff + il);
-  float w = (wl > 0.0f && wu > 0.0f) ? wl * expf(f * log_ratio(wu, wl)) : wl;
+  const int gl = M.rita_eoff + il;
+  const int ml = r1 - r0, mu = r2 - r1;
+  const float wl = rita_finish(T, r0, ml, y, rita_nodes(T, r0, ml, rita_guide(T, gl, y)));
+  const float wu = rita_finish(T, r1, mu, y, rita_nodes(T, r1, mu, rita_guide(T, gl + 1, y)));
+  const float f = log_ratio(E, gld(Eg + il)) * gld(T->rita_inv_dlnE + M.rita_eoff + il);
+  const float w = (wl > 0.0f && wu > 0.0f) ? wl * expf(f * log_ratio(wu, wl)) : wl;
   return w - 1.0f;
 }
 

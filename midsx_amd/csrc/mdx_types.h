@@ -43,12 +43,21 @@ struct mdx_curve {
   float pad;
 };
 
+// RITA lookups: an energy hash over ln E gives the largest grid index below
+// each bin's start, and every row has MDX_RITA_GUIDE guide entries, entry j
+// holding the number of the row's CDF values below j / MDX_RITA_GUIDE.
+#define MDX_RITA_GUIDE 64
+
 struct mdx_material_dev {
   mdx_curve total, coh, incoh, photo, sf;
   int32_t rita_ne;     // energies in the RITA grid
-  int32_t rita_eoff;   // offset into rita_E / rita_inv_dlnE
+  int32_t rita_eoff;   // offset into rita_E / rita_inv_dlnE (and row index for rita_guide)
   int32_t rita_roff;   // offset into rita_row_off (rita_ne + 1 entries)
   int32_t material_id;
+  int32_t rita_hoff;   // first entry of the energy hash in rita_ehash
+  int32_t rita_nh;     // its bins
+  float rita_lnE0;     // ln of the hash range start
+  float rita_hinv;     // hash bins per unit ln E
 };
 
 struct alignas(16) mdx_f4 {
@@ -65,6 +74,8 @@ struct mdx_tables_dev {
   const float* rita_inv_dlnE;   // 1 / ln(E_{i+1}/E_i)
   const int32_t* rita_row_off;  // per energy: first row; one extra entry per material
   const mdx_f4* rita_rows;
+  const int32_t* rita_ehash;    // energy hash (all materials)
+  const uint8_t* rita_guide;    // MDX_RITA_GUIDE entries per energy row (all materials)
   mdx_curve majorant;
   int32_t n_mat;
   int32_t pad[3];

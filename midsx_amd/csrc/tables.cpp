@@ -642,6 +642,8 @@ void pack_all(Packed& p, const std::vector<const MaterialTables*>& mats, const T
                                     ? (float)(1.0 / std::log(R.energies[i + 1] / R.energies[i]))
                                     : 0.0f);
       p.rita_row_off.push_back((int)p.rita_rows.size());
+      if (R.x[i].size() > 255) throw std::runtime_error("a RITA row has more than 255 points");
+      std::vector<float> ys;
       for (size_t r = 0; r < R.x[i].size(); ++r) {
         mdx_f4 q;
         q.x = (float)(1.0 + R.x[i][r]);
@@ -649,9 +651,34 @@ void pack_all(Packed& p, const std::vector<const MaterialTables*>& mats, const T
         q.z = r < R.a[i].size() ? (float)R.a[i][r] : 0.0f;
         q.w = r < R.b[i].size() ? (float)R.b[i][r] : 0.0f;
         p.rita_rows.push_back(q);
+        ys.push_back(q.y);
+      }
+      // guide: entry j = number of CDF values (as stored) below j / G; the
+      // device scales y by G (a power of two, so exactly) to find j
+      for (int j = 0; j < MDX_RITA_GUIDE; ++j) {
+        const float t = (float)j / (float)MDX_RITA_GUIDE;
+        int cnt = 0;
+        while (cnt < (int)ys.size() && ys[cnt] < t) ++cnt;
+        p.rita_guide.push_back((uint8_t)cnt);
       }
     }
     p.rita_row_off.push_back((int)p.rita_rows.size());
+    // energy hash over [e_lo, e_hi]: entry b = largest index whose (stored)
+    // energy lies below the bin's start, -1 if none
+    {
+      const int nh = 512;
+      const double l0 = std::log(e_lo), l1 = std::log(e_hi);
+      d.rita_hoff = (int)p.rita_ehash.size();
+      d.rita_nh = nh;
+      d.rita_lnE0 = (float)l0;
+      d.rita_hinv = (float)(nh / (l1 - l0));
+      int k = -1;
+      for (int b = 0; b < nh; ++b) {
+        const double eb = std::exp(l0 + b / (double)d.rita_hinv);
+        while (k + 1 < d.rita_ne && (double)p.rita_E[d.rita_eoff + k + 1] < eb) ++k;
+        p.rita_ehash.push_back(k);
+      }
+    }
     p.mats.push_back(d);
     p.id_to_slot[m.def.id & 255] = (uint8_t)s;
   }

@@ -93,6 +93,8 @@ struct Packed {
   std::vector<float> rita_E, rita_inv_dlnE;
   std::vector<int32_t> rita_row_off;
   std::vector<mdx_f4> rita_rows;
+  std::vector<int32_t> rita_ehash;
+  std::vector<uint8_t> rita_guide;
   mdx_curve majorant{};
   std::vector<mdx_material_dev> mats;
   uint8_t id_to_slot[256];
