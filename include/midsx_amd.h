@@ -100,7 +100,7 @@ typedef struct {
   double c[3];        /* rect: edge2 */
   double radius;      /* disc radius (cm) */
   double energy;      /* monoenergetic energy (eV) */
-  int n_bins;         /* polyenergetic: number of (energy, weight) bins */
+  int n_bins;         /* polyenergetic: number of (energy, weight) bins, at most 65535 */
   const double* bin_energy; /* eV */
   const double* bin_weight; /* normalised internally (probability_dist.cpp:34-39) */
 } mdx_source;

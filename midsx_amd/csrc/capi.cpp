@@ -1073,6 +1073,7 @@ int mdx_set_source(mdx_ctx* c, const mdx_source* s) {
       d.energy = (float)s->energy;
     } else if (s->spectrum == MDX_SPEC_POLY) {
       need(s->n_bins > 0 && s->bin_energy && s->bin_weight, "polyenergetic spectrum needs bins");
+      need(s->n_bins <= 65535, "polyenergetic spectrum has more than 65535 bins");
       // probability_dist.cpp:11-60: normalise, cumulative sum
       double sum = 0;
       for (int i = 0; i < s->n_bins; ++i) sum += s->bin_weight[i];
@@ -1089,6 +1090,12 @@ int mdx_set_source(mdx_ctx* c, const mdx_source* s) {
         c->J().d_energies.upload(en.data(), en.size(), c->js);
       }
       d.n_bins = s->n_bins;
+      for (int j = 0; j < MDX_SPEC_GUIDE; ++j) {  // the device scales u by a power of two, exactly
+        const float t = (float)j / (float)MDX_SPEC_GUIDE;
+        int k = 0;
+        while (k < s->n_bins && cdf[k] < t) ++k;
+        d.guide[j] = (uint16_t)k;
+      }
       d.cdf = c->J().d_cdf.as<float>();
       d.energies = c->J().d_energies.as<float>();
     } else {

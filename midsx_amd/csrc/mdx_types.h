@@ -116,6 +116,7 @@ struct mdx_domain_dev {
 #define MDX_DIR_ISOTROPIC 1
 #define MDX_DIR_RECT_ISOTROPIC 2
 #define MDX_DIR_DISC_ISOTROPIC 3
+#define MDX_SPEC_GUIDE 64  // guide entries of a polyenergetic source's CDF
 #define MDX_SPEC_MONO 0
 #define MDX_SPEC_POLY 1
 
@@ -132,6 +133,7 @@ struct mdx_source_dev {
   float energy;
   const float* cdf;       // poly: cumulative probabilities
   const float* energies;  // poly: bin energies (eV)
+  uint16_t guide[MDX_SPEC_GUIDE];  // poly: guide[j] = number of cdf values below j / MDX_SPEC_GUIDE
 };
 
 #define MDX_TALLY_AACUBOID 0

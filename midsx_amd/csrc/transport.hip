@@ -234,13 +234,19 @@ __device__ void generate(const mdx_source_dev* S, Philox& rng, float& px, float&
   float inv = rsqrtf(vx * vx + vy * vy + vz * vz);
   dx = vx * inv; dy = vy * inv; dz = vz * inv;
   if (S->spec_kind == MDX_SPEC_POLY) {  // probability_dist.cpp:18-28
-    float u = rng.uniform();
-    int lo = 0, hi = S->n_bins;
+    // lower_bound of u in the CDF, searched between the guide entries of
+    // u's bin [j / G, (j + 1) / G): at least guide[j] values lie below u and
+    // at most guide[j + 1] (all n for the last bin)
+    const float u = rng.uniform();
+    int j = (int)(u * (float)MDX_SPEC_GUIDE);
+    j = j > MDX_SPEC_GUIDE - 1 ? MDX_SPEC_GUIDE - 1 : j;
+    const int n = S->n_bins;
+    int lo = S->guide[j], hi = j + 1 < MDX_SPEC_GUIDE ? S->guide[j + 1] : n;
     while (lo < hi) {
-      int mid = (lo + hi) >> 1;
+      const int mid = (lo + hi) >> 1;
       if (gld(S->cdf + mid) < u) lo = mid + 1; else hi = mid;
     }
-    lo = lo >= S->n_bins ? S->n_bins - 1 : lo;
+    lo = lo >= n ? n - 1 : lo;
     E = gld(S->energies + lo);
   } else {
     E = S->energy;
