@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <set>
 #include <stdexcept>
 #include <string>
@@ -69,6 +70,8 @@ struct Pool {
   }
 };
 
+void forget_clear(void* p, size_t bytes);
+
 // Device allocation owned by the context (from its pool when one is set).
 struct DBuf {
   void* p = nullptr;
@@ -81,6 +84,7 @@ struct DBuf {
   ~DBuf() { release(); }
   void release() {
     if (p) {
+      forget_clear(p, bytes);
       if (pool) pool->put(p, bytes);
       else (void)hipFree(p);
     }
@@ -294,7 +298,35 @@ namespace {
 constexpr int kRoundMax = 1024;
 constexpr size_t kRoundHeader = 2 * sizeof(uint64_t);
 constexpr size_t kRoundPrefix = (kRoundMax + 1) * sizeof(uint64_t);
-constexpr size_t kRoundBytes = kRoundHeader + kRoundPrefix + kRoundMax * sizeof(mdx_job_dev);
+constexpr int kRoundClearMax = 1024;  // ranges a round's graph clears before its transport kernel
+constexpr size_t kRoundClearOff = kRoundHeader + kRoundPrefix + kRoundMax * sizeof(mdx_job_dev);
+constexpr size_t kRoundBytes = kRoundClearOff + (1 + 2 * kRoundClearMax) * sizeof(uint64_t);
+
+// Deferred clearing.  While a graph round runs, its blocks hold every CU,
+// so a kernel that clears a new job's dose or tally buffers on the set-up
+// stream would only start when the round ends, and the set-up call would wait
+// for it.  Such ranges are queued instead and cleared by the next round's
+// graph, right before its transport kernel; anything that reads or launches
+// on a queued range first clears it (flush_clears).  Ranges of a buffer that
+// is freed or reallocated leave the queue.
+struct ClearReq {
+  const void* ctx;
+  void* p;
+  size_t bytes;
+};
+std::mutex g_clear_mu;
+std::vector<ClearReq> g_clears;
+
+void forget_clear(void* p, size_t bytes) {
+  std::lock_guard<std::mutex> lk(g_clear_mu);
+  const char* a = static_cast<const char*>(p);
+  g_clears.erase(std::remove_if(g_clears.begin(), g_clears.end(),
+                                [&](const ClearReq& r) {
+                                  const char* b = static_cast<const char*>(r.p);
+                                  return b < a + bytes && a < b + r.bytes;
+                                }),
+                 g_clears.end());
+}
 
 template <class F>
 int guard(F&& f) {
@@ -357,8 +389,66 @@ void set_device_job(mdx_ctx* c) {
 }
 
 // Device-to-host copy on the I/O stream, which runs beside a round in flight.
+// Clears bytes at p on stream st now (p holds doubles when bytes is a
+// multiple of 8, 32-bit words otherwise).
+void clear_now(void* p, size_t bytes, hipStream_t st) {
+  if (bytes % 8 == 0) hip_check(mdx_launch::zero(static_cast<double*>(p), bytes / 8, st), "clear");
+  else hip_check(mdx_launch::zero_u32(static_cast<uint32_t*>(p), bytes / 4, st), "clear");
+}
+
+// Clears a device buffer for a job's set-up: on the set-up stream, or queued
+// for the next round while a graph round is in flight.
+void clear_dev(mdx_ctx* c, void* p, size_t bytes) {
+  if (!p || !bytes) return;
+  if (c->use_graphs && c->inflight && (reinterpret_cast<uintptr_t>(p) & 15) == 0 && bytes % 4 == 0) {
+    std::lock_guard<std::mutex> lk(g_clear_mu);
+    int mine = 0;
+    for (auto& r : g_clears) mine += r.ctx == c;
+    if (mine < kRoundClearMax) {
+      g_clears.push_back(ClearReq{c, p, bytes});
+      return;
+    }
+  }
+  clear_now(p, bytes, c->js);
+}
+
+// Takes this context's queued ranges off the queue.
+std::vector<ClearReq> take_clears(mdx_ctx* c) {
+  std::lock_guard<std::mutex> lk(g_clear_mu);
+  std::vector<ClearReq> mine;
+  for (auto& r : g_clears)
+    if (r.ctx == c) mine.push_back(r);
+  g_clears.erase(std::remove_if(g_clears.begin(), g_clears.end(), [&](const ClearReq& r) { return r.ctx == c; }),
+                 g_clears.end());
+  return mine;
+}
+
+// Clears this context's queued ranges on its transport stream (after a round
+// in flight) and waits for them.
+void flush_clears(mdx_ctx* c) {
+  std::vector<ClearReq> mine = take_clears(c);
+  if (mine.empty()) return;
+  for (auto& r : mine) clear_now(r.p, r.bytes, c->stream);
+  hip_check(hipStreamSynchronize(c->stream), "clear");
+}
+
+// flush_clears when [p, p + bytes) overlaps a queued range of this context.
+void flush_clears_over(mdx_ctx* c, const void* p, size_t bytes) {
+  bool hit = false;
+  {
+    std::lock_guard<std::mutex> lk(g_clear_mu);
+    const char* a = static_cast<const char*>(p);
+    for (auto& r : g_clears) {
+      const char* b = static_cast<const char*>(r.p);
+      if (r.ctx == c && b < a + bytes && a < b + r.bytes) hit = true;
+    }
+  }
+  if (hit) flush_clears(c);
+}
+
 void read_back(mdx_ctx* c, void* dst, const void* src, size_t bytes, const char* what) {
   if (!bytes) return;
+  flush_clears_over(c, src, bytes);
   hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->io_stream), what);
   hip_check(hipStreamSynchronize(c->io_stream), what);
 }
@@ -387,8 +477,8 @@ void upload_domain(mdx_ctx* c) {
         G.d_dose.alloc(nv * per);
         G.d_dose_n.alloc(nv * 4);
         G.dose_mode = mode;
-        hip_check(mdx_launch::zero(G.d_dose.as<double>(), nv * per / 8, c->js), "zero dose");
-        hip_check(mdx_launch::zero_u32(G.d_dose_n.as<uint32_t>(), nv, c->js), "zero dose count");
+        clear_dev(c, G.d_dose.p, nv * per);
+        clear_dev(c, G.d_dose_n.p, nv * 4);
       }
       d.dose_mode = mode;
       d.dose = mode == MDX_DOSE_F64 ? G.d_dose.as<double>() : nullptr;
@@ -402,7 +492,7 @@ void upload_domain(mdx_ctx* c) {
   }
   if (!c->J().d_bg.p) {
     c->J().d_bg.alloc(3 * sizeof(double));
-    hip_check(mdx_launch::zero(c->J().d_bg.as<double>(), 3, c->js), "zero bg");
+    clear_dev(c, c->J().d_bg.p, 3 * sizeof(double));
   }
   D.bg_dose = c->J().d_bg.as<double>();
   c->J().d_domain.upload(&D, 1, c->js);
@@ -563,15 +653,15 @@ void upload_tallies(mdx_ctx* c) {
 
 void zero_tallies(mdx_ctx* c) {
   for (auto& h : c->J().tallies) {
-    hip_check(mdx_launch::zero(h->acc.as<double>(), h->acc.bytes / 8, c->js), "zero acc");
-    hip_check(mdx_launch::zero(h->hist.as<double>(), h->hist.bytes / 8, c->js), "zero hist");
-    hip_check(mdx_launch::zero(h->image.as<double>(), h->image.bytes / 8, c->js), "zero image");
+    clear_dev(c, h->acc.p, h->acc.bytes);
+    clear_dev(c, h->hist.p, h->hist.bytes);
+    clear_dev(c, h->image.p, h->image.bytes);
   }
   for (auto& g : c->J().grids) {
-    hip_check(mdx_launch::zero(g->d_dose.as<double>(), g->d_dose.bytes / 8, c->js), "zero dose");
-    hip_check(mdx_launch::zero_u32(g->d_dose_n.as<uint32_t>(), g->d_dose_n.bytes / 4, c->js), "zero n");
+    clear_dev(c, g->d_dose.p, g->d_dose.bytes);
+    clear_dev(c, g->d_dose_n.p, g->d_dose_n.bytes);
   }
-  if (c->J().d_bg.p) hip_check(mdx_launch::zero(c->J().d_bg.as<double>(), 3, c->js), "zero bg");
+  if (c->J().d_bg.p) clear_dev(c, c->J().d_bg.p, c->J().d_bg.bytes);
   hip_check(hipStreamSynchronize(c->js), "reset tallies");
 }
 
@@ -657,6 +747,13 @@ uint64_t fill_round(mdx_ctx* c, int n_jobs, const int* ids, const uint64_t* firs
   return prefix[n_jobs];
 }
 
+// Workgroups of a graph round: the queue kernel's persistent blocks fill
+// every slot a CU has, so a few slots (one per XCD of the MI355X) are left
+// free for what the next jobs' set-up puts on the I/O stream meanwhile
+// (small copies and kernels), which would otherwise start only when the round
+// ends.
+int round_blocks(const mdx_ctx* c) { return std::max(1, c->num_cu * c->queue_blocks_per_cu - 8); }
+
 // Captures the graph of one multi-job round: copy of the pinned round buffer,
 // counter zeroing (kernels, no memset nodes) and the queue kernel, which reads
 // the job count and total from the copied header.
@@ -671,11 +768,12 @@ void build_graph(mdx_ctx* c) {
   hipGraph_t g;
   hip_check(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "begin capture");
   hip_check(hipMemcpyAsync(d, c->h_round, kRoundBytes, hipMemcpyHostToDevice, c->stream), "capture copy");
+  hip_check(mdx_launch::zero_list(reinterpret_cast<const uint64_t*>(d + kRoundClearOff), c->stream), "capture clear");
   hip_check(mdx_launch::zero_u64(c->d_counters.as<unsigned long long>(), MDX_NCOUNTER, c->stream), "capture zero");
   hip_check(mdx_launch::zero_u64(c->d_work.as<unsigned long long>(), kRoundMax, c->stream), "capture zero");
   hip_check(mdx_launch::transport_queue_round(c->d_tables.as<mdx_tables_dev>(), d, c->d_work.as<unsigned long long>(),
                                               c->d_counters.as<unsigned long long>(),
-                                              c->num_cu * c->queue_blocks_per_cu, c->refill_min, c->interact_min,
+                                              round_blocks(c), c->refill_min, c->interact_min,
                                               c->stream),
             "capture kernel");
   hip_check(hipStreamEndCapture(c->stream, &g), "end capture");
@@ -712,6 +810,13 @@ void run_jobs(mdx_ctx* c, int n_jobs, const int* ids, const uint64_t* first, con
     const uint64_t total = fill_round(c, n_jobs, ids, first, n, seed, jd, prefix);
     hdr[0] = (uint64_t)n_jobs;
     hdr[1] = total;
+    uint64_t* cl = reinterpret_cast<uint64_t*>(h + kRoundClearOff);
+    std::vector<ClearReq> clears = take_clears(c);
+    cl[0] = clears.size();
+    for (size_t k = 0; k < clears.size(); ++k) {
+      cl[1 + 2 * k] = reinterpret_cast<uint64_t>(clears[k].p);
+      cl[2 + 2 * k] = clears[k].bytes;
+    }
     hip_check(hipEventRecord(c->ev0, c->stream), "hipEventRecord");
     hip_check(hipGraphLaunch(c->graph, c->stream), "hipGraphLaunch");
     hip_check(hipEventRecord(c->ev1, c->stream), "hipEventRecord");
@@ -719,6 +824,7 @@ void run_jobs(mdx_ctx* c, int n_jobs, const int* ids, const uint64_t* first, con
     if (wait) finish_round(c, stats);
     return;
   }
+  flush_clears(c);
   std::vector<mdx_job_dev> jd(n_jobs);
   std::vector<uint64_t> prefix(n_jobs + 1, 0);
   const uint64_t total = fill_round(c, n_jobs, ids, first, n, seed, jd.data(), prefix.data());
@@ -1192,6 +1298,7 @@ int mdx_run(mdx_ctx* c, uint64_t n, uint64_t seed, uint64_t first, int kernel, m
     }
     need(c->J().source_set, "source not set (mdx_set_source)");
     set_device(c);
+    flush_clears(c);
     if (!c->J().d_tallies.p) upload_tallies(c);
     unsigned long long* cnt = c->d_counters.as<unsigned long long>();
     hip_check(mdx_launch::zero_u64(cnt, MDX_NCOUNTER, c->stream), "zero counters");
@@ -1538,6 +1645,7 @@ int mdx_get_voxel_dose_f32(mdx_ctx* c, int grid, uint64_t n_histories, float* ou
       return;
     }
     c->d_dose32.ensure(nv * 8);
+    flush_clears_over(c, g.d_dose.p, g.d_dose.bytes);
     hip_check(mdx_launch::dose_f32(g.d_dose.as<double>(), nv, (double)n_histories, c->d_dose32.as<float>(),
                                    c->io_stream),
               "dose f32");
@@ -1599,6 +1707,7 @@ int mdx_stage_voxel_dose_f32(mdx_ctx* c, int grid, uint64_t n_histories) {
     if (!D.d_stage.pool) D.d_stage.pool = &c->pool;
     D.d_stage.ensure(nv * 8);
     if (!D.ev_stage) hip_check(hipEventCreateWithFlags(&D.ev_stage, hipEventDisableTiming), "hipEventCreate");
+    flush_clears_over(c, g.d_dose.p, g.d_dose.bytes);
     hip_check(mdx_launch::dose_f32(g.d_dose.as<double>(), nv, (double)n_histories, D.d_stage.as<float>(), c->stream),
               "dose f32");
     hip_check(hipEventRecord(D.ev_stage, c->stream), "hipEventRecord");

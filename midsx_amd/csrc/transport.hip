@@ -1262,6 +1262,24 @@ __global__ void k_zero(double* p, size_t n) {
   if (i < n) p[i] = 0.0;
 }
 
+// Clears the ranges of a list {count, (address, bytes) x count} read from
+// device memory (a round buffer), so that one captured launch serves any
+// list.  Addresses are 16-byte aligned and sizes multiples of 4 bytes.
+__global__ void k_zero_list(const uint64_t* __restrict__ list) {
+  const uint64_t n = list[0];
+  const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (uint64_t r = 0; r < n; ++r) {
+    char* base = reinterpret_cast<char*>(list[1 + 2 * r]);
+    const size_t bytes = list[2 + 2 * r];
+    uint4* q = reinterpret_cast<uint4*>(base);
+    const size_t n16 = bytes / 16;
+    for (size_t i = tid; i < n16; i += stride) q[i] = make_uint4(0u, 0u, 0u, 0u);
+    const size_t tail = (bytes - n16 * 16) / 4;
+    if (tid < tail) reinterpret_cast<uint32_t*>(base + n16 * 16)[tid] = 0u;
+  }
+}
+
 __global__ void k_zero_u32(uint32_t* p, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0u;
@@ -1381,6 +1399,11 @@ hipError_t kahn(const mdx_tables_dev* T, int slot, int n, const float* E, uint64
 hipError_t zero(double* p, size_t n, hipStream_t st) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_zero, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+hipError_t zero_list(const uint64_t* list, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_list, dim3(2048), dim3(256), 0, st, list);
   return hipGetLastError();
 }
 

@@ -250,3 +250,46 @@ def test_free_flights_match_majorant_steps():
     b = ctx.run(20000, seed=9, first=0, kernel=M.KERNEL_QUEUE)
     assert a["steps"] == b["steps"] and a["real"] == b["real"]
     ctx.close()
+
+
+def test_setup_during_round_clears_reused_blocks():
+    """A job set up while a graph round is in flight has its dose and tally
+    buffers cleared by the next round's graph (the set-up calls do not wait
+    for the round).  Its buffers here come back from the pool holding a
+    released job's sums, so a lost clear would show in its results, which
+    must equal those of the same job set up with the device idle."""
+    ctx = M.Context(0)
+    ctx.build_tables(list(scenes.DOSE_MATERIALS))
+    ctx.set_graphs(True)
+    small = dict(shape=(32, 32, 32), voxel_cm=0.2, phantom_seed=2, image=(16, 16), energy=50e3)
+    big = dict(shape=(64, 64, 64), voxel_cm=0.2, phantom_seed=1, image=(32, 32), energy=80e3)
+
+    def results(j):
+        ctx.job_select(j)
+        s, s2, n = ctx.voxel_dose(0)
+        return s.copy(), n.copy(), ctx.tally(0)["image"].copy(), ctx.background_dose()
+
+    ref = ctx.job_create()
+    scenes.setup_dose_job(ctx, small)
+    ref_st = ctx.run_jobs([ref], [200000], [0], [5])
+    want = results(ref)
+    ctx.job_release(ref)
+    dirty = ctx.job_create()  # same shapes: its blocks return to the pool with sums in them
+    scenes.setup_dose_job(ctx, small)
+    ctx.run_jobs([dirty], [200000], [0], [77])
+    ctx.job_release(dirty)
+    b = ctx.job_create()
+    scenes.setup_dose_job(ctx, big)
+    ctx.run_jobs_async([b], [4_000_000], [0], [3])
+    j = ctx.job_create()
+    scenes.setup_dose_job(ctx, small)  # while the round runs
+    ctx.wait()
+    st = ctx.run_jobs([j], [200000], [0], [5])
+    got = results(j)
+    for k in ("steps", "real", "deposits", "surface_hits"):
+        assert st[k] == ref_st[k], k
+    for a, m in zip(want[:3], got[:3]):
+        assert np.count_nonzero(a) > 10
+        assert np.allclose(m, a, rtol=1e-12, atol=1e-12 * np.abs(a).max()), np.max(np.abs(m - a))
+    assert np.allclose(got[3], want[3], rtol=1e-12, atol=0)
+    ctx.close()
