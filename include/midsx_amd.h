@@ -256,14 +256,19 @@ int mdx_set_rita_builder(mdx_ctx* ctx, int on_device);
  * voxels).  Other launches use the global form. */
 #define MDX_DOSE_F32C_LDS 3
 int mdx_set_dose_precision(mdx_ctx* ctx, int mode);
-/* Run mdx_run_jobs rounds as a replayed hipGraph (round buffer copy, counter
- * zeroing, queue kernel), at most 1024 jobs per round. */
+/* Run mdx_run_jobs rounds as a replayed hipGraph (round buffer copy, the
+ * clears queued by job set-up during the previous round, counter zeroing,
+ * queue kernel), at most 1024 jobs per round. */
 int mdx_set_graphs(mdx_ctx* ctx, int enable);
 /* mdx_run_jobs split in two: the round is queued and the call returns; while
  * it runs, mdx_get_tally, mdx_get_voxel_dose and mdx_get_background_dose copy
  * on a second stream beside it (a job in the running round reads a partial
- * sum), and every other call waits for the round first.  mdx_wait waits for
- * the round and returns its counters.  One round in flight at a time. */
+ * sum), and so does the set-up (domain, grids, source, tallies, voxel dose)
+ * of jobs that are not in the round; with graphs on, the clearing of their
+ * dose and tally buffers is left to the next round's graph (reading such a
+ * buffer before clears it first).  Every other call waits for the round.
+ * mdx_wait waits for the round and returns its counters.  One round in
+ * flight at a time. */
 int mdx_run_jobs_async(mdx_ctx* ctx, int n_jobs, const int* jobs, const uint64_t* first, const uint64_t* n,
                        const uint64_t* seed);
 int mdx_wait(mdx_ctx* ctx, mdx_stats* stats);
