@@ -348,28 +348,38 @@ __device__ __forceinline__ float rita_finish(const mdx_tables_dev* T, int row0, 
 
 // probability_dist.cpp:73-88: energy bracket by lower_bound - 1 on the
 // incoherent-CS grid (the largest i with E_i < E, clamped to the last
-// interval; the energy hash gives the start), x at both energies from the
-// same y, then log-log interpolation of (x + 1) in E.
+// interval), x at both energies from the same y, then log-log interpolation
+// of (x + 1) in E.  The energy hash gives the bracket, which holds unless its
+// bin contains more grid energies; everything that depends only on the
+// bracket (the two energies, the row offsets, both rows' guide entries) is
+// loaded at once on that assumption and reloaded when the check fails.
 __device__ __forceinline__ float rita_sample(const mdx_tables_dev* T, const mdx_material_dev& M, float E, float y) {
   const float* Eg = T->rita_E + M.rita_eoff;
+  const int* ro = T->rita_row_off + M.rita_roff;
   const int ne = M.rita_ne;
   int b = (int)((logf(E) - M.rita_lnE0) * M.rita_hinv);
   b = b < 0 ? 0 : (b >= M.rita_nh ? M.rita_nh - 1 : b);
-  int i = gld(T->rita_ehash + M.rita_hoff + b);
-  float e0 = i >= 0 ? gld(Eg + i) : 0.0f;
-  float e1 = i + 1 < ne ? gld(Eg + i + 1) : INFINITY;
-  if (!(e0 < E || i < 0) || !(e1 >= E)) {  // the bin holds more energies (or E sits on its edge)
+  int il = gld(T->rita_ehash + M.rita_hoff + b);
+  il = il < 0 ? 0 : (il > ne - 2 ? ne - 2 : il);
+  float e0 = gld(Eg + il), e1 = gld(Eg + il + 1);
+  int r0 = gld(ro + il), r1 = gld(ro + il + 1), r2 = gld(ro + il + 2);
+  int jl = rita_guide(T, M.rita_eoff + il, y), ju = rita_guide(T, M.rita_eoff + il + 1, y);
+  if (!((e0 < E || il == 0) && (E <= e1 || il == ne - 2))) {  // the hash bin holds more energies
+    int i = il;
     while (i >= 0 && gld(Eg + i) >= E) --i;
     while (i + 1 < ne && gld(Eg + i + 1) < E) ++i;
+    il = i < 0 ? 0 : (i > ne - 2 ? ne - 2 : i);
+    e0 = gld(Eg + il);
+    r0 = gld(ro + il);
+    r1 = gld(ro + il + 1);
+    r2 = gld(ro + il + 2);
+    jl = rita_guide(T, M.rita_eoff + il, y);
+    ju = rita_guide(T, M.rita_eoff + il + 1, y);
   }
-  const int il = i < 0 ? 0 : (i > ne - 2 ? ne - 2 : i);
-  const int* ro = T->rita_row_off + M.rita_roff;
-  const int r0 = gld(ro + il), r1 = gld(ro + il + 1), r2 = gld(ro + il + 2);
-  const int gl = M.rita_eoff + il;
   const int ml = r1 - r0, mu = r2 - r1;
-  const float wl = rita_finish(T, r0, ml, y, rita_nodes(T, r0, ml, rita_guide(T, gl, y)));
-  const float wu = rita_finish(T, r1, mu, y, rita_nodes(T, r1, mu, rita_guide(T, gl + 1, y)));
-  const float f = log_ratio(E, gld(Eg + il)) * gld(T->rita_inv_dlnE + M.rita_eoff + il);
+  const float wl = rita_finish(T, r0, ml, y, rita_nodes(T, r0, ml, jl));
+  const float wu = rita_finish(T, r1, mu, y, rita_nodes(T, r1, mu, ju));
+  const float f = log_ratio(E, e0) * gld(T->rita_inv_dlnE + M.rita_eoff + il);
   const float w = (wl > 0.0f && wu > 0.0f) ? wl * expf(f * log_ratio(wu, wl)) : wl;
   return w - 1.0f;
 }
