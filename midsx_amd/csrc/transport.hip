@@ -852,7 +852,9 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
         cslot = -1;
         nstep = 0;
         tfl = next_tally_event(TL, px, py, pz, dx, dy, dz);
-        fly = false;
+        // a source photon in the background starts with a free flight
+        uint64_t lin0;
+        fly = flights && in_domain(D, px, py, pz) && voxel_lookup(D, px, py, pz, lin0) < 0;
         alive = true;
       }
     }
