@@ -229,6 +229,8 @@ def precision_study(ctx, n_hist, step=10 ** 8):
     from midsx_amd import scenes
     job = ctx.job_create()
     scenes.setup_slab(ctx)
+    keep = ctx.queue_params()
+    tuned = ctx.autotune(probe_histories=4000000)  # the slab's own launch parameters
     out = {}
     res = {}
     for name, mode in (("fp64", M.DOSE_F64), ("fp32", M.DOSE_F32), ("fp32_compensated", M.DOSE_F32C),
@@ -253,8 +255,10 @@ def precision_study(ctx, n_hist, step=10 ** 8):
         res[name].update({"rel_err_sum_max": float(e.max()), "rel_err_sum_median": float(np.median(e)),
                           "rel_err_sum2_max": float(e2.max()), "same_deposits": bool(np.array_equal(cnt, c64))})
     ctx.job_release(job)
+    ctx.set_queue_params(*keep)
     return {"workload": "slab 40x40x10 water voxels, 60 keV beam, %d histories per mode, identical RNG streams; "
                         "errors over the %d voxels with >= 1000 deposits" % (n_hist, int(roi.sum())),
+            "queue_params": {k: tuned[k] for k in ("blocks_per_cu", "refill_min", "interact_min")},
             "modes": res}
 
 
