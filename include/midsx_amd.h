@@ -235,6 +235,21 @@ int mdx_run_jobs(mdx_ctx* ctx, int n_jobs, const int* jobs, const uint64_t* firs
  * has no volume (AACuboid) tally, whose per-segment counts (volume_tally.cpp:
  * 5-27) depend on where steps start. */
 int mdx_set_flights(mdx_ctx* ctx, int enable);
+/* Local majorants for the queue kernel (default on; a domain with one grid).
+ * The grid is cut into macro cells of 8 x 8 x 8 voxels; each cell
+ * records the material that bounds the total cross section of every
+ * material in it at all energies from 1 to 500 keV, and the radius (in
+ * cells) around it within which that still holds and the grid continues.
+ * Inside that radius the Woodcock step samples with the bounding material's
+ * cross section instead of the global majorant, and a step that would leave
+ * the radius stops at its edge without a collision test (memoryless
+ * restart), so the distribution of real interactions is the reference's
+ * (physics_engine.cpp:38-66) with fewer rejected steps.  Off: the global
+ * majorant everywhere.  No reference counterpart. */
+int mdx_set_local_majorants(mdx_ctx* ctx, int enable);
+/* The selected job's macro-cell table (cells[3] cells per axis, 0 when there
+ * is none; out: class | radius << 8 per cell, x fastest; may be NULL). */
+int mdx_get_macro_cells(mdx_ctx* ctx, int cells[3], uint16_t* out);
 /* Where mdx_build_tables builds the Rayleigh RITA tables (FP64 either way):
  * on the device, one wavefront per energy (default for device contexts), or
  * on the host threads (always for host-only contexts).  The adaptive scheme

@@ -45,6 +45,7 @@ EXPORTS = [
     "mdx_run_jobs_async", "mdx_wait", "mdx_mem_info", "mdx_host_alloc", "mdx_host_free",
     "mdx_set_flights", "mdx_set_rita_builder", "mdx_set_dose_precision", "mdx_get_voxel_dose_f32", "mdx_stage_voxel_dose_f32",
     "mdx_read_staged_dose_f32_async", "mdx_read_done", "mdx_copy_probe", "mdx_air_muen", "mdx_set_air_muen_table",
+    "mdx_set_local_majorants", "mdx_get_macro_cells",
 ]
 
 
@@ -375,6 +376,22 @@ class Context:
     def set_flights(self, on=True):
         """Background free flights for the selected job (queue kernel)."""
         self._call(self.lib.mdx_set_flights, self._h, ctypes.c_int(1 if on else 0))
+
+    def set_local_majorants(self, on=True):
+        """Local majorants on macro cells for the queue kernel (all jobs)."""
+        self._call(self.lib.mdx_set_local_majorants, self._h, ctypes.c_int(1 if on else 0))
+
+    def macro_cells(self):
+        """The selected job's macro-cell table as (class, radius) arrays of
+        shape (cz, cy, cx), or None when it has none."""
+        cells = (ctypes.c_int * 3)()
+        self._call(self.lib.mdx_get_macro_cells, self._h, cells, None)
+        if cells[0] == 0:
+            return None
+        w = np.zeros(cells[0] * cells[1] * cells[2], np.uint16)
+        self._call(self.lib.mdx_get_macro_cells, self._h, cells, _ptr(w, ctypes.c_uint16))
+        w = w.reshape(cells[2], cells[1], cells[0])
+        return (w & 255).astype(np.int32), (w >> 8).astype(np.int32)
 
     def set_graphs(self, on=True):
         self._call(self.lib.mdx_set_graphs, self._h, ctypes.c_int(1 if on else 0))

@@ -160,7 +160,10 @@ struct GridHost {
   bool present[256] = {false};  // material IDs the grid holds
   DBuf d_ids, d_dose, d_dose_n;  // d_dose holds the sums in the job's MDX_DOSE_* layout
   int dose_mode = -1;            // layout d_dose was set up for
-  explicit GridHost(Pool* pl) : d_ids(pl), d_dose(pl), d_dose_n(pl) {}
+  DBuf d_macro;                  // local majorants per macro cell (class | radius << 8)
+  int macro_gen = -1;            // tables generation d_macro was built for
+  int mc[3] = {0, 0, 0};         // macro cells per axis
+  explicit GridHost(Pool* pl) : d_ids(pl), d_dose(pl), d_dose_n(pl), d_macro(pl) {}
   size_t voxels() const { return (size_t)dims[0] * dims[1] * dims[2]; }
 };
 
@@ -232,6 +235,11 @@ struct mdx_ctx {
   mdx::Packed packed;
   DBuf d_xs, d_nodes, d_hash, d_rE, d_rinv, d_roff, d_rrows, d_rhash, d_rguide, d_tables;
   bool tables_ready = false;
+  int tables_gen = 0;  // bumped by every table upload
+  // per slot k: the slots whose total cross section k's bounds from above on
+  // [kMacroElo, kMacroEhi] (local majorants)
+  std::vector<uint32_t> dominated;
+  bool local_majorants = true;
   bool rita_on_device = true;  // RITA tables built by k_rita_build (FP64) instead of on the host
 
   // scenes
@@ -453,6 +461,105 @@ void read_back(mdx_ctx* c, void* dst, const void* src, size_t bytes, const char*
   hip_check(hipStreamSynchronize(c->io_stream), what);
 }
 
+// Local majorants.  Material k bounds material m when its total cross
+// section is at least (1 + 1e-5) times m's at every node of either table in
+// [kMacroElo, kMacroEhi] (both are log-log linear between their nodes, so the
+// ratio's extremes lie on the nodes); the margin covers the FP32 evaluation.
+constexpr double kMacroElo = 1.0e3, kMacroEhi = 5.0e5;
+
+std::vector<uint32_t> dominance(const std::vector<std::unique_ptr<mdx::MaterialTables>>& mats) {
+  const size_t n = std::min<size_t>(mats.size(), 32);
+  std::vector<uint32_t> dom(n, 0);
+  for (size_t a = 0; a < n; ++a)
+    for (size_t b = 0; b < n; ++b) {
+      std::vector<double> xs = {kMacroElo, kMacroEhi};
+      for (const mdx::Table* t : {&mats[a]->total, &mats[b]->total})
+        for (double x : t->x)
+          if (x > kMacroElo && x < kMacroEhi) xs.push_back(x);
+      bool ok = true;
+      for (double x : xs)
+        if (!(mats[a]->total_i(x) >= mats[b]->total_i(x) * (1.0 + 1e-5) || a == b)) {
+          ok = false;
+          break;
+        }
+      if (ok) dom[a] |= 1u << b;
+    }
+  return dom;
+}
+
+// Chebyshev distance (in cells) from every cell to the nearest bad cell or to
+// the layer of cells around the grid: the two-pass chamfer transform with
+// unit weights on the 26-neighbourhood, which is exact for this metric.
+std::vector<int> cheb_distance(const std::vector<uint8_t>& bad, int nx, int ny, int nz) {
+  const size_t n = (size_t)nx * ny * nz;
+  std::vector<int> d(n);
+  auto id = [&](int i, int j, int k) { return (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k); };
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i)
+        d[id(i, j, k)] = bad[id(i, j, k)] ? 0 : std::min({i + 1, nx - i, j + 1, ny - j, k + 1, nz - k});
+  auto relax = [&](int i, int j, int k, int sgn) {
+    int v = d[id(i, j, k)];
+    for (int dk = -1; dk <= 1; ++dk)
+      for (int dj = -1; dj <= 1; ++dj)
+        for (int di = -1; di <= 1; ++di) {
+          const int order = dk * 9 + dj * 3 + di;  // neighbours already visited in this pass
+          if (order * sgn >= 0) continue;
+          const int ii = i + di, jj = j + dj, kk = k + dk;
+          if (ii < 0 || jj < 0 || kk < 0 || ii >= nx || jj >= ny || kk >= nz) continue;
+          v = std::min(v, d[id(ii, jj, kk)] + 1);
+        }
+    d[id(i, j, k)] = v;
+  };
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) relax(i, j, k, 1);
+  for (int k = nz - 1; k >= 0; --k)
+    for (int j = ny - 1; j >= 0; --j)
+      for (int i = nx - 1; i >= 0; --i) relax(i, j, k, -1);
+  return d;
+}
+
+// The macro-cell table of grid G (one grid per domain): per cell the slot of
+// the material that bounds every material present in the cell (the one with
+// the smallest cross sections among those that do), and the Chebyshev radius
+// in cells within which every cell holds only materials it bounds and lies
+// inside the grid.
+void build_macro(mdx_ctx* c, GridHost& G) {
+  for (int a = 0; a < 3; ++a) G.mc[a] = (G.dims[a] + MDX_MACRO_EDGE - 1) >> MDX_MACRO_SHIFT;
+  const size_t nc = (size_t)G.mc[0] * G.mc[1] * G.mc[2];
+  DBuf dm;
+  dm.alloc(nc * 4);
+  hip_check(mdx_launch::macro_mask(G.d_ids.as<uint8_t>(), G.dims[0], G.dims[1], G.dims[2], G.mc[0], G.mc[1], G.mc[2],
+                                   c->d_tables.as<mdx_tables_dev>(), dm.as<uint32_t>(), c->js),
+            "macro mask");
+  std::vector<uint32_t> mask(nc);
+  hip_check(hipMemcpyAsync(mask.data(), dm.p, nc * 4, hipMemcpyDeviceToHost, c->js), "macro mask");
+  hip_check(hipStreamSynchronize(c->js), "macro mask");
+  const std::vector<uint32_t>& dom = c->dominated;
+  const int ns = (int)dom.size();
+  std::vector<uint8_t> cls(nc, 255);
+  for (size_t i = 0; i < nc; ++i) {
+    const uint32_t m = mask[i];
+    for (int k = 0; k < ns; ++k)
+      if ((m >> k & 1u) && (m & ~dom[k]) == 0 && (cls[i] == 255 || (dom[cls[i]] >> k & 1u))) cls[i] = (uint8_t)k;
+  }
+  std::vector<uint16_t> word(nc, 255);
+  std::vector<uint8_t> bad(nc);
+  for (int k = 0; k < ns; ++k) {
+    bool used = false;
+    for (size_t i = 0; i < nc && !used; ++i) used = cls[i] == k;
+    if (!used) continue;
+    for (size_t i = 0; i < nc; ++i) bad[i] = (mask[i] & ~dom[k]) != 0;
+    const std::vector<int> d = cheb_distance(bad, G.mc[0], G.mc[1], G.mc[2]);
+    for (size_t i = 0; i < nc; ++i)
+      if (cls[i] == k) word[i] = (uint16_t)(k | (std::min(std::max(d[i] - 1, 0), 255) << 8));
+  }
+  G.d_macro.upload(word.data(), nc, c->js);
+  hip_check(hipStreamSynchronize(c->js), "macro upload");
+  G.macro_gen = c->tables_gen;
+}
+
 void upload_domain(mdx_ctx* c) {
   mdx_domain_dev D{};
   for (int k = 0; k < 3; ++k) D.dim_space[k] = (float)c->J().dim_space[k];
@@ -470,6 +577,14 @@ void upload_domain(mdx_ctx* c) {
       d.n[k] = G.dims[k];
     }
     d.ids = G.d_ids.as<uint8_t>();
+    d.macro = nullptr;
+    if (c->tables_ready && c->local_majorants && c->J().grids.size() == 1 && !c->dominated.empty()) {
+      if (G.macro_gen != c->tables_gen) build_macro(c, G);
+      d.macro = G.d_macro.as<uint16_t>();
+      d.mcx = G.mc[0];
+      d.mcy = G.mc[1];
+      d.mcell = (float)(MDX_MACRO_EDGE * std::min(G.spacing[0], std::min(G.spacing[1], G.spacing[2])));
+    }
     if (c->J().voxel_dose) {
       const int mode = c->J().dose_mode;
       const size_t per = mode == MDX_DOSE_F32 ? 8 : 16;
@@ -612,6 +727,8 @@ void build_and_upload(mdx_ctx* c, const std::vector<int>& ids) {
   c->d_tables.upload(&T, 1, c->stream);
   hip_check(hipStreamSynchronize(c->stream), "upload tables");
   c->tables_ready = true;
+  ++c->tables_gen;
+  c->dominated = dominance(c->mats);
   const int keep = c->cur;
   for (size_t j = 0; j < c->jobs.size(); ++j) {
     if (!c->jobs[j] || c->jobs[j]->domain_of >= 0) continue;
@@ -1453,6 +1570,37 @@ int mdx_set_flights(mdx_ctx* c, int enable) {
   return guard([&] {
     need(c, "NULL context");
     c->J().flights = enable != 0;
+  });
+}
+
+int mdx_set_local_majorants(mdx_ctx* c, int enable) {
+  return guard([&] {
+    need(c, "NULL context");
+    c->local_majorants = enable != 0;
+    if (c->device < 0 || !c->tables_ready) return;
+    const int keep = c->cur;
+    for (size_t j = 0; j < c->jobs.size(); ++j) {  // every job's domain record
+      if (!c->jobs[j] || c->jobs[j]->domain_of >= 0 || !c->jobs[j]->domain_set) continue;
+      c->cur = (int)j;
+      set_device_job(c);
+      upload_domain(c);
+    }
+    c->cur = keep;
+  });
+}
+
+int mdx_get_macro_cells(mdx_ctx* c, int cells[3], uint16_t* out) {
+  return guard([&] {
+    need(c && cells, "NULL argument");
+    Job& D = c->DJ();
+    cells[0] = cells[1] = cells[2] = 0;
+    if (D.grids.size() != 1 || !D.grids[0]->d_macro.p || !c->local_majorants) return;
+    GridHost& g = *D.grids[0];
+    for (int a = 0; a < 3; ++a) cells[a] = g.mc[a];
+    if (out) {
+      set_device_nowait(c);
+      read_back(c, out, g.d_macro.p, g.d_macro.bytes, "read macro cells");
+    }
   });
 }
 

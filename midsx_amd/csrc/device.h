@@ -252,7 +252,8 @@ __device__ __forceinline__ int trunc_quotient(float q, float s, float inv_s) {
 // (position - origin) / spacing.  A point exactly on a grid's far face maps to
 // the last voxel (the reference's at() throws there).  Returns the grid, or -1
 // for the background voxel.
-__device__ __forceinline__ int voxel_lookup(const mdx_domain_dev* D, float x, float y, float z, uint64_t& lin) {
+__device__ __forceinline__ int voxel_lookup(const mdx_domain_dev* D, float x, float y, float z, uint64_t& lin,
+                                            uint32_t* cell = nullptr) {
   for (int g = 0; g < D->n_grids; ++g) {
     const mdx_grid_dev& G = D->grids[g];
     float qx = x - G.origin[0], qy = y - G.origin[1], qz = z - G.origin[2];
@@ -264,6 +265,9 @@ __device__ __forceinline__ int voxel_lookup(const mdx_domain_dev* D, float x, fl
       j = j >= G.n[1] ? G.n[1] - 1 : j;
       k = k >= G.n[2] ? G.n[2] - 1 : k;
       lin = (uint64_t)i + (uint64_t)G.n[0] * ((uint64_t)j + (uint64_t)G.n[1] * (uint64_t)k);
+      if (cell)  // the voxel's macro cell (local majorants)
+        *cell = (uint32_t)(i >> MDX_MACRO_SHIFT) +
+                (uint32_t)G.mcx * ((uint32_t)(j >> MDX_MACRO_SHIFT) + (uint32_t)G.mcy * (uint32_t)(k >> MDX_MACRO_SHIFT));
       return g;
     }
   }

@@ -697,6 +697,7 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
   // vector-memory round trip, and uniform ones are not re-read from memory
   // after every atomic.
   __shared__ mdx_material_dev lmat[MDX_MAX_MATERIALS];
+  __shared__ mdx_curve lmaj;  // the global majorant
   __shared__ uint8_t lslot[256];
   __shared__ JobDesc desc[MULTI ? 2 * kWaves : 1];
   extern __shared__ float ldose_mem[];  // MDX_DOSE_F32C_LDS: (sum, error) per voxel of grid 0
@@ -708,6 +709,7 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
   }
   block_copy(lmat, T->mat, T->n_mat * (int)sizeof(mdx_material_dev));
   block_copy(lslot, T->id_to_slot, 256);
+  block_copy(&lmaj, &T->majorant, (int)sizeof(mdx_curve));
   if (!MULTI) {
     block_copy(&desc[0].D, jobs[0].D, (int)sizeof(mdx_domain_dev));
     block_copy(&desc[0].S, jobs[0].S, (int)sizeof(mdx_source_dev));
@@ -752,9 +754,14 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
   StepRng srng;   // Woodcock steps
   float px = 0, py = 0, pz = 0, dx = 0, dy = 0, dz = 1, E = 1;
   int nc = 0, ni = 0;
-  float smax = 1, step_scale = 1;  // step_scale = ln 2 / sigma_max
+  // The majorant of the next step: the global one (mcls 255) or, inside a
+  // grid with local majorants, the total cross section of material mcls,
+  // which bounds every cross section within dsafe of the photon.
+  float smax = 1, step_scale = 1;  // step_scale = ln 2 / majorant
+  int mcls = 255;
+  float dsafe = INFINITY;
   int cslot = -1;
-  float stot = 0, thr = 0;         // thr = 1 - sigma_tot / sigma_max
+  float stot = 0, thr = 0;         // thr = 1 - sigma_tot / majorant
   // the pending step: start point, length, voxel
   float qx = 0, qy = 0, qz = 0, qs = 0;
   int qg = -1;
@@ -847,8 +854,10 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
         generate(S, rng, px, py, pz, dx, dy, dz, E);
         ++c_hist;
         nc = ni = 0;
-        smax = curve_eval(T, T->majorant, E);
+        smax = curve_eval(T, lmaj, E);
         step_scale = 0.69314718055994531f / smax;
+        mcls = 255;
+        dsafe = INFINITY;
         cslot = -1;
         nstep = 0;
         tfl = next_tally_event(TL, px, py, pz, dx, dy, dz);
@@ -875,14 +884,24 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
       uint64_t lin = 0;
       float s = 0.0f;
       int slot = D->bg_slot;  // free flights: the background's cross section
+      const bool flew = fly;
+      bool limited = false;  // the step stopped where the local majorant's bound ends
+      uint32_t mword = 255u;  // the new position's macro cell (class | radius << 8)
       if (!fly) {
         s = -__log2f(u1) * step_scale;
+        if (s > dsafe) {
+          s = dsafe;
+          limited = true;
+        }
         px = px + s * dx;
         py = py + s * dy;
         pz = pz + s * dz;
         escaped = !in_domain(D, px, py, pz);
         if (!escaped) {
-          g = voxel_lookup(D, px, py, pz, lin);
+          uint32_t cell = 0;
+          g = voxel_lookup(D, px, py, pz, lin, &cell);
+          const uint16_t* mc = D->grids[0].macro;
+          if (g == 0 && mc) mword = gld(mc + cell);
           slot = g < 0 ? D->bg_slot : (int)lslot[gld(D->grids[g].ids + lin)];
         }
       }
@@ -928,16 +947,28 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
           replay = true;
           alive = false;
         }
-      } else if (!(u2 < thr)) {  // real interaction: park it for the interaction phase
+      } else if (!limited && !(u2 < thr)) {  // real interaction: park it for the interaction phase
         pend = true;
         qx = x0; qy = y0; qz = z0; qs = s;
         qg = g;
         qlin = lin;
-      } else {
+      } else {  // a rejected step, or one cut at the local majorant's bound (memoryless restart)
         tfl -= s;
         replay = tfl <= 0.0f;  // the step may reach a tally
         if (nstep >= kMaxSteps) alive = false;
         fly = flights && g < 0;
+      }
+      if (!escaped && !flew) {  // the majorant for the step from the new position
+        int ncls = (int)(mword & 255u);
+        const uint32_t nr = mword >> 8;
+        if (nr == 0u || E < 1.0e3f) ncls = 255;
+        if (ncls != mcls) {
+          mcls = ncls;
+          smax = curve_eval(T, ncls == 255 ? lmaj : lmat[ncls].total, E);
+          step_scale = 0.69314718055994531f / smax;
+          thr = 1.0f - stot / smax;
+        }
+        dsafe = ncls == 255 ? INFINITY : (float)nr * D->grids[0].mcell;
       }
       if (replay) {
         Step r{x0, y0, z0, dx, dy, dz, seg, E, nc, ni, 0, 0, 0.0f, false};
@@ -971,7 +1002,11 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
           rotate(dx, dy, dz, mu, 2.0f * kPiF * rng.uniform());
           dep = E - Ep;
           E = Ep;
-          smax = curve_eval(T, T->majorant, E);
+          if (E < 1.0e3f) {
+            mcls = 255;
+            dsafe = INFINITY;
+          }
+          smax = curve_eval(T, mcls == 255 ? lmaj : lmat[mcls].total, E);
           step_scale = 0.69314718055994531f / smax;
           cslot = -1;
         } else {
@@ -1264,6 +1299,25 @@ __global__ void k_zero(double* p, size_t n) {
   if (i < n) p[i] = 0.0;
 }
 
+// The set of material slots present in each macro cell of a grid (bit s for
+// slot s), one thread per cell.
+__global__ void k_macro_mask(const uint8_t* __restrict__ ids, int nx, int ny, int nz, int mcx, int mcy, int mcz,
+                             const mdx_tables_dev* __restrict__ T, uint32_t* __restrict__ mask) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= mcx * mcy * mcz) return;
+  const int ci = c % mcx, cj = (c / mcx) % mcy, ck = c / (mcx * mcy);
+  const int i0 = ci * MDX_MACRO_EDGE, j0 = cj * MDX_MACRO_EDGE, k0 = ck * MDX_MACRO_EDGE;
+  const int i1 = min(i0 + MDX_MACRO_EDGE, nx), j1 = min(j0 + MDX_MACRO_EDGE, ny), k1 = min(k0 + MDX_MACRO_EDGE, nz);
+  uint32_t m = 0;
+  for (int k = k0; k < k1; ++k)
+    for (int j = j0; j < j1; ++j)
+      for (int i = i0; i < i1; ++i) {
+        const uint8_t slot = T->id_to_slot[ids[(size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k)]];
+        m |= slot < 32 ? (1u << slot) : 0x80000000u;
+      }
+  mask[c] = m;
+}
+
 // Clears the ranges of a list {count, (address, bytes) x count} read from
 // device memory (a round buffer), so that one captured launch serves any
 // list.  Addresses are 16-byte aligned and sizes multiples of 4 bytes.
@@ -1401,6 +1455,14 @@ hipError_t kahn(const mdx_tables_dev* T, int slot, int n, const float* E, uint64
 hipError_t zero(double* p, size_t n, hipStream_t st) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_zero, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+hipError_t macro_mask(const uint8_t* ids, int nx, int ny, int nz, int mcx, int mcy, int mcz, const mdx_tables_dev* T,
+                      uint32_t* mask, hipStream_t st) {
+  const int n = mcx * mcy * mcz;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_macro_mask, dim3((n + 255) / 256), dim3(256), 0, st, ids, nx, ny, nz, mcx, mcy, mcz, T, mask);
   return hipGetLastError();
 }
 
