@@ -47,15 +47,21 @@ struct Step {
 
 // LDS budget for block-level IncidentEnergy histograms (doubles); tallies whose
 // histograms do not fit go straight to global atomics.
-constexpr int kLdsHist = 2048;
+constexpr int kLdsHist = 2048;       // plain kernel
+constexpr int kQueueLdsHist = 512;   // queue kernel (its LDS also holds descriptors and cross-section caches)
+// The queue kernel keeps, per lane, the total cross sections of material
+// slots 0 .. kSigmaCache - 2 and the global majorant (entry kSigmaCache - 1)
+// at the photon's current energy once evaluated.
+constexpr int kSigmaCache = 9;
 
-// Offsets of each tally's histogram in the block's LDS histogram (-1: global).
-__device__ __forceinline__ void hist_layout(const mdx_tallies_dev* TL, int* off) {
+// Offsets of each tally's histogram in the block's LDS histogram of cap
+// doubles (-1: global).
+__device__ __forceinline__ void hist_layout(const mdx_tallies_dev* TL, int* off, int cap = kLdsHist) {
   int used = 0;
   for (int t = 0; t < TL->n; ++t) {
     const mdx_tally_dev& T = TL->t[t];
     int need = (T.kind != MDX_TALLY_AACUBOID && T.hist_bins > 0) ? MDX_NBUCKET * T.hist_bins : 0;
-    if (need > 0 && used + need <= kLdsHist) {
+    if (need > 0 && used + need <= cap) {
       off[t] = used;
       used += need;
     } else {
@@ -635,7 +641,8 @@ static_assert(sizeof(mdx_material_dev) % 4 == 0 && sizeof(mdx_domain_dev) % 4 ==
 // An upper bound of the one-job queue kernel's static LDS (its shared arrays
 // plus 1 KB for alignment), for hosts that cannot ask the runtime.
 constexpr size_t kQueueStaticLdsBound =
-    sizeof(double) * (MDX_MAX_TALLIES * kAccPerTally + kLdsHist) + sizeof(unsigned long long) * MDX_NCOUNTER +
+    sizeof(double) * (MDX_MAX_TALLIES * kAccPerTally + kQueueLdsHist) + sizeof(unsigned long long) * MDX_NCOUNTER +
+    sizeof(float) * kSigmaCache * kBlock + sizeof(mdx_curve) +
     sizeof(mdx_material_dev) * MDX_MAX_MATERIALS + 256 + sizeof(mdx_domain_dev) + sizeof(mdx_source_dev) +
     sizeof(mdx_tallies_dev) + 1024;
 
@@ -688,7 +695,8 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
                                                      unsigned long long* __restrict__ counters, int refill_min,
                                                      int interact_min, uint32_t lds_voxels) {
   __shared__ double acc[MULTI ? 1 : MDX_MAX_TALLIES * kAccPerTally];
-  __shared__ double lhist[MULTI ? 1 : kLdsHist];
+  __shared__ double lhist[MULTI ? 1 : kQueueLdsHist];
+  __shared__ float scache[kWaves][kSigmaCache][64];  // per lane: cached total cross sections
   __shared__ unsigned long long cnt[MDX_NCOUNTER];
   // Per-block copies of what every step and interaction reads: the material
   // descriptors and the ID -> slot map, and the scene descriptors of the
@@ -714,10 +722,10 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
     block_copy(&desc[0].D, jobs[0].D, (int)sizeof(mdx_domain_dev));
     block_copy(&desc[0].S, jobs[0].S, (int)sizeof(mdx_source_dev));
     block_copy(&desc[0].TL, jobs[0].TL, (int)sizeof(mdx_tallies_dev));
-    hist_layout(jobs[0].TL, hoff);
+    hist_layout(jobs[0].TL, hoff, kQueueLdsHist);
     const int ntl = jobs[0].TL->n;
     for (int i = threadIdx.x; i < ntl * kAccPerTally; i += blockDim.x) acc[i] = 0.0;
-    for (int i = threadIdx.x; i < kLdsHist; i += blockDim.x) lhist[i] = 0.0;
+    for (int i = threadIdx.x; i < kQueueLdsHist; i += blockDim.x) lhist[i] = 0.0;
   }
   if (threadIdx.x < MDX_NCOUNTER) cnt[threadIdx.x] = 0;
   __syncthreads();
@@ -760,6 +768,7 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
   float smax = 1, step_scale = 1;  // step_scale = ln 2 / majorant
   int mcls = 255;
   float dsafe = INFINITY;
+  uint32_t cvalid = 0;  // cache entries valid at the current energy
   int cslot = -1;
   float stot = 0, thr = 0;         // thr = 1 - sigma_tot / majorant
   // the pending step: start point, length, voxel
@@ -855,6 +864,8 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
         ++c_hist;
         nc = ni = 0;
         smax = curve_eval(T, lmaj, E);
+        scache[wave][kSigmaCache - 1][lane] = smax;
+        cvalid = 1u << (kSigmaCache - 1);
         step_scale = 0.69314718055994531f / smax;
         mcls = 255;
         dsafe = INFINITY;
@@ -906,7 +917,15 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
         }
       }
       if (!escaped && slot != cslot) {
-        stot = curve_eval(T, lmat[slot].total, E);
+        if (slot < kSigmaCache - 1 && (cvalid >> slot & 1u)) {
+          stot = scache[wave][slot][lane];
+        } else {
+          stot = curve_eval(T, lmat[slot].total, E);
+          if (slot < kSigmaCache - 1) {
+            scache[wave][slot][lane] = stot;
+            cvalid |= 1u << slot;
+          }
+        }
         thr = 1.0f - stot / smax;
         cslot = slot;
       }
@@ -964,7 +983,16 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
         if (nr == 0u || E < 1.0e3f) ncls = 255;
         if (ncls != mcls) {
           mcls = ncls;
-          smax = curve_eval(T, ncls == 255 ? lmaj : lmat[ncls].total, E);
+          const int ce = ncls == 255 ? kSigmaCache - 1 : ncls;  // cache entry
+          if (ce < kSigmaCache && (cvalid >> ce & 1u)) {
+            smax = scache[wave][ce][lane];
+          } else {
+            smax = curve_eval(T, ncls == 255 ? lmaj : lmat[ncls].total, E);
+            if (ce < kSigmaCache) {
+              scache[wave][ce][lane] = smax;
+              cvalid |= 1u << ce;
+            }
+          }
           step_scale = 0.69314718055994531f / smax;
           thr = 1.0f - stot / smax;
         }
@@ -1007,6 +1035,14 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
             dsafe = INFINITY;
           }
           smax = curve_eval(T, mcls == 255 ? lmaj : lmat[mcls].total, E);
+          {
+            const int ce = mcls == 255 ? kSigmaCache - 1 : mcls;
+            cvalid = 0;  // a new energy
+            if (ce < kSigmaCache) {
+              scache[wave][ce][lane] = smax;
+              cvalid = 1u << ce;
+            }
+          }
           step_scale = 0.69314718055994531f / smax;
           cslot = -1;
         } else {
