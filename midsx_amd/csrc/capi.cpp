@@ -123,22 +123,49 @@ struct DBuf {
   }
 };
 
-// The material IDs present in a grid, eight voxels at a time (phantoms hold
-// long runs of one material, so most words repeat the one before).
-void scan_ids(const uint8_t* ids, size_t n, bool* present) {
+// The material IDs present in each macro cell of a grid (MDX_MACRO_EDGE^3
+// voxels, x fastest; 256 bits per cell), read eight voxels at a time along x
+// (a word of one repeated byte sets one bit); threads take the layers of
+// cells along z.  Also the IDs present anywhere.
+void scan_cells(const uint8_t* ids, const int dims[3], std::vector<uint64_t>& cells, int mc[3], bool* present) {
+  for (int a = 0; a < 3; ++a) mc[a] = (dims[a] + MDX_MACRO_EDGE - 1) >> MDX_MACRO_SHIFT;
+  const size_t nc = (size_t)mc[0] * mc[1] * mc[2];
+  cells.assign(nc * 4, 0);
+  const int nx = dims[0], ny = dims[1], nz = dims[2];
+  const int nth = std::max(1, std::min({16, (int)std::thread::hardware_concurrency(), mc[2]}));
+  auto work = [&](int t) {
+    for (int ck = t; ck < mc[2]; ck += nth)
+      for (int cj = 0; cj < mc[1]; ++cj)
+        for (int ci = 0; ci < mc[0]; ++ci) {
+          uint64_t* m = &cells[4 * ((size_t)ci + (size_t)mc[0] * ((size_t)cj + (size_t)mc[1] * ck))];
+          const int i0 = ci * MDX_MACRO_EDGE, w = std::min(MDX_MACRO_EDGE, nx - i0);
+          const int j1 = std::min(cj * MDX_MACRO_EDGE + MDX_MACRO_EDGE, ny);
+          const int k1 = std::min(ck * MDX_MACRO_EDGE + MDX_MACRO_EDGE, nz);
+          for (int k = ck * MDX_MACRO_EDGE; k < k1; ++k)
+            for (int j = cj * MDX_MACRO_EDGE; j < j1; ++j) {
+              const uint8_t* row = ids + i0 + (size_t)nx * ((size_t)j + (size_t)ny * k);
+              if (w == 8) {
+                uint64_t v;
+                std::memcpy(&v, row, 8);
+                const uint64_t b = v & 255u;
+                if (v == b * 0x0101010101010101ull) {
+                  m[b >> 6] |= 1ull << (b & 63);
+                  continue;
+                }
+              }
+              for (int i = 0; i < w; ++i) m[row[i] >> 6] |= 1ull << (row[i] & 63);
+            }
+        }
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < nth; ++t) th.emplace_back(work, t);
+  work(0);
+  for (auto& x : th) x.join();
   std::fill(present, present + 256, false);
-  size_t i = 0;
-  uint64_t last = 0;
-  bool first = true;
-  for (; i + 8 <= n; i += 8) {
-    uint64_t w;
-    std::memcpy(&w, ids + i, 8);
-    if (!first && w == last) continue;
-    first = false;
-    last = w;
-    for (int b = 0; b < 8; ++b) present[(w >> (8 * b)) & 255u] = true;
-  }
-  for (; i < n; ++i) present[ids[i]] = true;
+  uint64_t all[4] = {0, 0, 0, 0};
+  for (size_t c = 0; c < nc; ++c)
+    for (int q = 0; q < 4; ++q) all[q] |= cells[4 * c + q];
+  for (int id = 0; id < 256; ++id) present[id] = (all[id >> 6] >> (id & 63)) & 1u;
 }
 
 std::string default_data_dir() {
@@ -163,6 +190,7 @@ struct GridHost {
   DBuf d_macro;                  // local majorants per macro cell (class | radius << 8)
   int macro_gen = -1;            // tables generation d_macro was built for
   int mc[3] = {0, 0, 0};         // macro cells per axis
+  std::vector<uint64_t> cell_ids;  // material IDs present per macro cell (256 bits each)
   explicit GridHost(Pool* pl) : d_ids(pl), d_dose(pl), d_dose_n(pl), d_macro(pl) {}
   size_t voxels() const { return (size_t)dims[0] * dims[1] * dims[2]; }
 };
@@ -487,37 +515,44 @@ std::vector<uint32_t> dominance(const std::vector<std::unique_ptr<mdx::MaterialT
   return dom;
 }
 
-// Chebyshev distance (in cells) from every cell to the nearest bad cell or to
-// the layer of cells around the grid: the two-pass chamfer transform with
-// unit weights on the 26-neighbourhood, which is exact for this metric.
-std::vector<int> cheb_distance(const std::vector<uint8_t>& bad, int nx, int ny, int nz) {
-  const size_t n = (size_t)nx * ny * nz;
-  std::vector<int> d(n);
-  auto id = [&](int i, int j, int k) { return (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k); };
+// Chebyshev distance (in cells, capped at 255) from every cell to the nearest
+// bad cell or to the layer of cells around the grid: the two-pass chamfer
+// transform with unit weights on the 26-neighbourhood, which is exact for
+// this metric, on an array padded with that layer.
+std::vector<uint8_t> cheb_distance(const std::vector<uint8_t>& bad, int nx, int ny, int nz) {
+  const int px = nx + 2, py = ny + 2, pz = nz + 2;
+  const size_t sy = (size_t)px, sz = (size_t)px * py;
+  std::vector<uint8_t> d((size_t)px * py * pz, 0);
   for (int k = 0; k < nz; ++k)
     for (int j = 0; j < ny; ++j)
       for (int i = 0; i < nx; ++i)
-        d[id(i, j, k)] = bad[id(i, j, k)] ? 0 : std::min({i + 1, nx - i, j + 1, ny - j, k + 1, nz - k});
-  auto relax = [&](int i, int j, int k, int sgn) {
-    int v = d[id(i, j, k)];
-    for (int dk = -1; dk <= 1; ++dk)
-      for (int dj = -1; dj <= 1; ++dj)
-        for (int di = -1; di <= 1; ++di) {
-          const int order = dk * 9 + dj * 3 + di;  // neighbours already visited in this pass
-          if (order * sgn >= 0) continue;
-          const int ii = i + di, jj = j + dj, kk = k + dk;
-          if (ii < 0 || jj < 0 || kk < 0 || ii >= nx || jj >= ny || kk >= nz) continue;
-          v = std::min(v, d[id(ii, jj, kk)] + 1);
-        }
-    d[id(i, j, k)] = v;
-  };
+        d[(i + 1) + sy * (j + 1) + sz * (k + 1)] = bad[(size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k)] ? 0 : 255;
+  std::vector<ptrdiff_t> back;  // the 13 neighbours before a cell in raster order
+  for (int dk = -1; dk <= 1; ++dk)
+    for (int dj = -1; dj <= 1; ++dj)
+      for (int di = -1; di <= 1; ++di)
+        if (dk * 9 + dj * 3 + di < 0) back.push_back(di + (ptrdiff_t)sy * dj + (ptrdiff_t)sz * dk);
+  for (int k = 1; k <= nz; ++k)
+    for (int j = 1; j <= ny; ++j)
+      for (int i = 1; i <= nx; ++i) {
+        const size_t c = i + sy * j + sz * k;
+        int v = d[c];
+        for (ptrdiff_t o : back) v = std::min(v, d[c + o] + 1);
+        d[c] = (uint8_t)v;
+      }
+  for (int k = nz; k >= 1; --k)
+    for (int j = ny; j >= 1; --j)
+      for (int i = nx; i >= 1; --i) {
+        const size_t c = i + sy * j + sz * k;
+        int v = d[c];
+        for (ptrdiff_t o : back) v = std::min(v, d[c - o] + 1);
+        d[c] = (uint8_t)v;
+      }
+  std::vector<uint8_t> out((size_t)nx * ny * nz);
   for (int k = 0; k < nz; ++k)
     for (int j = 0; j < ny; ++j)
-      for (int i = 0; i < nx; ++i) relax(i, j, k, 1);
-  for (int k = nz - 1; k >= 0; --k)
-    for (int j = ny - 1; j >= 0; --j)
-      for (int i = nx - 1; i >= 0; --i) relax(i, j, k, -1);
-  return d;
+      for (int i = 0; i < nx; ++i) out[(size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k)] = d[(i + 1) + sy * (j + 1) + sz * (k + 1)];
+  return out;
 }
 
 // The macro-cell table of grid G (one grid per domain): per cell the slot of
@@ -526,16 +561,15 @@ std::vector<int> cheb_distance(const std::vector<uint8_t>& bad, int nx, int ny, 
 // in cells within which every cell holds only materials it bounds and lies
 // inside the grid.
 void build_macro(mdx_ctx* c, GridHost& G) {
-  for (int a = 0; a < 3; ++a) G.mc[a] = (G.dims[a] + MDX_MACRO_EDGE - 1) >> MDX_MACRO_SHIFT;
   const size_t nc = (size_t)G.mc[0] * G.mc[1] * G.mc[2];
-  DBuf dm;
-  dm.alloc(nc * 4);
-  hip_check(mdx_launch::macro_mask(G.d_ids.as<uint8_t>(), G.dims[0], G.dims[1], G.dims[2], G.mc[0], G.mc[1], G.mc[2],
-                                   c->d_tables.as<mdx_tables_dev>(), dm.as<uint32_t>(), c->js),
-            "macro mask");
-  std::vector<uint32_t> mask(nc);
-  hip_check(hipMemcpyAsync(mask.data(), dm.p, nc * 4, hipMemcpyDeviceToHost, c->js), "macro mask");
-  hip_check(hipStreamSynchronize(c->js), "macro mask");
+  std::vector<uint32_t> mask(nc, 0);
+  for (size_t i = 0; i < nc; ++i)
+    for (int q = 0; q < 4; ++q)
+      for (uint64_t w = G.cell_ids[4 * i + q]; w; w &= w - 1) {
+        const int id = q * 64 + __builtin_ctzll(w);
+        const int slot = c->slot_of(id);
+        mask[i] |= slot < 32 ? 1u << slot : 0x80000000u;
+      }
   const std::vector<uint32_t>& dom = c->dominated;
   const int ns = (int)dom.size();
   std::vector<uint8_t> cls(nc, 255);
@@ -545,16 +579,20 @@ void build_macro(mdx_ctx* c, GridHost& G) {
       if ((m >> k & 1u) && (m & ~dom[k]) == 0 && (cls[i] == 255 || (dom[cls[i]] >> k & 1u))) cls[i] = (uint8_t)k;
   }
   std::vector<uint16_t> word(nc, 255);
-  std::vector<uint8_t> bad(nc);
-  for (int k = 0; k < ns; ++k) {
-    bool used = false;
-    for (size_t i = 0; i < nc && !used; ++i) used = cls[i] == k;
-    if (!used) continue;
+  std::vector<bool> used(ns, false);
+  for (size_t i = 0; i < nc; ++i)
+    if (cls[i] != 255) used[cls[i]] = true;
+  auto radius = [&](int k) {  // cells of class k only: one writer per entry
+    std::vector<uint8_t> bad(nc);
     for (size_t i = 0; i < nc; ++i) bad[i] = (mask[i] & ~dom[k]) != 0;
-    const std::vector<int> d = cheb_distance(bad, G.mc[0], G.mc[1], G.mc[2]);
+    const std::vector<uint8_t> d = cheb_distance(bad, G.mc[0], G.mc[1], G.mc[2]);
     for (size_t i = 0; i < nc; ++i)
-      if (cls[i] == k) word[i] = (uint16_t)(k | (std::min(std::max(d[i] - 1, 0), 255) << 8));
-  }
+      if (cls[i] == k) word[i] = (uint16_t)(k | (std::max((int)d[i] - 1, 0) << 8));
+  };
+  std::vector<std::thread> th;
+  for (int k = 0; k < ns; ++k)
+    if (used[k]) th.emplace_back(radius, k);
+  for (auto& t : th) t.join();
   G.d_macro.upload(word.data(), nc, c->js);
   hip_check(hipStreamSynchronize(c->js), "macro upload");
   G.macro_gen = c->tables_gen;
@@ -1149,12 +1187,13 @@ int mdx_add_grid(mdx_ctx* c, const uint8_t* ids, const int dims[3], const double
       g->origin[k] = origin[k];
       nv *= (size_t)dims[k];
     }
-    scan_ids(ids, nv, g->present);
     if (c->device >= 0) {  // straight from the caller's memory, which it may reuse on return
       set_device_job(c);
       g->d_ids.upload(ids, nv, c->js);
+      scan_cells(ids, dims, g->cell_ids, g->mc, g->present);  // while the copy runs
       hip_check(hipStreamSynchronize(c->js), "upload grid");
     } else {
+      scan_cells(ids, dims, g->cell_ids, g->mc, g->present);
       g->ids.assign(ids, ids + nv);
     }
     c->J().grids.push_back(std::move(g));
@@ -1192,7 +1231,7 @@ int mdx_load_scene(mdx_ctx* c, const char* json_path) {
         g->origin[k] = sc.grids[i].origin[k];
       }
       g->ids = std::move(grids[i].ids);
-      scan_ids(g->ids.data(), g->ids.size(), g->present);
+      scan_cells(g->ids.data(), g->dims, g->cell_ids, g->mc, g->present);
       if (c->device >= 0) {
         set_device_job(c);
         g->d_ids.upload(g->ids.data(), g->ids.size(), c->js);
@@ -1478,7 +1517,9 @@ int mdx_autotune(mdx_ctx* c, uint64_t probe_histories, int* blocks_per_cu, int* 
     int b = c->queue_blocks_per_cu, r = c->refill_min, i = c->interact_min;
     double cur = probe(b, r, i);
     // coordinate search: interaction batch, refill batch, both again, then
-    // occupancy (the kernel's registers allow four waves per SIMD)
+    // occupancy (the kernel's 128 registers allow four waves per SIMD, i.e.
+    // four blocks of 256 lanes per CU; three is the only smaller choice worth
+    // trying, more blocks than fit only queue behind the resident ones)
     for (int pass = 0; pass < 2; ++pass) {
       for (int v : {8, 16, 24, 32, 40, 48}) {
         if (v == i) continue;
@@ -1491,7 +1532,7 @@ int mdx_autotune(mdx_ctx* c, uint64_t probe_histories, int* blocks_per_cu, int* 
         if (t < cur) { cur = t; r = v; }
       }
     }
-    for (int v : {3, 4, 8}) {
+    for (int v : {3, 4}) {
       if (v == b) continue;
       double t = probe(v, r, i);
       if (t < cur) { cur = t; b = v; }

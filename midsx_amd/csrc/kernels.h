@@ -38,9 +38,6 @@ hipError_t kahn(const mdx_tables_dev* T, int slot, int n, const float* E, uint64
 hipError_t zero(double* p, size_t n, hipStream_t st);
 hipError_t zero_u32(uint32_t* p, size_t n, hipStream_t st);
 hipError_t zero_u64(unsigned long long* p, size_t n, hipStream_t st);
-// Bit masks of the material slots present per macro cell of a grid.
-hipError_t macro_mask(const uint8_t* ids, int nx, int ny, int nz, int mcx, int mcy, int mcz, const mdx_tables_dev* T,
-                      uint32_t* mask, hipStream_t st);
 // Clears the ranges listed in device memory as {count, (address, bytes) x
 // count}; addresses 16-byte aligned, sizes multiples of 4.
 hipError_t zero_list(const uint64_t* list, hipStream_t st);

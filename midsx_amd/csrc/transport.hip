@@ -1335,25 +1335,6 @@ __global__ void k_zero(double* p, size_t n) {
   if (i < n) p[i] = 0.0;
 }
 
-// The set of material slots present in each macro cell of a grid (bit s for
-// slot s), one thread per cell.
-__global__ void k_macro_mask(const uint8_t* __restrict__ ids, int nx, int ny, int nz, int mcx, int mcy, int mcz,
-                             const mdx_tables_dev* __restrict__ T, uint32_t* __restrict__ mask) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= mcx * mcy * mcz) return;
-  const int ci = c % mcx, cj = (c / mcx) % mcy, ck = c / (mcx * mcy);
-  const int i0 = ci * MDX_MACRO_EDGE, j0 = cj * MDX_MACRO_EDGE, k0 = ck * MDX_MACRO_EDGE;
-  const int i1 = min(i0 + MDX_MACRO_EDGE, nx), j1 = min(j0 + MDX_MACRO_EDGE, ny), k1 = min(k0 + MDX_MACRO_EDGE, nz);
-  uint32_t m = 0;
-  for (int k = k0; k < k1; ++k)
-    for (int j = j0; j < j1; ++j)
-      for (int i = i0; i < i1; ++i) {
-        const uint8_t slot = T->id_to_slot[ids[(size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k)]];
-        m |= slot < 32 ? (1u << slot) : 0x80000000u;
-      }
-  mask[c] = m;
-}
-
 // Clears the ranges of a list {count, (address, bytes) x count} read from
 // device memory (a round buffer), so that one captured launch serves any
 // list.  Addresses are 16-byte aligned and sizes multiples of 4 bytes.
@@ -1491,14 +1472,6 @@ hipError_t kahn(const mdx_tables_dev* T, int slot, int n, const float* E, uint64
 hipError_t zero(double* p, size_t n, hipStream_t st) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_zero, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, n);
-  return hipGetLastError();
-}
-
-hipError_t macro_mask(const uint8_t* ids, int nx, int ny, int nz, int mcx, int mcy, int mcz, const mdx_tables_dev* T,
-                      uint32_t* mask, hipStream_t st) {
-  const int n = mcx * mcy * mcz;
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_macro_mask, dim3((n + 255) / 256), dim3(256), 0, st, ids, nx, ny, nz, mcx, mcy, mcz, T, mask);
   return hipGetLastError();
 }
 
