@@ -177,7 +177,7 @@ def ct_scan(ctx, n_views, scale, seed=4):
     seeds = [10_000 + v for v in range(n_views)]
     H = sum(n)
     ctx.job_select(views[0])
-    tuned = ctx.autotune(probe_histories=1000000)
+    tuned = ctx.autotune(probe_histories=4000000)  # one view: launches of ~15 ms time stably
     ctx.set_graphs(True)
     ctx.run_jobs(views[:8], [100000] * 8, [1 << 40] * 8, seeds[:8])  # warm (capture)
     for j in [owner] + views:
