@@ -382,8 +382,9 @@ class Context:
         self._call(self.lib.mdx_set_local_majorants, self._h, ctypes.c_int(1 if on else 0))
 
     def macro_cells(self):
-        """The selected job's macro-cell table as (class, radius) arrays of
-        shape (cz, cy, cx), or None when it has none."""
+        """The selected job's macro-cell table as (class, radius, uniform)
+        arrays of shape (cz, cy, cx) (class 255: none; uniform: the cell holds
+        only its class material), or None when it has none."""
         cells = (ctypes.c_int * 3)()
         self._call(self.lib.mdx_get_macro_cells, self._h, cells, None)
         if cells[0] == 0:
@@ -391,7 +392,9 @@ class Context:
         w = np.zeros(cells[0] * cells[1] * cells[2], np.uint16)
         self._call(self.lib.mdx_get_macro_cells, self._h, cells, _ptr(w, ctypes.c_uint16))
         w = w.reshape(cells[2], cells[1], cells[0])
-        return (w & 255).astype(np.int32), (w >> 8).astype(np.int32)
+        low = (w & 255).astype(np.int32)
+        cls = np.where(low == 255, 255, low & 0x7F)
+        return cls, (w >> 8).astype(np.int32), (low != 255) & ((low & 0x80) != 0)
 
     def set_graphs(self, on=True):
         self._call(self.lib.mdx_set_graphs, self._h, ctypes.c_int(1 if on else 0))

@@ -593,6 +593,8 @@ void build_macro(mdx_ctx* c, GridHost& G) {
   for (int k = 0; k < ns; ++k)
     if (used[k]) th.emplace_back(radius, k);
   for (auto& t : th) t.join();
+  for (size_t i = 0; i < nc; ++i)  // one material in the cell: the voxel IDs need not be read
+    if (cls[i] != 255 && __builtin_popcount(mask[i]) == 1 && mask[i] == 1u << cls[i]) word[i] |= MDX_MACRO_UNIFORM;
   G.d_macro.upload(word.data(), nc, c->js);
   hip_check(hipStreamSynchronize(c->js), "macro upload");
   G.macro_gen = c->tables_gen;

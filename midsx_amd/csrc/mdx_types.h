@@ -8,6 +8,7 @@
 #define MDX_MAX_GRIDS 8
 #define MDX_MACRO_SHIFT 3  // macro cells of 8^3 voxels (local majorants)
 #define MDX_MACRO_EDGE (1 << MDX_MACRO_SHIFT)
+#define MDX_MACRO_UNIFORM 0x80u  // the macro cell holds one material, its class
 #define MDX_MAX_TALLIES 16
 
 // Tally quantity / bucket layout (mirrors include/MIDSX/Core/volume_quantity.h:13-28,
@@ -104,10 +105,12 @@ struct mdx_grid_dev {
   int32_t dose_mode;   // MDX_DOSE_*
   int32_t pad;
   // Local majorants (nullptr = off): per macro cell of MDX_MACRO_EDGE^3
-  // voxels, x fastest, the slot of a material whose total cross section is at
-  // least every cross section found within `radius` cells of it (Chebyshev
-  // distance, inside the grid), in the low byte, and that radius in the high
-  // byte; class 255 or radius 0: the global majorant applies.
+  // voxels, x fastest, a 16-bit word: bits 0-6 the slot of a material whose
+  // total cross section is at least every cross section found within
+  // `radius` cells of it (Chebyshev distance, inside the grid), bit 7 set when
+  // every voxel of the cell holds that material (MDX_MACRO_UNIFORM), bits
+  // 8-15 that radius; a low byte of 255 (no class) or radius 0: the global
+  // majorant applies.
   const uint16_t* macro;
   int32_t mcx, mcy;    // macro cells along x and y
   float mcell;         // smallest edge of a macro cell (cm)

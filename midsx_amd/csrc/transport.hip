@@ -913,7 +913,13 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
           g = voxel_lookup(D, px, py, pz, lin, &cell);
           const uint16_t* mc = D->grids[0].macro;
           if (g == 0 && mc) mword = gld(mc + cell);
-          slot = g < 0 ? D->bg_slot : (int)lslot[gld(D->grids[g].ids + lin)];
+          const uint32_t c8 = mword & 255u;
+          if (g < 0)
+            slot = D->bg_slot;
+          else if (c8 != 255u && (c8 & MDX_MACRO_UNIFORM))  // a one-material macro cell: its class
+            slot = (int)(c8 & 0x7Fu);
+          else
+            slot = (int)lslot[gld(D->grids[g].ids + lin)];
         }
       }
       if (!escaped && slot != cslot) {
@@ -978,7 +984,7 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
         fly = flights && g < 0;
       }
       if (!escaped && !flew) {  // the majorant for the step from the new position
-        int ncls = (int)(mword & 255u);
+        int ncls = (mword & 255u) == 255u ? 255 : (int)(mword & 0x7Fu);
         const uint32_t nr = mword >> 8;
         if (nr == 0u || E < 1.0e3f) ncls = 255;
         if (ncls != mcls) {

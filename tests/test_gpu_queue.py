@@ -300,13 +300,14 @@ def check_macro_table(ctx, ids, materials):
     class is a material present in it that bounds the total cross section of
     every material within its radius (host FP64 tables, 400 energies over
     1-500 keV), those cells lie inside the grid, and the radius cannot grow
-    by one (up to its cap of 255)."""
+    by one (up to its cap of 255); a cell is flagged uniform exactly when it
+    holds one material, which is then its class."""
     uniq = list(dict.fromkeys(materials))  # slot order of build_tables
     energies = np.geomspace(1e3, 5e5, 400)
     sig = np.array([ctx.interpolate_host(m, M.XS_TOTAL, energies) for m in uniq])
     bounds = np.all(sig[:, None, :] >= sig[None, :, :] * (1 + 1e-6), axis=2) | np.eye(len(uniq), dtype=bool)
     slot_of = {m: s for s, m in enumerate(uniq)}
-    cls, rad = ctx.macro_cells()
+    cls, rad, uni = ctx.macro_cells()
     cz, cy, cx = cls.shape
     e = 8
     present = np.zeros((cz, cy, cx, len(uniq)), bool)
@@ -320,6 +321,10 @@ def check_macro_table(ctx, ids, materials):
         for j in range(cy):
             for i in range(cx):
                 c, r = cls[k, j, i], rad[k, j, i]
+                only = np.nonzero(present[k, j, i])[0]
+                assert uni[k, j, i] == (len(only) == 1 and c != 255), (k, j, i)
+                if uni[k, j, i]:
+                    assert only[0] == c
                 if c == 255:
                     assert not any(bounds[s][present[k, j, i]].all() for s in np.nonzero(present[k, j, i])[0])
                     continue
