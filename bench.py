@@ -122,6 +122,16 @@ def served_stream(ctx, n_jobs, seed, load):
         cache[id(j)] = ids
     ph = lambda j: cache[id(j)]
     H = sum(j["histories"] for j in jobs)
+    # the stream's own launch parameters, tuned before timing on one job of
+    # it (a 256^3 phantom if there is one), as a server does at start-up
+    keep_params = ctx.queue_params()
+    keep_job = ctx.job
+    probe = next((j for j in jobs if j["shape"][0] == 256), jobs[0])
+    pj = ctx.job_create()
+    scenes.setup_dose_job(ctx, probe, ids=ph(probe))
+    tuned = ctx.autotune(probe_histories=4000000)
+    ctx.job_release(pj)
+    ctx.job_select(keep_job)
     srv = Server(ctx, phantoms=ph)
     t = time.perf_counter()
     srv.run(jobs)
@@ -142,6 +152,7 @@ def served_stream(ctx, n_jobs, seed, load):
 
     Server(ctx, phantoms=ph, on_result=keep).run(jobs)
     rse = [summaries[id(j)]["roi_median_rse"] for j in jobs]
+    ctx.set_queue_params(*keep_params)
     return {
         "workload": "%d dose jobs (seed %d): phantoms %s, histories 1e6-1e8 log-uniform, total %d" % (
             n_jobs, seed, sorted(set(j["shape"][0] for j in jobs)), H),
@@ -160,6 +171,7 @@ def served_stream(ctx, n_jobs, seed, load):
         "latency": {"arrival_rate_jobs_per_s": load * cap, "load": load, "p50_s": p50, "p99_s": p99,
                     "plain_fifo_p50_s": q50, "plain_fifo_p99_s": q99},
         "roi_median_rse": {"median_over_jobs": float(np.median(rse)), "max_over_jobs": float(np.max(rse))},
+        "queue_params": tuned,
     }
 
 

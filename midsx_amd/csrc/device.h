@@ -144,13 +144,20 @@ __device__ __forceinline__ T gadd(T* p, T v) {  // atomicAdd on global memory; r
   return __hip_atomic_fetch_add((MDX_GLOBAL T*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// The hardware reciprocal (v_rcp_f32, 1 ulp): quotients that feed sampling
+// and geometry, where a last-place difference from the IEEE quotient is
+// immaterial, are taken as a * frcp(b).
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // ------------------------------------------------------------------ curves
-// ln(X / xk) for X, xk > 0, with the rounding error of the quotient folded
-// back in (the quotient's exact residual comes from one fma).
+// ln(X / xk) for X, xk > 0: r approximates the quotient within a few ulp,
+// e = X - r xk is its residual (one fma), and ln(X / xk) = ln r + ln(1 + e /
+// (r xk)) = ln r + e / X to well below an ulp of the result, so the
+// approximate quotient costs no accuracy.
 __device__ __forceinline__ float log_ratio(float X, float xk) {
-  float r = X / xk;
-  float e = __builtin_fmaf(-r, xk, X);
-  return logf(r) + e / (r * xk);
+  const float r = X * frcp(xk);
+  const float e = __builtin_fmaf(-r, xk, X);
+  return logf(r) + e * frcp(X);
 }
 
 // A curve lookup is two dependent loads: the hash bin of ln X gives the
@@ -165,8 +172,11 @@ struct CurveAt {
   float xn;    // abscissa of node k + 1
 };
 
+// The bin comes from the hardware log2: a point within its error of a bin
+// edge may land in the neighbouring bin, whose node curve_fix then corrects.
 __device__ __forceinline__ int curve_hash(const mdx_tables_dev* T, const mdx_curve& c, float X) {
-  int b = (int)((logf(X) - c.lnx0) * c.inv_dlnx);
+  constexpr float kLn2 = 0.69314718055994531f;
+  int b = (int)((__log2f(X) * kLn2 - c.lnx0) * c.inv_dlnx);
   b = b < 0 ? 0 : (b >= c.nh ? c.nh - 1 : b);
   const int k = gld(T->hash + c.hoff + b);
   return k > c.n - 2 ? c.n - 2 : k;
@@ -302,8 +312,7 @@ __device__ __forceinline__ void rotate(float& dx, float& dy, float& dz, float mu
   nx *= inv; ny *= inv; nz *= inv;
   float cx = ny * dz - nz * dy, cy = nz * dx - nx * dz, cz = nx * dy - ny * dx;  // n x d
   float st = sqrtf(fmaxf(0.0f, 1.0f - mu * mu));
-  float sp, cp;
-  sincosf(phi, &sp, &cp);
+  const float sp = __sinf(phi), cp = __cosf(phi);  // hardware sine and cosine, phi in [0, 2 pi)
   float ox = mu * dx + st * (cp * cx + sp * nx);
   float oy = mu * dy + st * (cp * cy + sp * ny);
   float oz = mu * dz + st * (cp * cz + sp * nz);

@@ -269,8 +269,9 @@ constexpr uint32_t kMaxKahnTrials = 4096;
 __device__ float compton(const mdx_tables_dev* T, const mdx_material_dev& M, Philox& rng, float E, float& mu,
                          uint32_t& trials, uint32_t& s_tests) {
   const float k = E * kInvMc2;
-  const float r1p = (1.0f + 2.0f * k) / (9.0f + 2.0f * k);
-  const float s_max = curve_eval(T, M.sf, kAlphaSqrt2 * k);
+  const float ik = frcp(k), k2 = 1.0f + 2.0f * k;
+  const float r1p = k2 * frcp(9.0f + 2.0f * k);
+  const float is_max = frcp(curve_eval(T, M.sf, kAlphaSqrt2 * k));
   // Trial it draws (r1, r2, r3) for Kahn's test and, once that passes (or
   // the trial cap is reached), r4 for the S(x)/S_max test.  The Kahn draws
   // loop on their own so that a wave runs the table lookup of the S-test
@@ -283,20 +284,21 @@ __device__ float compton(const mdx_tables_dev* T, const mdx_material_dev& M, Phi
       const float r1 = rng.uniform(), r2 = rng.uniform(), r3 = rng.uniform();
       if (r1 <= r1p) {
         chi = 1.0f + 2.0f * k * r2;
-        if (r3 <= 4.0f * (1.0f / chi - 1.0f / (chi * chi)) || it >= kMaxKahnTrials) break;
+        const float ic = frcp(chi);
+        if (r3 <= 4.0f * (ic - ic * ic) || it >= kMaxKahnTrials) break;
       } else {
-        chi = (1.0f + 2.0f * k) / (1.0f + 2.0f * k * r2);
-        const float q = 1.0f / k - chi / k + 1.0f;
-        if (2.0f * r3 <= q * q + 1.0f / chi || it >= kMaxKahnTrials) break;
+        chi = k2 * frcp(1.0f + 2.0f * k * r2);
+        const float q = (1.0f - chi) * ik + 1.0f;
+        if (2.0f * r3 <= q * q + frcp(chi) || it >= kMaxKahnTrials) break;
       }
     }
     ++s_tests;
-    const float one_minus_mu = (chi - 1.0f) / k;  // 1 - mu, mu = 1 + 1/k - 1/k'
+    const float one_minus_mu = fminf((chi - 1.0f) * ik, 2.0f);  // 1 - mu, mu = 1 + 1/k - 1/k'
     const float x = kAlpha * k * sqrtf(one_minus_mu);
     const float sx = curve_eval(T, M.sf, x);
-    if (rng.uniform() <= sx / s_max || it >= kMaxKahnTrials) {
+    if (rng.uniform() <= sx * is_max || it >= kMaxKahnTrials) {
       mu = 1.0f - one_minus_mu;
-      return E / chi;
+      return E * frcp(chi);
     }
     ++it;
   }
@@ -577,8 +579,9 @@ __device__ float next_tally_event(const mdx_tallies_dev* TL, float x, float y, f
       float enter = -INFINITY, exit = INFINITY;
       for (int i = 0; i < 3; ++i) {
         if (fabsf(dir[i]) < kEps) continue;
-        float a = (T.p0[i] - pos[i]) / dir[i];
-        float b = (T.p1[i] - pos[i]) / dir[i];
+        const float id = frcp(dir[i]);
+        float a = (T.p0[i] - pos[i]) * id;
+        float b = (T.p1[i] - pos[i]) * id;
         if (a > b) { float tmp = a; a = b; b = tmp; }
         enter = fmaxf(enter, a);
         exit = fminf(exit, b);
@@ -588,7 +591,7 @@ __device__ float next_tally_event(const mdx_tallies_dev* TL, float x, float y, f
     } else {
       const float cosv = dx * T.nrm[0] + dy * T.nrm[1] + dz * T.nrm[2];
       if (fabsf(cosv) < kEps || cosv < 0.0f) continue;
-      const float tt = ((T.p0[0] - x) * T.nrm[0] + (T.p0[1] - y) * T.nrm[1] + (T.p0[2] - z) * T.nrm[2]) / cosv;
+      const float tt = ((T.p0[0] - x) * T.nrm[0] + (T.p0[1] - y) * T.nrm[1] + (T.p0[2] - z) * T.nrm[2]) * frcp(cosv);
       if (tt < -0.01f) continue;
       e = fminf(e, tt);
     }
@@ -605,7 +608,7 @@ __device__ __forceinline__ void flight_bounds(const mdx_domain_dev* D, float x, 
   float inv[3];
   dexit = INFINITY;
   for (int i = 0; i < 3; ++i) {
-    inv[i] = 1.0f / dir[i];
+    inv[i] = frcp(dir[i]);  // +-inf for a zero component, as 1 / 0
     if (dir[i] > 0.0f) dexit = fminf(dexit, (D->dim_space[i] - pos[i]) * inv[i]);
     else if (dir[i] < 0.0f) dexit = fminf(dexit, -pos[i] * inv[i]);
   }
@@ -866,7 +869,7 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
         smax = curve_eval(T, lmaj, E);
         scache[wave][kSigmaCache - 1][lane] = smax;
         cvalid = 1u << (kSigmaCache - 1);
-        step_scale = 0.69314718055994531f / smax;
+        step_scale = 0.69314718055994531f * frcp(smax);
         mcls = 255;
         dsafe = INFINITY;
         cslot = -1;
@@ -943,7 +946,7 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
       } else if (fly) {
         // background free flight: distance at the background's own cross
         // section, cut at the next grid face or the domain's boundary
-        const float si = -__log2f(u1) * (0.69314718055994531f / stot);
+        const float si = -__log2f(u1) * (0.69314718055994531f * frcp(stot));
         float dgrid, dexit;
         flight_bounds(D, x0, y0, z0, dx, dy, dz, dgrid, dexit);
         if (si < dgrid && si < dexit) {  // real interaction in the background
@@ -999,7 +1002,7 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
               cvalid |= 1u << ce;
             }
           }
-          step_scale = 0.69314718055994531f / smax;
+          step_scale = 0.69314718055994531f * frcp(smax);
           thr = 1.0f - stot / smax;
         }
         dsafe = ncls == 255 ? INFINITY : (float)nr * D->grids[0].mcell;
@@ -1049,7 +1052,7 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
               cvalid = 1u << ce;
             }
           }
-          step_scale = 0.69314718055994531f / smax;
+          step_scale = 0.69314718055994531f * frcp(smax);
           cslot = -1;
         } else {
           ++c_photo;
