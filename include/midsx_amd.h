@@ -239,7 +239,8 @@ int mdx_set_flights(mdx_ctx* ctx, int enable);
  * The grid is cut into macro cells of 8 x 8 x 8 voxels; each cell
  * records the material that bounds the total cross section of every
  * material in it at all energies from 1 to 500 keV, and the radius (in
- * cells) around it within which that still holds and the grid continues.
+ * cells) around it within which that still holds, and the grid continues
+ * unless that material also bounds the background (outside the grid).
  * Inside that radius the Woodcock step samples with the bounding material's
  * cross section instead of the global majorant, and a step that would leave
  * the radius stops at its edge without a collision test (memoryless

@@ -515,18 +515,19 @@ std::vector<uint32_t> dominance(const std::vector<std::unique_ptr<mdx::MaterialT
   return dom;
 }
 
-// Chebyshev distance (in cells, capped at 255) from every cell to the nearest
-// bad cell or to the layer of cells around the grid: the two-pass chamfer
-// transform with unit weights on the 26-neighbourhood, which is exact for
-// this metric, on an array padded with that layer.
-std::vector<uint8_t> cheb_distance(const std::vector<uint8_t>& bad, int nx, int ny, int nz) {
+// Chebyshev distance (in cells; 65535 = none) from every cell to the nearest
+// bad cell, and to the layer of cells around the grid when outside_bad: the
+// two-pass chamfer transform with unit weights on the 26-neighbourhood, which
+// is exact for this metric, on an array padded with that layer.
+std::vector<uint16_t> cheb_distance(const std::vector<uint8_t>& bad, int nx, int ny, int nz, bool outside_bad) {
+  constexpr uint16_t kNone = 65535;
   const int px = nx + 2, py = ny + 2, pz = nz + 2;
   const size_t sy = (size_t)px, sz = (size_t)px * py;
-  std::vector<uint8_t> d((size_t)px * py * pz, 0);
+  std::vector<uint16_t> d((size_t)px * py * pz, outside_bad ? 0 : kNone);
   for (int k = 0; k < nz; ++k)
     for (int j = 0; j < ny; ++j)
       for (int i = 0; i < nx; ++i)
-        d[(i + 1) + sy * (j + 1) + sz * (k + 1)] = bad[(size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k)] ? 0 : 255;
+        d[(i + 1) + sy * (j + 1) + sz * (k + 1)] = bad[(size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k)] ? 0 : kNone;
   std::vector<ptrdiff_t> back;  // the 13 neighbours before a cell in raster order
   for (int dk = -1; dk <= 1; ++dk)
     for (int dj = -1; dj <= 1; ++dj)
@@ -538,7 +539,7 @@ std::vector<uint8_t> cheb_distance(const std::vector<uint8_t>& bad, int nx, int 
         const size_t c = i + sy * j + sz * k;
         int v = d[c];
         for (ptrdiff_t o : back) v = std::min(v, d[c + o] + 1);
-        d[c] = (uint8_t)v;
+        d[c] = (uint16_t)std::min(v, (int)kNone);
       }
   for (int k = nz; k >= 1; --k)
     for (int j = ny; j >= 1; --j)
@@ -546,9 +547,9 @@ std::vector<uint8_t> cheb_distance(const std::vector<uint8_t>& bad, int nx, int 
         const size_t c = i + sy * j + sz * k;
         int v = d[c];
         for (ptrdiff_t o : back) v = std::min(v, d[c - o] + 1);
-        d[c] = (uint8_t)v;
+        d[c] = (uint16_t)std::min(v, (int)kNone);
       }
-  std::vector<uint8_t> out((size_t)nx * ny * nz);
+  std::vector<uint16_t> out((size_t)nx * ny * nz);
   for (int k = 0; k < nz; ++k)
     for (int j = 0; j < ny; ++j)
       for (int i = 0; i < nx; ++i) out[(size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k)] = d[(i + 1) + sy * (j + 1) + sz * (k + 1)];
@@ -558,8 +559,8 @@ std::vector<uint8_t> cheb_distance(const std::vector<uint8_t>& bad, int nx, int 
 // The macro-cell table of grid G (one grid per domain): per cell the slot of
 // the material that bounds every material present in the cell (the one with
 // the smallest cross sections among those that do), and the Chebyshev radius
-// in cells within which every cell holds only materials it bounds and lies
-// inside the grid.
+// in cells within which every cell holds only materials it bounds and, unless
+// it bounds the domain's background material, lies inside the grid.
 void build_macro(mdx_ctx* c, GridHost& G) {
   const size_t nc = (size_t)G.mc[0] * G.mc[1] * G.mc[2];
   std::vector<uint32_t> mask(nc, 0);
@@ -582,12 +583,17 @@ void build_macro(mdx_ctx* c, GridHost& G) {
   std::vector<bool> used(ns, false);
   for (size_t i = 0; i < nc; ++i)
     if (cls[i] != 255) used[cls[i]] = true;
+  // Outside the (only) grid lies the background material, or the outside of
+  // the domain, where a photon escapes: a class that bounds the background
+  // is not limited by the grid's faces.
+  const int bg = c->slot_of(c->J().background_id);
   auto radius = [&](int k) {  // cells of class k only: one writer per entry
     std::vector<uint8_t> bad(nc);
     for (size_t i = 0; i < nc; ++i) bad[i] = (mask[i] & ~dom[k]) != 0;
-    const std::vector<uint8_t> d = cheb_distance(bad, G.mc[0], G.mc[1], G.mc[2]);
-    for (size_t i = 0; i < nc; ++i)
-      if (cls[i] == k) word[i] = (uint16_t)(k | (std::max((int)d[i] - 1, 0) << 8));
+    const bool outside_bad = bg >= 32 || !(dom[k] >> bg & 1u);
+    const std::vector<uint16_t> d = cheb_distance(bad, G.mc[0], G.mc[1], G.mc[2], outside_bad);
+    for (size_t i = 0; i < nc; ++i)  // radius = distance - 1, capped at 255
+      if (cls[i] == k) word[i] = (uint16_t)(k | (std::min(std::max((int)d[i] - 1, 0), 255) << 8));
   };
   std::vector<std::thread> th;
   for (int k = 0; k < ns; ++k)

@@ -107,7 +107,8 @@ struct mdx_grid_dev {
   // Local majorants (nullptr = off): per macro cell of MDX_MACRO_EDGE^3
   // voxels, x fastest, a 16-bit word: bits 0-6 the slot of a material whose
   // total cross section is at least every cross section found within
-  // `radius` cells of it (Chebyshev distance, inside the grid), bit 7 set when
+  // `radius` cells of it (Chebyshev distance; outside the grid the background
+  // material), bit 7 set when
   // every voxel of the cell holds that material (MDX_MACRO_UNIFORM), bits
   // 8-15 that radius; a low byte of 255 (no class) or radius 0: the global
   // majorant applies.

@@ -295,13 +295,13 @@ def test_setup_during_round_clears_reused_blocks():
     ctx.close()
 
 
-def check_macro_table(ctx, ids, materials):
+def check_macro_table(ctx, ids, materials, background):
     """The macro-cell table against a brute-force restatement: each cell's
     class is a material present in it that bounds the total cross section of
     every material within its radius (host FP64 tables, 400 energies over
-    1-500 keV), those cells lie inside the grid, and the radius cannot grow
-    by one (up to its cap of 255); a cell is flagged uniform exactly when it
-    holds one material, which is then its class."""
+    1-500 keV), cells outside the grid hold the background material, and the
+    radius cannot grow by one (up to its cap of 255); a cell is flagged
+    uniform exactly when it holds one material, which is then its class."""
     uniq = list(dict.fromkeys(materials))  # slot order of build_tables
     energies = np.geomspace(1e3, 5e5, 400)
     sig = np.array([ctx.interpolate_host(m, M.XS_TOTAL, energies) for m in uniq])
@@ -309,6 +309,7 @@ def check_macro_table(ctx, ids, materials):
     slot_of = {m: s for s, m in enumerate(uniq)}
     cls, rad, uni = ctx.macro_cells()
     cz, cy, cx = cls.shape
+    dims = np.array([cz, cy, cx])
     e = 8
     present = np.zeros((cz, cy, cx, len(uniq)), bool)
     for k in range(cz):
@@ -316,32 +317,36 @@ def check_macro_table(ctx, ids, materials):
             for i in range(cx):
                 for m in np.unique(ids[i * e:(i + 1) * e, j * e:(j + 1) * e, k * e:(k + 1) * e]):
                     present[k, j, i, slot_of[int(m)]] = True
+
+    def held(lo, hi):  # materials in the cells [lo, hi), the background outside the grid
+        a, b = np.maximum(lo, 0), np.minimum(hi, dims)
+        m = present[a[0]:b[0], a[1]:b[1], a[2]:b[2]].reshape(-1, len(uniq)).any(axis=0)
+        if (lo < 0).any() or (hi > dims).any():
+            m[slot_of[background]] = True
+        return m
+
     local = 0
     for k in range(cz):
         for j in range(cy):
             for i in range(cx):
                 c, r = cls[k, j, i], rad[k, j, i]
-                only = np.nonzero(present[k, j, i])[0]
+                here = present[k, j, i]
+                only = np.nonzero(here)[0]
                 assert uni[k, j, i] == (len(only) == 1 and c != 255), (k, j, i)
                 if uni[k, j, i]:
                     assert only[0] == c
                 if c == 255:
-                    assert not any(bounds[s][present[k, j, i]].all() for s in np.nonzero(present[k, j, i])[0])
+                    assert not any(bounds[s][here].all() for s in only)
                     continue
-                assert present[k, j, i, c] and bounds[c][present[k, j, i]].all()
+                assert here[c] and bounds[c][here].all()
                 if r == 0:
                     continue
                 local += 1
                 lo = np.array([k, j, i]) - r
                 hi = np.array([k, j, i]) + r + 1
-                assert (lo >= 0).all() and (hi <= np.array([cz, cy, cx])).all(), (k, j, i, r)
-                box = present[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]].reshape(-1, len(uniq))
-                assert bounds[c][box.any(axis=0)].all(), (k, j, i, r)
-                if r < 255:  # one more cell would leave the grid or hold a material c does not bound
-                    lo1, hi1 = lo - 1, hi + 1
-                    if (lo1 >= 0).all() and (hi1 <= np.array([cz, cy, cx])).all():
-                        box1 = present[lo1[0]:hi1[0], lo1[1]:hi1[1], lo1[2]:hi1[2]].reshape(-1, len(uniq))
-                        assert not bounds[c][box1.any(axis=0)].all(), (k, j, i, r)
+                assert bounds[c][held(lo, hi)].all(), (k, j, i, r)
+                if r < 255:  # one more cell would hold a material c does not bound
+                    assert not bounds[c][held(lo - 1, hi + 1)].all(), (k, j, i, r)
     return local
 
 
@@ -358,7 +363,7 @@ def test_local_majorants_match_global_majorant():
     shape = (64, 64, 48)
     scenes.setup_radiography(ctx, shape=shape, voxel_cm=0.5, seed=4, image=(16, 16))
     ids = scenes.ellipsoid_phantom(shape, 4, voxel_cm=0.5)
-    assert check_macro_table(ctx, ids, materials) > 10
+    assert check_macro_table(ctx, ids, materials, scenes.AIR) > 10
     k, n = 40, 40000
 
     def batches(seed):
