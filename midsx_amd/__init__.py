@@ -120,6 +120,10 @@ class Context:
             self._call(self.lib.mdx_set_data_dir, self._h, data_dir.encode())
         self._job = 0
         self._job_tallies = {0: []}
+        # free pinned host buffers by element count, shared by the request
+        # loops run on this context (serve.Server), so that a long-running
+        # service pins its result buffers once
+        self.pinned_pool = {}
 
     @property
     def tallies(self):

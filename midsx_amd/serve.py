@@ -61,11 +61,11 @@ def roi_summary(job, mean, rel, image, stride=4):
 
 
 def read_dose(ctx, job, bufs, dose):
-    """The selected job's voxel dose into a reusable (pinned) buffer: dose
-    "f32" gives (mean per history, relative standard error) as float32,
-    "f64" the (sum, sum^2) of the deposits as float64."""
+    """The selected job's voxel dose into a reusable (pinned) buffer kept in
+    bufs: dose "f32" gives (mean per history, relative standard error) as
+    float32, "f64" the (sum, sum^2) of the deposits as float64."""
     nv = int(np.prod(job["shape"]))
-    key = (nv, dose)
+    key = ("read", nv, dose)
     if key not in bufs:
         dt = np.float32 if dose == "f32" else np.float64
         try:
@@ -120,8 +120,8 @@ class Server:
         self.on_result = on_result
         self.dose = dose
         self.admit_slice = float(admit_slice)
-        self._buf = {}
-        self._free = {}  # pinned float32 buffers by size, for copies in flight
+        self._buf = ctx.pinned_pool  # f64 read buffers by size, kept by the context
+        self._free = ctx.pinned_pool  # free pinned float32 buffers by size, kept by the context
         self.rounds = 0
         self.kernel_ms = 0.0
         self.counters = {}
@@ -291,7 +291,7 @@ def run_plain(ctx, jobs, arrivals=None, phantoms=None, on_result=None, dose="f32
     n = len(jobs)
     arrivals = np.zeros(n) if arrivals is None else np.asarray(arrivals, dtype=float)
     results = [JobResult(i, float(arrivals[i])) for i in range(n)]
-    bufs = {}
+    bufs = ctx.pinned_pool  # read buffers by size, kept by the context
     counters = {}
     kernel_ms = 0.0
     host = dict(admit=0.0, run=0.0, read=0.0, release=0.0)
@@ -326,6 +326,19 @@ def run_plain(ctx, jobs, arrivals=None, phantoms=None, on_result=None, dose="f32
         host["release"] += te - td
     makespan = time.perf_counter() - t0
     return results, dict(makespan=makespan, kernel_ms=kernel_ms, counters=counters, host_s=host)
+
+
+def warm_read_buffers(ctx, jobs, dose="f32"):
+    """Pins run_plain's read buffers for the jobs' phantom sizes ahead of
+    time, as a service that has been running holds them."""
+    dt = np.float32 if dose == "f32" else np.float64
+    for nv in sorted(set(int(np.prod(j["shape"])) for j in jobs)):
+        key = ("read", nv, dose)
+        if key not in ctx.pinned_pool:
+            try:
+                ctx.pinned_pool[key] = ctx.pinned(2 * nv, dt)
+            except Exception:
+                ctx.pinned_pool[key] = np.empty(2 * nv, dt)
 
 
 def poisson_arrivals(n, rate, seed):

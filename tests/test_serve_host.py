@@ -20,6 +20,7 @@ class FakeCtx:
         self.peak = 0
         self.bytes = {}
         self.admit_order = []
+        self.pinned_pool = {}
 
     # calls the server makes
     def set_graphs(self, on):
