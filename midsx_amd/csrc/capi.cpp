@@ -133,29 +133,27 @@ void scan_cells(const uint8_t* ids, const int dims[3], std::vector<uint64_t>& ce
   cells.assign(nc * 4, 0);
   const int nx = dims[0], ny = dims[1], nz = dims[2];
   const int nth = std::max(1, std::min({16, (int)std::thread::hardware_concurrency(), mc[2]}));
-  auto work = [&](int t) {
-    for (int ck = t; ck < mc[2]; ck += nth)
-      for (int cj = 0; cj < mc[1]; ++cj)
-        for (int ci = 0; ci < mc[0]; ++ci) {
-          uint64_t* m = &cells[4 * ((size_t)ci + (size_t)mc[0] * ((size_t)cj + (size_t)mc[1] * ck))];
-          const int i0 = ci * MDX_MACRO_EDGE, w = std::min(MDX_MACRO_EDGE, nx - i0);
-          const int j1 = std::min(cj * MDX_MACRO_EDGE + MDX_MACRO_EDGE, ny);
-          const int k1 = std::min(ck * MDX_MACRO_EDGE + MDX_MACRO_EDGE, nz);
-          for (int k = ck * MDX_MACRO_EDGE; k < k1; ++k)
-            for (int j = cj * MDX_MACRO_EDGE; j < j1; ++j) {
-              const uint8_t* row = ids + i0 + (size_t)nx * ((size_t)j + (size_t)ny * k);
-              if (w == 8) {
-                uint64_t v;
-                std::memcpy(&v, row, 8);
-                const uint64_t b = v & 255u;
-                if (v == b * 0x0101010101010101ull) {
-                  m[b >> 6] |= 1ull << (b & 63);
-                  continue;
-                }
-              }
-              for (int i = 0; i < w; ++i) m[row[i] >> 6] |= 1ull << (row[i] & 63);
+  auto work = [&](int t) {  // a thread takes layers of cells along z and reads their voxel rows in memory order
+    for (int ck = t; ck < mc[2]; ck += nth) {
+      const int k1 = std::min(ck * MDX_MACRO_EDGE + MDX_MACRO_EDGE, nz);
+      for (int k = ck * MDX_MACRO_EDGE; k < k1; ++k)
+        for (int j = 0; j < ny; ++j) {
+          const uint8_t* row = ids + (size_t)nx * ((size_t)j + (size_t)ny * k);
+          uint64_t* m = &cells[4 * (size_t)mc[0] * ((size_t)(j >> MDX_MACRO_SHIFT) + (size_t)mc[1] * ck)];
+          int i = 0;
+          for (; i + MDX_MACRO_EDGE <= nx; i += MDX_MACRO_EDGE, m += 4) {
+            uint64_t v;
+            std::memcpy(&v, row + i, 8);
+            const uint64_t b = v & 255u;
+            if (v == b * 0x0101010101010101ull) {
+              m[b >> 6] |= 1ull << (b & 63);
+            } else {
+              for (int q = 0; q < 8; ++q) m[row[i + q] >> 6] |= 1ull << (row[i + q] & 63);
             }
+          }
+          for (; i < nx; ++i) m[row[i] >> 6] |= 1ull << (row[i] & 63);
         }
+    }
   };
   std::vector<std::thread> th;
   for (int t = 1; t < nth; ++t) th.emplace_back(work, t);
