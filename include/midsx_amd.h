@@ -192,7 +192,8 @@ int mdx_run(mdx_ctx* ctx, uint64_t n_histories, uint64_t seed, uint64_t first_hi
 /* Launch parameters of the queue kernel: workgroups per compute unit, and
  * the number of lanes of a wave that must be free (resp. parked at a real
  * interaction) before the wave claims new histories (resp. runs the parked
- * interactions).  They change speed, not results; the defaults are 4, 32,
+ * interactions, which it also does once as many lanes are parked as can
+ * still step).  They change speed, not results; the defaults are 4, 32,
  * 32.  No reference counterpart
  * (the reference's only parallel knob is the OpenMP thread count). */
 int mdx_set_queue_params(mdx_ctx* ctx, int blocks_per_cu, int refill_min, int interact_min);

@@ -1016,7 +1016,9 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
 
     // ---- interaction phase: the parked lanes together (physics_engine.cpp:62-78)
     const unsigned long long pmask = __ballot(pend);
-    if (pmask && (__popcll(pmask) >= interact_min || __ballot(alive && !pend) == 0ull)) {
+    // once interact_min lanes are parked, or at least as many as can still
+    // step (their photons would otherwise wait on a thinning step phase)
+    if (pmask && (__popcll(pmask) >= interact_min || __popcll(pmask) >= __popcll(__ballot(alive && !pend)))) {
       if (pend) {
         pend = false;
         ++c_real;
