@@ -253,7 +253,8 @@ int mdx_set_local_majorants(mdx_ctx* ctx, int enable);
  * is none; out: class | uniform << 7 | radius << 8 per cell, x fastest, class
  * 0-126 or a low byte of 255 for none, uniform = every voxel of the cell holds
  * the class material, whose voxel IDs the kernel then does not read; may be
- * NULL). */
+ * NULL).  A device context builds the table when the domain is uploaded, a
+ * host-only context on this call. */
 int mdx_get_macro_cells(mdx_ctx* ctx, int cells[3], uint16_t* out);
 /* Where mdx_build_tables builds the Rayleigh RITA tables (FP64 either way):
  * on the device, one wavefront per energy (default for device contexts), or

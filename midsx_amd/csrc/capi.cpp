@@ -185,8 +185,9 @@ struct GridHost {
   bool present[256] = {false};  // material IDs the grid holds
   DBuf d_ids, d_dose, d_dose_n;  // d_dose holds the sums in the job's MDX_DOSE_* layout
   int dose_mode = -1;            // layout d_dose was set up for
-  DBuf d_macro;                  // local majorants per macro cell (class | radius << 8)
-  int macro_gen = -1;            // tables generation d_macro was built for
+  std::vector<uint16_t> macro;   // local majorants per macro cell (see mdx_grid_dev::macro)
+  DBuf d_macro;                  // its device copy
+  int macro_gen = -1;            // tables generation the table was built for
   int mc[3] = {0, 0, 0};         // macro cells per axis
   std::vector<uint64_t> cell_ids;  // material IDs present per macro cell (256 bits each)
   explicit GridHost(Pool* pl) : d_ids(pl), d_dose(pl), d_dose_n(pl), d_macro(pl) {}
@@ -261,7 +262,7 @@ struct mdx_ctx {
   mdx::Packed packed;
   DBuf d_xs, d_nodes, d_hash, d_rE, d_rinv, d_roff, d_rrows, d_rhash, d_rguide, d_tables;
   bool tables_ready = false;
-  int tables_gen = 0;  // bumped by every table upload
+  int tables_gen = 0;  // bumped by every table build
   // per slot k: the slots whose total cross section k's bounds from above on
   // [kMacroElo, kMacroEhi] (local majorants)
   std::vector<uint32_t> dominated;
@@ -559,7 +560,7 @@ std::vector<uint16_t> cheb_distance(const std::vector<uint8_t>& bad, int nx, int
 // the smallest cross sections among those that do), and the Chebyshev radius
 // in cells within which every cell holds only materials it bounds and, unless
 // it bounds the domain's background material, lies inside the grid.
-void build_macro(mdx_ctx* c, GridHost& G) {
+void build_macro(mdx_ctx* c, const Job& J, GridHost& G) {
   const size_t nc = (size_t)G.mc[0] * G.mc[1] * G.mc[2];
   std::vector<uint32_t> mask(nc, 0);
   for (size_t i = 0; i < nc; ++i)
@@ -584,7 +585,7 @@ void build_macro(mdx_ctx* c, GridHost& G) {
   // Outside the (only) grid lies the background material, or the outside of
   // the domain, where a photon escapes: a class that bounds the background
   // is not limited by the grid's faces.
-  const int bg = c->slot_of(c->J().background_id);
+  const int bg = c->slot_of(J.background_id);
   auto radius = [&](int k) {  // cells of class k only: one writer per entry
     std::vector<uint8_t> bad(nc);
     for (size_t i = 0; i < nc; ++i) bad[i] = (mask[i] & ~dom[k]) != 0;
@@ -599,9 +600,12 @@ void build_macro(mdx_ctx* c, GridHost& G) {
   for (auto& t : th) t.join();
   for (size_t i = 0; i < nc; ++i)  // one material in the cell: the voxel IDs need not be read
     if (cls[i] != 255 && __builtin_popcount(mask[i]) == 1 && mask[i] == 1u << cls[i]) word[i] |= MDX_MACRO_UNIFORM;
-  G.d_macro.upload(word.data(), nc, c->js);
-  hip_check(hipStreamSynchronize(c->js), "macro upload");
+  G.macro = std::move(word);
   G.macro_gen = c->tables_gen;
+  if (c->device >= 0) {
+    G.d_macro.upload(G.macro.data(), nc, c->js);
+    hip_check(hipStreamSynchronize(c->js), "macro upload");
+  }
 }
 
 void upload_domain(mdx_ctx* c) {
@@ -623,7 +627,7 @@ void upload_domain(mdx_ctx* c) {
     d.ids = G.d_ids.as<uint8_t>();
     d.macro = nullptr;
     if (c->tables_ready && c->local_majorants && c->J().grids.size() == 1 && !c->dominated.empty()) {
-      if (G.macro_gen != c->tables_gen) build_macro(c, G);
+      if (G.macro_gen != c->tables_gen) build_macro(c, c->J(), G);
       d.macro = G.d_macro.as<uint16_t>();
       d.mcx = G.mc[0];
       d.mcy = G.mc[1];
@@ -739,6 +743,8 @@ void build_and_upload(mdx_ctx* c, const std::vector<int>& ids) {
   c->majorant = mdx::build_majorant(mp);
   mdx::pack_all(c->packed, mp, c->majorant, 1.0e3, 5.0e5);
   c->mats = std::move(mats);
+  c->dominated = dominance(c->mats);
+  ++c->tables_gen;
   if (c->device < 0) {  // host-only context: FP64 tables and packing, no upload
     c->tables_ready = true;
     return;
@@ -771,8 +777,6 @@ void build_and_upload(mdx_ctx* c, const std::vector<int>& ids) {
   c->d_tables.upload(&T, 1, c->stream);
   hip_check(hipStreamSynchronize(c->stream), "upload tables");
   c->tables_ready = true;
-  ++c->tables_gen;
-  c->dominated = dominance(c->mats);
   const int keep = c->cur;
   for (size_t j = 0; j < c->jobs.size(); ++j) {
     if (!c->jobs[j] || c->jobs[j]->domain_of >= 0) continue;
@@ -1643,13 +1647,14 @@ int mdx_get_macro_cells(mdx_ctx* c, int cells[3], uint16_t* out) {
     need(c && cells, "NULL argument");
     Job& D = c->DJ();
     cells[0] = cells[1] = cells[2] = 0;
-    if (D.grids.size() != 1 || !D.grids[0]->d_macro.p || !c->local_majorants) return;
+    if (D.grids.size() != 1 || !c->local_majorants || !c->tables_ready || c->dominated.empty()) return;
     GridHost& g = *D.grids[0];
-    for (int a = 0; a < 3; ++a) cells[a] = g.mc[a];
-    if (out) {
-      set_device_nowait(c);
-      read_back(c, out, g.d_macro.p, g.d_macro.bytes, "read macro cells");
+    if (g.macro_gen != c->tables_gen) {
+      if (c->device >= 0) return;  // a device context builds it with the domain's upload
+      build_macro(c, D, g);        // a host-only context builds it on request
     }
+    for (int a = 0; a < 3; ++a) cells[a] = g.mc[a];
+    if (out) std::memcpy(out, g.macro.data(), g.macro.size() * sizeof(uint16_t));
   });
 }
 

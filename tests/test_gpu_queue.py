@@ -12,7 +12,7 @@ import pytest
 import midsx_amd as M
 import oracle
 from midsx_amd import scenes
-from helpers import batch_mean_sigma, hotelling, welch_normal_scores
+from helpers import batch_mean_sigma, check_macro_table, hotelling, welch_normal_scores
 from test_gpu_parity import orc_batches, slab_extract
 
 pytestmark = pytest.mark.gpu
@@ -293,61 +293,6 @@ def test_setup_during_round_clears_reused_blocks():
         assert np.allclose(m, a, rtol=1e-12, atol=1e-12 * np.abs(a).max()), np.max(np.abs(m - a))
     assert np.allclose(got[3], want[3], rtol=1e-12, atol=0)
     ctx.close()
-
-
-def check_macro_table(ctx, ids, materials, background):
-    """The macro-cell table against a brute-force restatement: each cell's
-    class is a material present in it that bounds the total cross section of
-    every material within its radius (host FP64 tables, 400 energies over
-    1-500 keV), cells outside the grid hold the background material, and the
-    radius cannot grow by one (up to its cap of 255); a cell is flagged
-    uniform exactly when it holds one material, which is then its class."""
-    uniq = list(dict.fromkeys(materials))  # slot order of build_tables
-    energies = np.geomspace(1e3, 5e5, 400)
-    sig = np.array([ctx.interpolate_host(m, M.XS_TOTAL, energies) for m in uniq])
-    bounds = np.all(sig[:, None, :] >= sig[None, :, :] * (1 + 1e-6), axis=2) | np.eye(len(uniq), dtype=bool)
-    slot_of = {m: s for s, m in enumerate(uniq)}
-    cls, rad, uni = ctx.macro_cells()
-    cz, cy, cx = cls.shape
-    dims = np.array([cz, cy, cx])
-    e = 8
-    present = np.zeros((cz, cy, cx, len(uniq)), bool)
-    for k in range(cz):
-        for j in range(cy):
-            for i in range(cx):
-                for m in np.unique(ids[i * e:(i + 1) * e, j * e:(j + 1) * e, k * e:(k + 1) * e]):
-                    present[k, j, i, slot_of[int(m)]] = True
-
-    def held(lo, hi):  # materials in the cells [lo, hi), the background outside the grid
-        a, b = np.maximum(lo, 0), np.minimum(hi, dims)
-        m = present[a[0]:b[0], a[1]:b[1], a[2]:b[2]].reshape(-1, len(uniq)).any(axis=0)
-        if (lo < 0).any() or (hi > dims).any():
-            m[slot_of[background]] = True
-        return m
-
-    local = 0
-    for k in range(cz):
-        for j in range(cy):
-            for i in range(cx):
-                c, r = cls[k, j, i], rad[k, j, i]
-                here = present[k, j, i]
-                only = np.nonzero(here)[0]
-                assert uni[k, j, i] == (len(only) == 1 and c != 255), (k, j, i)
-                if uni[k, j, i]:
-                    assert only[0] == c
-                if c == 255:
-                    assert not any(bounds[s][here].all() for s in only)
-                    continue
-                assert here[c] and bounds[c][here].all()
-                if r == 0:
-                    continue
-                local += 1
-                lo = np.array([k, j, i]) - r
-                hi = np.array([k, j, i]) + r + 1
-                assert bounds[c][held(lo, hi)].all(), (k, j, i, r)
-                if r < 255:  # one more cell would hold a material c does not bound
-                    assert not bounds[c][held(lo - 1, hi + 1)].all(), (k, j, i, r)
-    return local
 
 
 def test_local_majorants_match_global_majorant():

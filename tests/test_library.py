@@ -204,3 +204,28 @@ def test_air_muen_table_and_air_kerma_arithmetic():
     ak = derived.primary_spectrum_air_kerma(Fake(), 0, energies)
     assert abs(ak - 10.25e3 * 1000.0 / np.pi * c.air_muen([10.25e3])[0]) < 1e-9 * ak
     c.close()
+
+
+def test_macro_table_host():
+    """The macro-cell table (local majorants) built on a host-only context
+    against the brute-force restatement the GPU tests also apply: class,
+    uniform flag and safe radius of every cell of a radiography phantom in
+    air, whose surface cells' class (water) bounds the background and so
+    reaches past the grid's faces."""
+    from helpers import check_macro_table
+    from midsx_amd import scenes
+    c = M.Context(-1)
+    materials = list(scenes.C2_MATERIALS)
+    c.build_tables(materials)
+    shape = (64, 64, 48)
+    scenes.setup_radiography(c, shape=shape, voxel_cm=0.5, seed=4, image=(16, 16))
+    ids = scenes.ellipsoid_phantom(shape, 4, voxel_cm=0.5)
+    assert check_macro_table(c, ids, materials, scenes.AIR) > 10
+    cls, rad, uni = c.macro_cells()
+    assert uni.mean() > 0.5  # most cells hold water only
+    water = materials.index(scenes.WATER)
+    corner = (cls[0, 0, 0], rad[0, 0, 0], uni[0, 0, 0])
+    assert corner[0] == water and corner[2] and corner[1] > 0, corner  # not stopped by the grid's faces
+    c.set_local_majorants(False)
+    assert c.macro_cells() is None
+    c.close()
