@@ -1412,7 +1412,7 @@ int mdx_set_tallies(mdx_ctx* c, const mdx_tally* t, int n) {
       }
       if (q.kind != MDX_TALLY_AACUBOID && q.hist_bins > 0) need(q.hist_width > 0, "histogram width must be positive");
       if (dev) {
-        h->acc.alloc(MDX_TALLY_ACC_SIZE * sizeof(double));
+        h->acc.alloc((size_t)MDX_ACC_COPIES * MDX_TALLY_ACC_SIZE * sizeof(double));
         if (q.kind != MDX_TALLY_AACUBOID && q.hist_bins > 0) h->hist.alloc((size_t)4 * q.hist_bins * sizeof(double));
         if (q.kind == MDX_TALLY_IMAGE) h->image.alloc((size_t)3 * q.nx * q.ny * sizeof(double));
       }
@@ -1793,7 +1793,15 @@ int mdx_get_tally(mdx_ctx* c, int i, double* acc, double* hist, double* image) {
       hipStream_t s;
       ~Restore() { c->io_stream = s; }
     } restore{c, keep};
-    if (acc) read_back(c, acc, h.acc.p, h.acc.bytes, "read tally");
+    if (acc) {  // the sum of the copies
+      std::vector<double> all((size_t)MDX_ACC_COPIES * MDX_TALLY_ACC_SIZE);
+      read_back(c, all.data(), h.acc.p, h.acc.bytes, "read tally");
+      for (int k = 0; k < MDX_TALLY_ACC_SIZE; ++k) {
+        double s = 0.0;
+        for (int r = 0; r < MDX_ACC_COPIES; ++r) s += all[(size_t)r * MDX_TALLY_ACC_SIZE + k];
+        acc[k] = s;
+      }
+    }
     if (hist && h.hist.p) read_back(c, hist, h.hist.p, h.hist.bytes, "read hist");
     if (image && h.image.p) read_back(c, image, h.image.p, h.image.bytes, "read image");
   });

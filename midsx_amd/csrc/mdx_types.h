@@ -15,6 +15,9 @@
 // surface_quantity.h:13-28 and the four buckets of *_quantity.cpp measure()).
 #define MDX_NBUCKET 4  // 0 primary, 1 single incoherent, 2 single coherent, 3 multiple
 #define MDX_NFIELD 3   // n, sum, sum of squares (per recorded event)
+// Copies of each tally's scalar records: the atomics of the whole GPU on one
+// job's few records would otherwise queue on the same cache lines.
+#define MDX_ACC_COPIES 16
 #define MDX_VQ_ENERGY_DEPOSITION 0
 #define MDX_VQ_INCIDENT_ENERGY 1
 #define MDX_VQ_NUMBER_OF_PHOTONS 2
@@ -167,7 +170,8 @@ struct mdx_tally_dev {
   float radius;
   float e1e1, e2e2;
   float hist_lo, hist_inv_w;
-  double* acc;        // [MDX_NQ][MDX_NBUCKET][MDX_NFIELD]
+  double* acc;        // [MDX_ACC_COPIES][MDX_NQ][MDX_NBUCKET][MDX_NFIELD]: a workgroup adds
+                      // into copy blockIdx % MDX_ACC_COPIES, a read sums the copies
   double* hist;       // [MDX_NBUCKET][hist_bins] photon counts
   double* image;      // [ny*nx][3]: sum of incident energy, sum^2, count
 };

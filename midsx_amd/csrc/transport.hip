@@ -424,7 +424,7 @@ k_transport_plain(const mdx_tables_dev* __restrict__ T, const mdx_domain_dev* __
     double v = acc[i];
     if (v != 0.0) {
       int t = i / kAccPerTally;
-      gadd(TL->t[t].acc + (i - t * kAccPerTally), v);
+      gadd(TL->t[t].acc + (blockIdx.x % MDX_ACC_COPIES) * kAccPerTally + (i - t * kAccPerTally), v);
     }
   }
   flush_hist(TL, hoff, lhist);
@@ -440,26 +440,42 @@ k_transport_plain(const mdx_tables_dev* __restrict__ T, const mdx_domain_dev* __
 // (prefix[] holds their offsets); each job has its own scene and tallies and
 // shares the tables.  Scalar tallies are pre-reduced in LDS for one job and
 // go to global FP64 atomics for several.
+// A lane's value read by the whole wave (v_readlane: a scalar result, no LDS
+// round trip as a shuffle takes); src must be wave-uniform and active.
+__device__ __forceinline__ double readlane_f64(double v, int src) {
+  const unsigned long long b = __double_as_longlong(v);
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)b, src);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(b >> 32), src);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+__device__ __forceinline__ double* readlane_ptr(double* p, int src) {
+  const uintptr_t b = (uintptr_t)p;
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)b, src);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)((unsigned long long)b >> 32), src);
+  return (double*)(((uintptr_t)hi << 32) | lo);
+}
+
 // One scalar-tally event (count 1, value v, v^2) at acc[q][b].  The active
 // lanes of the wave that hit the same record are summed first (their values
 // are read with shuffles from active lanes only) and one of them adds the
 // sums: a job's tally records are few, so without this the atomics of a
 // whole wave would queue on one L2 line.
 __device__ __forceinline__ void event_global(double* acc, int q, int b, double v) {
-  double* p = acc + (q * MDX_NBUCKET + b) * MDX_NFIELD;
+  double* p = acc + (blockIdx.x % MDX_ACC_COPIES) * kAccPerTally + (q * MDX_NBUCKET + b) * MDX_NFIELD;
   const int lane = threadIdx.x & 63;
   const double v2 = v * v;
   unsigned long long todo = __ballot(true);
   while (todo) {
-    const int leader = __ffsll((long long)todo) - 1;
-    const unsigned long long lp = __shfl((unsigned long long)(uintptr_t)p, leader);
-    const unsigned long long same = __ballot((unsigned long long)(uintptr_t)p == lp);
+    const int leader = __ffsll((long long)todo) - 1;  // wave-uniform, as every source lane below
+    double* const lp = readlane_ptr(p, leader);
+    const unsigned long long same = __ballot(p == lp);
     todo &= ~same;
     double s = 0.0, s2 = 0.0;
     for (unsigned long long m = same; m; m &= m - 1) {
       const int src = __ffsll((long long)m) - 1;
-      s += __shfl(v, src);
-      s2 += __shfl(v2, src);
+      s += readlane_f64(v, src);
+      s2 += readlane_f64(v2, src);
     }
     if (lane == leader) {
       gadd(p, (double)__popcll(same));
@@ -476,11 +492,11 @@ __device__ __forceinline__ void wave_add(double* p, double v) {
   unsigned long long todo = __ballot(true);
   while (todo) {
     const int leader = __ffsll((long long)todo) - 1;
-    const unsigned long long lp = __shfl((unsigned long long)(uintptr_t)p, leader);
-    const unsigned long long same = __ballot((unsigned long long)(uintptr_t)p == lp);
+    double* const lp = readlane_ptr(p, leader);
+    const unsigned long long same = __ballot(p == lp);
     todo &= ~same;
     double s = 0.0;
-    for (unsigned long long m = same; m; m &= m - 1) s += __shfl(v, __ffsll((long long)m) - 1);
+    for (unsigned long long m = same; m; m &= m - 1) s += readlane_f64(v, __ffsll((long long)m) - 1);
     if (lane == leader) gadd(p, s);
   }
 }
@@ -1098,7 +1114,7 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
       double v = acc[i];
       if (v != 0.0) {
         int t = i / kAccPerTally;
-        gadd(jobs[0].TL->t[t].acc + (i - t * kAccPerTally), v);
+        gadd(jobs[0].TL->t[t].acc + (blockIdx.x % MDX_ACC_COPIES) * kAccPerTally + (i - t * kAccPerTally), v);
       }
     }
     flush_hist(jobs[0].TL, hoff, lhist);
