@@ -322,8 +322,10 @@ int mdx_get_voxel_dose_f32(mdx_ctx* ctx, int grid, uint64_t n_histories, float* 
 int mdx_stage_voxel_dose_f32(mdx_ctx* ctx, int grid, uint64_t n_histories);
 /* Starts the copy of the selected job's staged FP32 dose into out (pinned
  * host memory for a copy that does not block) and returns; out must stay
- * valid until mdx_read_done reports the copy done.  mdx_read_done polls
- * (wait = 0) or waits for the copy of job's dose, then frees the staging. */
+ * valid until mdx_read_done reports the copy done.  The copy runs on a
+ * stream of its own, so that the set-up of other jobs does not wait behind
+ * it.  mdx_read_done polls (wait = 0) or waits for the copy of job's dose,
+ * then frees the staging; staging again before that is an error. */
 int mdx_read_staged_dose_f32_async(mdx_ctx* ctx, float* out);
 int mdx_read_done(mdx_ctx* ctx, int job, int wait, int* done);
 int mdx_get_background_dose(mdx_ctx* ctx, double out[3]);

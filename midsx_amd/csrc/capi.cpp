@@ -295,6 +295,7 @@ struct mdx_ctx {
   bool inflight = false;  // a round was launched by mdx_run_jobs_async and not yet waited for
   std::vector<int> round_jobs, round_owners;  // the jobs of that round and the owners of their domains
   hipStream_t io_stream = nullptr;
+  hipStream_t copy_stream = nullptr;  // staged results to the host: large copies that set-up work must not queue behind
   hipStream_t js = nullptr;  // stream of the current call's job set-up work
   void* h_round = nullptr;  // pinned
   DBuf d_round;
@@ -1043,6 +1044,7 @@ int mdx_create(int device, mdx_ctx** out) {
     set_device(c.get());
     hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
     hip_check(hipStreamCreateWithFlags(&c->io_stream, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking), "hipStreamCreate");
     c->js = c->stream;
     hip_check(hipEventCreate(&c->ev0), "hipEventCreate");
     hip_check(hipEventCreate(&c->ev1), "hipEventCreate");
@@ -1067,7 +1069,8 @@ int mdx_destroy(mdx_ctx* c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     (void)hipStreamSynchronize(c->io_stream);
-    hipStream_t st = c->stream, io = c->io_stream;
+    (void)hipStreamSynchronize(c->copy_stream);
+    hipStream_t st = c->stream, io = c->io_stream, cp = c->copy_stream;
     hipEvent_t e0 = c->ev0, e1 = c->ev1;
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     void* pinned = c->h_round;
@@ -1077,6 +1080,7 @@ int mdx_destroy(mdx_ctx* c) {
     (void)hipEventDestroy(e1);
     (void)hipStreamDestroy(st);
     (void)hipStreamDestroy(io);
+    (void)hipStreamDestroy(cp);
   });
 }
 
@@ -1871,9 +1875,10 @@ int mdx_read_staged_dose_f32_async(mdx_ctx* c, float* out) {
     need(!D.copy_pending, "a copy of this job's dose is in flight");
     set_device_nowait(c);
     if (!D.ev_copy) hip_check(hipEventCreateWithFlags(&D.ev_copy, hipEventDisableTiming), "hipEventCreate");
-    hip_check(hipStreamWaitEvent(c->io_stream, D.ev_stage, 0), "wait staged dose");
-    hip_check(hipMemcpyAsync(out, D.d_stage.p, D.d_stage.bytes, hipMemcpyDeviceToHost, c->io_stream), "read dose f32");
-    hip_check(hipEventRecord(D.ev_copy, c->io_stream), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(c->copy_stream, D.ev_stage, 0), "wait staged dose");
+    hip_check(hipMemcpyAsync(out, D.d_stage.p, D.d_stage.bytes, hipMemcpyDeviceToHost, c->copy_stream),
+              "read dose f32");
+    hip_check(hipEventRecord(D.ev_copy, c->copy_stream), "hipEventRecord");
     D.copy_pending = true;
   });
 }
@@ -1910,6 +1915,7 @@ int mdx_stage_voxel_dose_f32(mdx_ctx* c, int grid, uint64_t n_histories) {
     Job& D = c->DJ();
     if (grid < 0 || grid >= (int)D.grids.size()) throw std::out_of_range("grid index out of range");
     need(D.voxel_dose, "voxel dose is not enabled");
+    need(!D.copy_pending, "a copy of this job's staged dose is in flight (mdx_read_done first)");
     set_device_nowait(c);
     GridHost& g = *D.grids[grid];
     need(g.dose_mode == MDX_DOSE_F64, "FP32 dose reads need FP64 accumulation (MDX_DOSE_F64)");
