@@ -700,12 +700,13 @@ constexpr int kWaves = kBlock / 64;
 
 // Work distribution.  One job: lanes claim histories from one counter.
 // Several jobs: each wave works on one job at a time, claiming from that
-// job's counter (work[j]), so that the scene a lane reads is the same for the
-// whole wave; the wave starts on the job its index falls in when waves are
-// spread over the jobs in proportion to their histories, and when its job
-// runs out it moves to the next job with histories left.  Its previous job's
-// photons finish meanwhile: a wave keeps two job slots, and waits for one to
-// empty before it takes a third job.
+// job's counter (work[j]); the wave starts on the job its index falls in when
+// waves are spread over the jobs in proportion to their histories, and when
+// its job runs out and its last photons of it are done it moves to the next
+// job with histories left.  A wave's photons thus always belong to the job
+// whose descriptors it holds, whose address is the same for every lane: the
+// descriptors are read as wave-uniform data.  (The wait for a job's last
+// photons costs little: a wave runs thousands of histories per launch.)
 template <bool MULTI>
 __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __restrict__ T,
                                                      const mdx_job_dev* __restrict__ jobs,
@@ -726,7 +727,7 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
   __shared__ mdx_material_dev lmat[MDX_MAX_MATERIALS];
   __shared__ mdx_curve lmaj;  // the global majorant
   __shared__ uint8_t lslot[256];
-  __shared__ JobDesc desc[MULTI ? 2 * kWaves : 1];
+  __shared__ JobDesc desc[MULTI ? kWaves : 1];
   extern __shared__ float ldose_mem[];  // MDX_DOSE_F32C_LDS: (sum, error) per voxel of grid 0
   int hoff[MDX_MAX_TALLIES];
   float* ldose = nullptr;
@@ -751,17 +752,16 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
 
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
+  const int wsl = __builtin_amdgcn_readfirstlane(wave);  // the same, known to be wave-uniform
   uint32_t c_hist = 0, c_step = 0, c_real = 0, c_ray = 0, c_comp = 0, c_photo = 0, c_trial = 0, c_stest = 0,
            c_dep = 0, c_surf = 0, c_esc = 0;
   // lane state: alive = carries a photon; pend = its last step ended in a
   // real interaction that waits for the wave's interaction phase
   bool alive = false, pend = false, exhausted = false;
   // several jobs (uniform per wave): the current job, the last one that ran
-  // out, the current job's slot, and whether every job has run out; per lane:
-  // the slot of the photon's job
-  int jc = -1, jc_last = -1, cs = 0;
+  // out, and whether every job has run out
+  int jc = -1, jc_last = -1;
   bool done_all = false;
-  int ls = 0;
   if (MULTI) {
     const uint64_t nw = (uint64_t)gridDim.x * kWaves;
     const uint64_t gw = (uint64_t)blockIdx.x * kWaves + (uint64_t)wave;
@@ -772,7 +772,7 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
       if (prefix[mid] <= h) lo = mid; else hi = mid - 1;
     }
     jc = lo;
-    load_job(&desc[2 * wave], jobs[jc]);
+    load_job(&desc[wsl], jobs[jc]);
   }
   const mdx_domain_dev* D = &desc[0].D;
   const mdx_source_dev* S = &desc[0].S;
@@ -805,7 +805,7 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
       // registers across the loop, which would spill.
       int o = 0;
       asm volatile("" : "+s"(o));
-      const JobDesc* jd = MULTI ? &desc[2 * wave + ls] + o : &desc[0] + o;
+      const JobDesc* jd = MULTI ? &desc[wsl] + o : &desc[0] + o;
       D = &jd->D;
       S = &jd->S;
       TL = &jd->TL;
@@ -834,11 +834,8 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
           }
         }
       } else {
-        if (jc < 0) {  // the current job ran out: the next one with histories left, in a free slot
-          const bool used0 = __ballot(alive && ls == 0) != 0ull;
-          const bool used1 = __ballot(alive && ls == 1) != 0ull;
-          const int sl = !used0 ? 0 : (!used1 ? 1 : -1);
-          if (sl >= 0) {  // else wait for a slot to empty
+        if (jc < 0) {  // the current job ran out: once its photons are done, the next one with histories left
+          if (__ballot(alive) == 0ull) {  // else wait for them
             int jn = -1;
             for (int k = 0; k < n_jobs; ++k) {
               const int j = (jc_last + 1 + k) % n_jobs;
@@ -850,9 +847,8 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
             if (jn < 0) {
               done_all = true;
             } else {
-              load_job(&desc[2 * wave + sl], jobs[jn]);
+              load_job(&desc[wsl], jobs[jn]);
               jc = jn;
-              cs = sl;
             }
           }
         }
@@ -869,11 +865,6 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
             rng.init(jobs[jc].seed, jobs[jc].stream, h);
             srng.init(jobs[jc].seed, jobs[jc].stream, h);
             flights = jobs[jc].flights != 0;
-            ls = cs;
-            const JobDesc* jd = &desc[2 * wave + ls];
-            D = &jd->D;
-            S = &jd->S;
-            TL = &jd->TL;
             fresh = true;
           }
         }
