@@ -228,7 +228,11 @@ struct Job {
   uint64_t staged_n = 0;
   hipEvent_t ev_copy = nullptr;  // end of the staged dose's copy to the host (mdx_read_staged_dose_f32_async)
   bool copy_pending = false;
-  Job() = default;
+  // Every buffer from the context's pool: a job's release then returns them
+  // for reuse instead of calling hipFree, which waits for the whole device
+  // (a round or another job's result copy in flight).
+  explicit Job(Pool* pl)
+      : d_bg(pl), d_domain(pl), d_cdf(pl), d_energies(pl), d_source(pl), d_tallies(pl), d_counters(pl), d_stage(pl) {}
   Job(const Job&) = delete;
   Job& operator=(const Job&) = delete;
   ~Job() {
@@ -1030,7 +1034,7 @@ int mdx_create(int device, mdx_ctx** out) {
       auto c = std::make_unique<mdx_ctx>();
       c->device = -1;
       c->data_dir = default_data_dir();
-      c->jobs.push_back(std::make_unique<Job>());
+      c->jobs.push_back(std::make_unique<Job>(&c->pool));
       *out = c.release();
       return;
     }
@@ -1040,7 +1044,7 @@ int mdx_create(int device, mdx_ctx** out) {
     auto c = std::make_unique<mdx_ctx>();
     c->device = device;
     c->data_dir = default_data_dir();
-    c->jobs.push_back(std::make_unique<Job>());
+    c->jobs.push_back(std::make_unique<Job>(&c->pool));
     set_device(c.get());
     hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
     hip_check(hipStreamCreateWithFlags(&c->io_stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -1574,7 +1578,7 @@ int mdx_job_create(mdx_ctx* c, int* job) {
       slot = (int)c->jobs.size();
       c->jobs.emplace_back();
     }
-    c->jobs[slot] = std::make_unique<Job>();
+    c->jobs[slot] = std::make_unique<Job>(&c->pool);
     c->cur = slot;
     *job = slot;
   });
@@ -1688,7 +1692,7 @@ int mdx_job_release(mdx_ctx* c, int job) {
       set_device(c);
       hip_check(hipStreamSynchronize(c->stream), "release");
     }
-    if (job == 0) c->jobs[0] = std::make_unique<Job>();
+    if (job == 0) c->jobs[0] = std::make_unique<Job>(&c->pool);
     else c->jobs[job].reset();
     if (c->cur == job) c->cur = 0;
   });
@@ -1920,7 +1924,6 @@ int mdx_stage_voxel_dose_f32(mdx_ctx* c, int grid, uint64_t n_histories) {
     GridHost& g = *D.grids[grid];
     need(g.dose_mode == MDX_DOSE_F64, "FP32 dose reads need FP64 accumulation (MDX_DOSE_F64)");
     const size_t nv = g.voxels();
-    if (!D.d_stage.pool) D.d_stage.pool = &c->pool;
     D.d_stage.ensure(nv * 8);
     if (!D.ev_stage) hip_check(hipEventCreateWithFlags(&D.ev_stage, hipEventDisableTiming), "hipEventCreate");
     flush_clears_over(c, g.d_dose.p, g.d_dose.bytes);

@@ -103,10 +103,14 @@ class Server:
     dose: format of the dose read back ("f32" or "f64", see `read_dose`);
     on_result(job, a, b, image) receives it (the buffers are reused).
     admit_slice: seconds of job set-up per round at most while others run
-    (the next round starts only when the loop gets back to it)."""
+    (the next round starts only when the loop gets back to it); a set-up is
+    not started when its estimated time (voxels / admit_voxels_per_s: the
+    phantom's upload and its macro-cell scan dominate) would overrun the
+    slice, unless it is the first of the slice."""
 
     def __init__(self, ctx, budget_bytes=None, round_histories=30_000_000, min_slice=100_000, max_active=512,
-                 graphs=True, phantoms=None, on_result=None, dose="f32", admit_slice=0.01):
+                 graphs=True, phantoms=None, on_result=None, dose="f32", admit_slice=0.01,
+                 admit_voxels_per_s=1.0e10):
         self.ctx = ctx
         if budget_bytes is None:
             free, _ = ctx.mem_info()
@@ -120,6 +124,7 @@ class Server:
         self.on_result = on_result
         self.dose = dose
         self.admit_slice = float(admit_slice)
+        self.admit_voxels_per_s = float(admit_voxels_per_s)
         self._buf = ctx.pinned_pool  # f64 read buffers by size, kept by the context
         self._free = ctx.pinned_pool  # free pinned float32 buffers by size, kept by the context
         self.rounds = 0
@@ -214,6 +219,7 @@ class Server:
                     ctx.job_select(a.slot)
                     a.buf = self._pinned(2 * int(np.prod(a.job["shape"])))
                     ctx.read_staged_dose_f32_async(a.buf)
+                    a.image = ctx.tally(0)["image"]  # beside the round, not between rounds
                     copying.append(a)
                 else:
                     self._read(a)
@@ -225,9 +231,10 @@ class Server:
             while waiting and len(active) + len(admitted) < self.max_active:
                 if not launched and queued >= self.round_histories:
                     break  # start the first round; admit the rest beside it
-                if (active or admitted) and clock() - t2 > self.admit_slice:
-                    break  # leave the rest for the next rounds: the device waits for this loop
                 i = waiting[0]
+                est = float(np.prod(jobs[i]["shape"])) / self.admit_voxels_per_s if admitted else 0.0
+                if (active or admitted) and clock() - t2 + est > self.admit_slice:
+                    break  # leave the rest for the next rounds: the device waits for this loop
                 need = scenes.dose_job_bytes(jobs[i])
                 if resident + need > self.budget and (active or to_read or admitted or copying):
                     break
@@ -274,10 +281,9 @@ class Server:
             to_read = [a for a in active if a.done >= a.job["histories"]]
             active = [a for a in active if a.done < a.job["histories"]] + admitted
             if self.dose == "f32":
-                for a in to_read:  # conversion queued ahead of the next round; the copy runs beside it
+                for a in to_read:  # conversion queued ahead of the next round; the copies run beside it
                     ctx.job_select(a.slot)
                     ctx.stage_voxel_dose_f32(0, a.job["histories"])
-                    a.image = ctx.tally(0)["image"]
             if idle and order:
                 time.sleep(max(0.0, min(arrivals[order[0]] - clock(), 0.05)))
         self.makespan = clock()
