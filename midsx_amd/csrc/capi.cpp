@@ -1534,7 +1534,7 @@ int mdx_autotune(mdx_ctx* c, uint64_t probe_histories, int* blocks_per_cu, int* 
     };
     int b = c->queue_blocks_per_cu, r = c->refill_min, i = c->interact_min;
     double cur = probe(b, r, i);
-    // coordinate search: interaction batch, refill batch (up to 60 of a
+    // coordinate search: interaction batch, refill batch (up to 62 of a
     // wave's 64 lanes: a source far from the grid wants its photons' free
     // flights in large batches), both again, then
     // occupancy (the kernel's 128 registers allow four waves per SIMD, i.e.
@@ -1546,7 +1546,7 @@ int mdx_autotune(mdx_ctx* c, uint64_t probe_histories, int* blocks_per_cu, int* 
         double t = probe(b, r, v);
         if (t < cur) { cur = t; i = v; }
       }
-      for (int v : {8, 16, 24, 32, 40, 48, 56, 60}) {
+      for (int v : {8, 16, 24, 32, 40, 48, 56, 60, 62}) {
         if (v == r) continue;
         double t = probe(b, v, i);
         if (t < cur) { cur = t; r = v; }
