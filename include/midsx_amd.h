@@ -200,7 +200,9 @@ int mdx_set_queue_params(mdx_ctx* ctx, int blocks_per_cu, int refill_min, int in
 int mdx_get_queue_params(mdx_ctx* ctx, int* blocks_per_cu, int* refill_min, int* interact_min);
 /* Autotuner: times the queue kernel on the selected job with probe_histories
  * per launch over a coordinate search of the three parameters, keeps the
- * fastest, and resets that job's tallies.  Outputs may be NULL. */
+ * fastest, and resets that job's tallies; for a job that uses another job's
+ * domain (mdx_job_alias_domain), also that domain's voxel and background
+ * dose, where the probes deposited.  Outputs may be NULL. */
 int mdx_autotune(mdx_ctx* ctx, uint64_t probe_histories, int* blocks_per_cu, int* refill_min, int* interact_min,
                  double* best_ms);
 
@@ -308,8 +310,9 @@ int mdx_host_free(void* p);
  * sum E^2, count per pixel; may be NULL). */
 int mdx_get_tally(mdx_ctx* ctx, int tally, double* acc, double* hist, double* image);
 /* VoxelGrid::getVoxel(i).dose (voxel_grid.cpp:43-45, 86-104): per voxel
- * (sum, sum^2) interleaved and the deposit count; either may be NULL. */
-int mdx_get_voxel_dose(mdx_ctx* ctx, int grid, double* sum_sumsq, uint32_t* count);
+ * (sum, sum^2) interleaved and the deposit count (64-bit, as the size of the
+ * reference's value list); either may be NULL. */
+int mdx_get_voxel_dose(mdx_ctx* ctx, int grid, double* sum_sumsq, uint64_t* count);
 /* ComputationalDomain background voxel dose: sum, sum^2, count. */
 /* The voxel dose of a grid as FP32 pairs (mean deposit per history S/N,
  * relative standard error of that mean) per voxel, x fastest: half the
@@ -360,6 +363,24 @@ int mdx_voxel_lookup(mdx_ctx* ctx, int n, const float* xyz, int* grid, long long
  * the device: 0 photoelectric, 1 Rayleigh, 2 Compton. */
 int mdx_select_interaction(mdx_ctx* ctx, int n, const float* s_coh, const float* s_inc, const float* s_tot,
                            const float* u, int* type);
+
+/* The delta-scatter decision isDeltaScatter (physics_engine.cpp:180-183)
+ * as both transport kernels take it on the device: real[i] = 1 when
+ * !(u < 1 - s_tot / s_max) in IEEE FP32 and the step was not cut at a local
+ * majorant's bound (limited[i] == 0; such a step takes no collision test),
+ * else 0. */
+int mdx_delta_decision(mdx_ctx* ctx, int n, const float* s_tot, const float* s_max, const float* u,
+                       const int* limited, int* real);
+/* PhotonSource::generatePhoton (source.cpp:150-155) on the device for the
+ * selected job's source: the directions (3 floats per photon) and energies
+ * of histories 0 .. n-1 under seed, as the transport kernels draw them. */
+int mdx_sample_source(mdx_ctx* ctx, int n, uint64_t seed, float* dir, float* energy);
+/* The device's counter-based generators (no reference counterpart: the
+ * reference draws from std::mt19937, probability_dist.cpp:4), for known-
+ * answer tests: words 4, Philox-4x32-10 of counter[4 i .. 4 i + 3] under
+ * key[2 i], key[2 i + 1]; words 2, Philox-2x32-10 of counter[2 i ..] under
+ * key[i]; out holds n * words words. */
+int mdx_philox(mdx_ctx* ctx, int n, int words, const uint32_t* counter, const uint32_t* key, uint32_t* out);
 
 #ifdef __cplusplus
 }

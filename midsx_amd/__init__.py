@@ -45,7 +45,8 @@ EXPORTS = [
     "mdx_run_jobs_async", "mdx_wait", "mdx_mem_info", "mdx_host_alloc", "mdx_host_free",
     "mdx_set_flights", "mdx_set_rita_builder", "mdx_set_dose_precision", "mdx_get_voxel_dose_f32", "mdx_stage_voxel_dose_f32",
     "mdx_read_staged_dose_f32_async", "mdx_read_done", "mdx_copy_probe", "mdx_air_muen", "mdx_set_air_muen_table",
-    "mdx_set_local_majorants", "mdx_get_macro_cells",
+    "mdx_set_local_majorants", "mdx_get_macro_cells", "mdx_delta_decision", "mdx_philox",
+    "mdx_sample_source",
 ]
 
 
@@ -226,6 +227,34 @@ class Context:
         out = np.empty(a.size, dtype=np.int32)
         self._call(self.lib.mdx_select_interaction, self._h, ctypes.c_int(a.size), _ptr(a, ctypes.c_float),
                    _ptr(b, ctypes.c_float), _ptr(c, ctypes.c_float), _ptr(d, ctypes.c_float), _ptr(out, ctypes.c_int))
+        return out
+
+    def delta_decision(self, s_tot, s_max, u, limited=None):
+        """1 where the device takes a real collision (physics_engine.cpp:180-183)."""
+        a, b, d = _f32(s_tot), _f32(s_max), _f32(u)
+        lim = np.zeros(a.size, np.int32) if limited is None else np.ascontiguousarray(limited, dtype=np.int32)
+        out = np.empty(a.size, dtype=np.int32)
+        self._call(self.lib.mdx_delta_decision, self._h, ctypes.c_int(a.size), _ptr(a, ctypes.c_float),
+                   _ptr(b, ctypes.c_float), _ptr(d, ctypes.c_float), _ptr(lim, ctypes.c_int), _ptr(out, ctypes.c_int))
+        return out
+
+    def sample_source(self, n, seed=1):
+        """Directions (n x 3) and energies of the selected job's source
+        photons for histories 0 .. n-1, drawn on the device."""
+        d = np.empty(3 * n, np.float32)
+        e = np.empty(n, np.float32)
+        self._call(self.lib.mdx_sample_source, self._h, ctypes.c_int(n), ctypes.c_uint64(seed),
+                   _ptr(d, ctypes.c_float), _ptr(e, ctypes.c_float))
+        return d.reshape(n, 3), e
+
+    def philox(self, counter, key, words=4):
+        """Raw Philox-4x32-10 (words 4: counter n x 4, key n x 2) or
+        Philox-2x32-10 (words 2: counter n x 2, key n) blocks on the device."""
+        c = np.ascontiguousarray(counter, dtype=np.uint32).reshape(-1, words)
+        k = np.ascontiguousarray(key, dtype=np.uint32).reshape(c.shape[0], -1)
+        out = np.empty_like(c)
+        self._call(self.lib.mdx_philox, self._h, ctypes.c_int(c.shape[0]), ctypes.c_int(words),
+                   _ptr(c, ctypes.c_uint32), _ptr(k, ctypes.c_uint32), _ptr(out, ctypes.c_uint32))
         return out
 
     # ---------------------------------------------------------- domain
@@ -469,9 +498,9 @@ class Context:
         nv = dims[0] * dims[1] * dims[2]
         ss = np.zeros(2 * nv) if out is None else out[:2 * nv]
         assert ss.dtype == np.float64 and ss.flags.c_contiguous and ss.size == 2 * nv
-        cnt = np.zeros(nv, dtype=np.uint32) if counts else None
+        cnt = np.zeros(nv, dtype=np.uint64) if counts else None
         self._call(self.lib.mdx_get_voxel_dose, self._h, ctypes.c_int(g), _ptr(ss, ctypes.c_double),
-                   None if cnt is None else _ptr(cnt, ctypes.c_uint32))
+                   None if cnt is None else _ptr(cnt, ctypes.c_uint64))
         return ss[0::2], ss[1::2], cnt
 
     def voxel_dose_f32(self, g, n_histories, out=None):
@@ -548,10 +577,19 @@ class Context:
 
 def vector_stats(acc_qb):
     """VectorValue statistics (quantity.cpp:27-94) from (n, sum, sum^2):
-    sum, sample variance, sigma of the sum sqrt(n s^2), mean and its sigma."""
+    sum, sample variance, sigma of the sum sqrt(n s^2), mean and its sigma.
+    The variance comes from the moments, clamped at 0 (rounding can take the
+    moment form below 0 for near-constant values; the reference's two-pass
+    form never goes there).  No values: variance and sigma of the sum 0, the
+    mean and its sigma NaN (0 / 0); one value: variance NaN (0 / 0), as
+    quantity.cpp computes them."""
     n, s, s2 = acc_qb
-    if n < 2:
-        return {"count": n, "sum": s, "variance": float("nan"), "sum_std": float("nan"), "mean": s / n if n else 0.0}
-    var = (s2 - s * s / n) / (n - 1)
-    return {"count": n, "sum": s, "variance": var, "sum_std": (n * var) ** 0.5, "mean": s / n,
+    nan = float("nan")
+    if n == 0:
+        return {"count": 0, "sum": s, "variance": 0.0, "sum_std": 0.0, "mean": nan, "mean_std": nan}
+    if n == 1:
+        return {"count": n, "sum": s, "variance": nan, "sum_std": nan, "mean": s, "mean_std": nan}
+    mean = s / n
+    var = max((s2 - mean * s) / (n - 1), 0.0)
+    return {"count": n, "sum": s, "variance": var, "sum_std": (n * var) ** 0.5, "mean": mean,
             "mean_std": (var / n) ** 0.5}

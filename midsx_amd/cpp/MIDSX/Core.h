@@ -9,7 +9,7 @@
 //   InteractionData interaction_data = comp_domain.getInteractionData();
 //   PhysicsEngine physics_engine(comp_domain, interaction_data);
 //   PhotonSource source(std::make_unique<MonoenergeticSpectrum>(30E3), ...);
-//   runSimulation(source, physics_engine, surface_init, volume_init, N, run_time);
+//   runSimulation(source, physics_engine, surface_init, volume_init, N);  // or ..., N, run_time)
 //   auto results = physics_engine.getSurfaceQuantityContainers();
 //
 // Differences from the reference, by design:
@@ -17,7 +17,11 @@
 //    the tally factories are called once (not once per OpenMP thread);
 //  * VectorValue keeps (count, sum, sum of squares) instead of every value, so
 //    getVector() is not available; getSum / getVariance / getSumSTD / getMean /
-//    getMeanSTD / getCount return what the reference computes from its list;
+//    getMeanSTD / getCount return what the reference computes from its list
+//    (the variance from the moments, clamped at 0);
+//  * vectors are Vec3 (or anything with x(), y(), z(), Eigen's included), and
+//    SourceHelpers::readCSV and DerivedQuantity take std::vector where the
+//    reference takes Eigen matrices;
 //  * only the closed set of tally, spectrum, directionality and geometry
 //    classes below is accepted (user subclasses throw std::invalid_argument).
 #pragma once
@@ -84,12 +88,25 @@ class VectorValue {
   double getCount() const { return (double)n_; }
   double getCountSTD() const { return std::sqrt((double)n_); }
   double getMean() const { return sum_ / (double)n_; }
-  double getVariance() const {  // sample variance, quantity.cpp:80-94
-    double mean = sum_ / (double)n_;
-    return (sum2_ - 2 * mean * sum_ + (double)n_ * mean * mean) / (double)(n_ - 1);
+  // Sample variance (quantity.cpp:80-94) from the moments: sum of squares
+  // less sum * mean over n - 1, clamped at 0 (the two-pass form of the
+  // reference never goes below 0; from the moments, rounding can for
+  // near-constant values).  No values: 0 (the reference divides 0 by
+  // size() - 1 in size_t, a huge number); one value: 0 / 0, NaN, as there.
+  double getVariance() const {
+    if (n_ == 0) return 0.0;
+    if (n_ == 1) return std::nan("");
+    const double mean = sum_ / (double)n_;
+    const double v = (sum2_ - mean * sum_) / (double)(n_ - 1);
+    return v > 0.0 ? v : 0.0;
   }
   double getSumSTD() const { return std::sqrt((double)n_ * getVariance()); }
   double getMeanSTD() const { return std::sqrt(getVariance() / (double)n_); }
+  // quantity.cpp:15-21 (the values are folded into the moments)
+  void addValue(double v) { addMoments(1.0, v, v * v); }
+  void addValues(const std::vector<double>& vs) {
+    for (double v : vs) addValue(v);
+  }
   void addMoments(double n, double s, double s2) {
     n_ += (long long)std::llround(n);
     sum_ += s;
@@ -110,6 +127,7 @@ class CountValue {
  public:
   int getCount() const { return count_; }
   double getCountSTD() const { return std::sqrt((double)count_); }
+  void addCount() { ++count_; }  // quantity.cpp:104-106
   void addCounts(int c) { count_ += c; }
   CountValue operator+(const CountValue& o) const {
     CountValue r;
@@ -723,10 +741,10 @@ class PhysicsEngine {
     for (int g = 0; g < domain_.getNumVoxelGrids(); ++g) {
       VoxelGrid& G = domain_.getVoxelGridN(g);
       std::vector<double> ss(2 * G.voxels.size());
-      std::vector<uint32_t> cnt(G.voxels.size());
+      std::vector<uint64_t> cnt(G.voxels.size());
       midsx_amd_detail::check(mdx_get_voxel_dose(ctx_.p, g, ss.data(), cnt.data()));
       for (size_t i = 0; i < G.voxels.size(); ++i)
-        if (cnt[i]) G.voxels[i].dose.addMoments(cnt[i], ss[2 * i], ss[2 * i + 1]);
+        if (cnt[i]) G.voxels[i].dose.addMoments((double)cnt[i], ss[2 * i], ss[2 * i + 1]);
     }
     midsx_amd_detail::check(mdx_reset_tallies(ctx_.p));
   }
@@ -791,4 +809,16 @@ inline void runSimulation(PhotonSource& source, PhysicsEngine& physics_engine,
   physics_engine.run(source, (uint64_t)N_photons);
   auto t1 = std::chrono::steady_clock::now();
   run_time = std::chrono::duration<double>(t1 - t0).count();
+}
+
+// The reference defaults run_time (run_simulation.h:27), so drivers call the
+// five-argument form (hvl_30kVp.cpp:70, CT_w_voxelized_solid.cpp:166,
+// radiography_15_degrees.cpp:326); the run time is then not reported.
+inline void runSimulation(PhotonSource& source, PhysicsEngine& physics_engine,
+                          std::function<std::vector<std::unique_ptr<SurfaceTally>>()> surface_tally_init,
+                          std::function<std::vector<std::unique_ptr<VolumeTally>>()> volume_tally_init,
+                          int N_photons) {
+  double run_time = 0.0;
+  runSimulation(source, physics_engine, std::move(surface_tally_init), std::move(volume_tally_init), N_photons,
+                run_time);
 }

@@ -37,9 +37,9 @@ int main(int argc, char** argv) {
                       std::make_unique<DiscIsotropicDirectionality>(Vec3(2, 2, 10), Vec3(0, 0, 1), 0.1),
                       std::make_unique<PointGeometry>(Vec3(2, 2, 0)));
   int n = std::atoi(argv[3]);
-  double run_time = 0;
+  // the five-argument call of hvl_30kVp.cpp:70 (run_time defaulted)
   runSimulation(source, physics_engine, initializeSurfaceTallies,
-                []() { return std::vector<std::unique_ptr<VolumeTally>>(); }, n, run_time);
+                []() { return std::vector<std::unique_ptr<VolumeTally>>(); }, n);
   auto surface = physics_engine.getSurfaceQuantityContainers();
   std::vector<double> energies;
   for (auto& r : spectrum_eV) energies.push_back(r.first);

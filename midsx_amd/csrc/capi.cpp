@@ -243,7 +243,8 @@ struct Job {
   size_t device_bytes() const {
     size_t b = d_bg.bytes + d_domain.bytes + d_cdf.bytes + d_energies.bytes + d_source.bytes + d_tallies.bytes +
                d_counters.bytes;
-    for (auto& g : grids) b += g->d_ids.bytes + g->d_dose.bytes + g->d_dose_n.bytes;
+    for (auto& g : grids) b += g->d_ids.bytes + g->d_dose.bytes + g->d_dose_n.bytes + g->d_macro.bytes;
+    b += d_stage.bytes;  // the FP32 dose staged for reading, held until its copy is done
     for (auto& t : tallies) b += t->acc.bytes + t->hist.bytes + t->image.bytes;
     return b;
   }
@@ -639,19 +640,22 @@ void upload_domain(mdx_ctx* c) {
       d.mcell = (float)(MDX_MACRO_EDGE * std::min(G.spacing[0], std::min(G.spacing[1], G.spacing[2])));
     }
     if (c->J().voxel_dose) {
+      // FP64: one record of MDX_DOSE_REC doubles per voxel (count inside);
+      // FP32 layouts: the sums, and the counts in an array of their own
       const int mode = c->J().dose_mode;
-      const size_t per = mode == MDX_DOSE_F32 ? 8 : 16;
-      if (G.d_dose.bytes != nv * per || G.dose_mode != mode || !G.d_dose_n.p) {
+      const size_t per = mode == MDX_DOSE_F64 ? MDX_DOSE_REC * sizeof(double) : (mode == MDX_DOSE_F32 ? 8 : 16);
+      const size_t per_n = mode == MDX_DOSE_F64 ? 0 : sizeof(unsigned long long);
+      if (G.d_dose.bytes != nv * per || G.dose_mode != mode || G.d_dose_n.bytes != nv * per_n) {
         G.d_dose.alloc(nv * per);
-        G.d_dose_n.alloc(nv * 4);
+        G.d_dose_n.alloc(nv * per_n);
         G.dose_mode = mode;
         clear_dev(c, G.d_dose.p, nv * per);
-        clear_dev(c, G.d_dose_n.p, nv * 4);
+        clear_dev(c, G.d_dose_n.p, nv * per_n);
       }
       d.dose_mode = mode;
       d.dose = mode == MDX_DOSE_F64 ? G.d_dose.as<double>() : nullptr;
       d.dose32 = mode == MDX_DOSE_F64 ? nullptr : G.d_dose.as<float>();
-      d.dose_n = G.d_dose_n.as<uint32_t>();
+      d.dose_n = G.d_dose_n.as<unsigned long long>();
     } else {
       G.d_dose.release();
       G.d_dose_n.release();
@@ -1561,6 +1565,17 @@ int mdx_autotune(mdx_ctx* c, uint64_t probe_histories, int* blocks_per_cu, int* 
     c->refill_min = r;
     c->interact_min = i;
     zero_tallies(c);
+    // a job that uses another job's domain deposited its probes' dose in
+    // that job's grids and background voxel: those are cleared as well
+    if (c->J().domain_of >= 0) {
+      Job& O = c->DJ();
+      for (auto& g : O.grids) {
+        clear_dev(c, g->d_dose.p, g->d_dose.bytes);
+        clear_dev(c, g->d_dose_n.p, g->d_dose_n.bytes);
+      }
+      if (O.d_bg.p) clear_dev(c, O.d_bg.p, O.d_bg.bytes);
+      hip_check(hipStreamSynchronize(c->js), "reset tallies");
+    }
     if (blocks_per_cu) *blocks_per_cu = b;
     if (refill_min) *refill_min = r;
     if (interact_min) *interact_min = i;
@@ -1815,7 +1830,7 @@ int mdx_get_tally(mdx_ctx* c, int i, double* acc, double* hist, double* image) {
   });
 }
 
-int mdx_get_voxel_dose(mdx_ctx* c, int grid, double* ss, uint32_t* count) {
+int mdx_get_voxel_dose(mdx_ctx* c, int grid, double* ss, uint64_t* count) {
   return guard([&] {
     need(c, "NULL context");
     Job& D = c->DJ();
@@ -1824,9 +1839,27 @@ int mdx_get_voxel_dose(mdx_ctx* c, int grid, double* ss, uint32_t* count) {
     set_device_nowait(c);
     GridHost& g = *D.grids[grid];
     const size_t nv = g.voxels();
-    if (ss && g.dose_mode == MDX_DOSE_F64) {
-      read_back(c, ss, g.d_dose.p, g.d_dose.bytes, "read dose");
-    } else if (ss) {  // FP32 layouts, widened: (sum, sum^2) or (sum + error, sum^2 + error)
+    if (g.dose_mode == MDX_DOSE_F64) {
+      if (!ss && !count) return;
+      // the records, in chunks, split into (sum, sum^2) pairs and counts
+      const size_t chunk = std::min<size_t>(nv, (size_t)1 << 20);
+      std::vector<double> rec(MDX_DOSE_REC * chunk);
+      for (size_t v0 = 0; v0 < nv; v0 += chunk) {
+        const size_t m = std::min(chunk, nv - v0);
+        read_back(c, rec.data(), g.d_dose.as<double>() + MDX_DOSE_REC * v0, MDX_DOSE_REC * m * sizeof(double),
+                  "read dose");
+        for (size_t i = 0; i < m; ++i) {
+          const double* r = &rec[MDX_DOSE_REC * i];
+          if (ss) {
+            ss[2 * (v0 + i)] = r[0];
+            ss[2 * (v0 + i) + 1] = r[1];
+          }
+          if (count) std::memcpy(&count[v0 + i], &r[2], sizeof(uint64_t));
+        }
+      }
+      return;
+    }
+    if (ss) {  // FP32 layouts, widened: (sum, sum^2) or (sum + error, sum^2 + error)
       const int w = g.dose_mode == MDX_DOSE_F32 ? 2 : 4;
       std::vector<float> f(nv * w);
       read_back(c, f.data(), g.d_dose.p, f.size() * sizeof(float), "read dose");
@@ -2111,6 +2144,49 @@ int mdx_select_interaction(mdx_ctx* c, int n, const float* sc, const float* si, 
     hip_check(mdx_launch::select(n, a.p(), b.p(), d.p(), e.p(), o.p(), c->stream), "select");
     hip_check(hipMemcpyAsync(type, o.p(), n * sizeof(int), hipMemcpyDeviceToHost, c->stream), "read");
     hip_check(hipStreamSynchronize(c->stream), "select");
+  });
+}
+
+int mdx_delta_decision(mdx_ctx* c, int n, const float* s_tot, const float* s_max, const float* u,
+                       const int* limited, int* real) {
+  return guard([&] {
+    need(c && s_tot && s_max && u && limited && real && n >= 0, "bad argument");
+    if (n == 0) return;
+    set_device(c);
+    DevArray<float> a(s_tot, n, c->stream), b(s_max, n, c->stream), d(u, n, c->stream);
+    DevArray<int> l(limited, n, c->stream), o(nullptr, n, c->stream);
+    hip_check(mdx_launch::delta(n, a.p(), b.p(), d.p(), l.p(), o.p(), c->stream), "delta");
+    hip_check(hipMemcpyAsync(real, o.p(), n * sizeof(int), hipMemcpyDeviceToHost, c->stream), "read");
+    hip_check(hipStreamSynchronize(c->stream), "delta");
+  });
+}
+
+int mdx_sample_source(mdx_ctx* c, int n, uint64_t seed, float* dir, float* energy) {
+  return guard([&] {
+    need(c && dir && energy && n >= 0, "bad argument");
+    need(c->J().source_set, "source not set (mdx_set_source)");
+    if (n == 0) return;
+    set_device(c);
+    DevArray<float> d(nullptr, 3 * (size_t)n, c->stream), e(nullptr, n, c->stream);
+    hip_check(mdx_launch::source(c->J().d_source.as<mdx_source_dev>(), n, seed, d.p(), e.p(), c->stream), "source");
+    hip_check(hipMemcpyAsync(dir, d.p(), 3 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream), "read");
+    hip_check(hipMemcpyAsync(energy, e.p(), n * sizeof(float), hipMemcpyDeviceToHost, c->stream), "read");
+    hip_check(hipStreamSynchronize(c->stream), "source");
+  });
+}
+
+int mdx_philox(mdx_ctx* c, int n, int words, const uint32_t* counter, const uint32_t* key, uint32_t* out) {
+  return guard([&] {
+    need(c && counter && key && out && n >= 0, "bad argument");
+    need(words == 2 || words == 4, "words must be 2 or 4");
+    if (n == 0) return;
+    set_device(c);
+    DevArray<uint32_t> dc(counter, (size_t)n * words, c->stream), dk(key, (size_t)n * (words / 2), c->stream);
+    DevArray<uint32_t> o(nullptr, (size_t)n * words, c->stream);
+    hip_check(mdx_launch::philox(n, words, dc.p(), dk.p(), o.p(), c->stream), "philox");
+    hip_check(hipMemcpyAsync(out, o.p(), (size_t)n * words * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream),
+              "read");
+    hip_check(hipStreamSynchronize(c->stream), "philox");
   });
 }
 

@@ -19,8 +19,43 @@ constexpr float kAlphaSqrt2 = (float)(kAlphaD * 1.41421356237309504880);
 constexpr float kEps = 1e-9f;  // constants.h:14
 
 // ------------------------------------------------------------------ RNG
-// Philox-4x32-10 (Salmon et al., SC'11), counter-based: one stream per
-// history, keyed by (seed, job) with the history index in the counter.
+// Philox-4x32-10 and Philox-2x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel
+// random numbers: as easy as 1, 2, 3", SC'11): ten rounds of the multiply /
+// xor-with-key bijection, the key bumped by the Weyl constants after each.
+// Output = the block of counter (x0..x3) under key (k0, k1).
+__device__ __forceinline__ void philox4x32_10(uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3, uint32_t a,
+                                              uint32_t b, uint32_t& o0, uint32_t& o1, uint32_t& o2, uint32_t& o3) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint32_t h0 = __umulhi(0xD2511F53u, x0), l0 = 0xD2511F53u * x0;
+    const uint32_t h1 = __umulhi(0xCD9E8D57u, x2), l1 = 0xCD9E8D57u * x2;
+    x0 = h1 ^ x1 ^ a;
+    x1 = l1;
+    x2 = h0 ^ x3 ^ b;
+    x3 = l0;
+    a += 0x9E3779B9u;
+    b += 0xBB67AE85u;
+  }
+  o0 = x0;
+  o1 = x1;
+  o2 = x2;
+  o3 = x3;
+}
+
+__device__ __forceinline__ void philox2x32_10(uint32_t x0, uint32_t x1, uint32_t k, uint32_t& o0, uint32_t& o1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint32_t hi = __umulhi(0xD256D193u, x0), lo = 0xD256D193u * x0;
+    x0 = hi ^ k ^ x1;
+    x1 = lo;
+    k += 0x9E3779B9u;
+  }
+  o0 = x0;
+  o1 = x1;
+}
+
+// Per-history stream of Philox-4x32-10 blocks, keyed by (seed, job) with the
+// history index in the counter.
 struct Philox {
   uint32_t k0, k1;
   uint32_t c0, c1, c2, c3;
@@ -37,22 +72,7 @@ struct Philox {
     left = 0;
   }
   __device__ void refill() {
-    uint32_t x0 = c0, x1 = c1, x2 = c2, x3 = c3, a = k0, b = k1;
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-      uint32_t h0 = __umulhi(0xD2511F53u, x0), l0 = 0xD2511F53u * x0;
-      uint32_t h1 = __umulhi(0xCD9E8D57u, x2), l1 = 0xCD9E8D57u * x2;
-      x0 = h1 ^ x1 ^ a;
-      x1 = l1;
-      x2 = h0 ^ x3 ^ b;
-      x3 = l0;
-      a += 0x9E3779B9u;
-      b += 0xBB67AE85u;
-    }
-    b0 = x0;
-    b1 = x1;
-    b2 = x2;
-    b3 = x3;
+    philox4x32_10(c0, c1, c2, c3, k0, k1, b0, b1, b2, b3);
     left = 4;
     ++c0;
   }
@@ -92,14 +112,8 @@ struct StepRng {
     step = 0;
   }
   __device__ void next(float& u1, float& u2) {
-    uint32_t x0 = hist, x1 = step, k = key;
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-      uint32_t hi = __umulhi(0xD256D193u, x0), lo = 0xD256D193u * x0;
-      x0 = hi ^ k ^ x1;
-      x1 = lo;
-      k += 0x9E3779B9u;
-    }
+    uint32_t x0, x1;
+    philox2x32_10(hist, step, key, x0, x1);
     ++step;
     u1 = (float)((x0 >> 8) + 1u) * (1.0f / 16777216.0f);
     u2 = (float)((x1 >> 8) + 1u) * (1.0f / 16777216.0f);
@@ -283,6 +297,12 @@ __device__ __forceinline__ int voxel_lookup(const mdx_domain_dev* D, float x, fl
   }
   return -1;
 }
+
+// physics_engine.cpp:180-183: a step is a delta (virtual) collision when
+// u < 1 - sigma_tot / sigma_max; the threshold is formed with the IEEE
+// quotient and difference, in that order, by both transport kernels.
+__device__ __forceinline__ float delta_threshold(float s_tot, float s_max) { return 1.0f - s_tot / s_max; }
+__device__ __forceinline__ bool real_collision(float u, float thr) { return !(u < thr); }
 
 // physics_engine.cpp:82-109: u < p_coh -> Rayleigh (1); u < p_coh + p_inc ->
 // Compton (2); otherwise photoelectric (0).  p = sigma / sigma_tot with IEEE

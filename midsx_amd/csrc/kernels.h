@@ -33,6 +33,14 @@ hipError_t voxel(const mdx_tables_dev* T, const mdx_domain_dev* D, int n, const 
                  long long* lin, int* mat, hipStream_t st);
 hipError_t select(int n, const float* sc, const float* si, const float* stt, const float* u, int* type,
                   hipStream_t st);
+hipError_t delta(int n, const float* stt, const float* sm, const float* u, const int* limited, int* real,
+                 hipStream_t st);
+// Directions (3 per photon) and energies of the source photons of histories
+// [0, n) under seed, as the transport kernels generate them.
+hipError_t source(const mdx_source_dev* S, int n, uint64_t seed, float* dir, float* E, hipStream_t st);
+// words 4: Philox-4x32-10 of ctr[4 i..] under key[2 i..]; words 2:
+// Philox-2x32-10 of ctr[2 i..] under key[i].
+hipError_t philox(int n, int words, const uint32_t* ctr, const uint32_t* key, uint32_t* out, hipStream_t st);
 hipError_t kahn(const mdx_tables_dev* T, int slot, int n, const float* E, uint64_t seed, float* mu, float* Ep,
                 hipStream_t st);
 hipError_t zero(double* p, size_t n, hipStream_t st);
@@ -50,7 +58,8 @@ hipError_t rita_build(const double* lx, const double* ly, int n_ff, double alpha
 int rita_cap();
 // Streaming copy of n16 16-byte words (HBM reference for traffic counters).
 hipError_t copy16(const void* src, void* dst, size_t n16, int blocks, hipStream_t st);
-// (mean per history, relative standard error) per voxel from (sum, sum^2).
+// (mean per history, relative standard error) per voxel from the FP64 voxel
+// records (sum, sum^2, count, 0).
 hipError_t dose_f32(const double* ss, size_t nv, double n, float* out, hipStream_t st);
 
 }  // namespace mdx_launch

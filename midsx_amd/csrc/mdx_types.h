@@ -94,6 +94,9 @@ struct mdx_tables_dev {
 #define MDX_DOSE_F32 1   // FP32 atomics
 #define MDX_DOSE_F32C 2  // FP32 atomics with the rounding error of each add kept in a second FP32
 #define MDX_DOSE_F32C_LDS 3  // as F32C, the sums of deposits first per block in LDS (one grid that fits)
+// MDX_DOSE_F64 voxel record, 32 bytes (one aligned sector, so a deposit
+// dirties one cache line): sum, sum of squares, deposit count (uint64), zero.
+#define MDX_DOSE_REC 4
 
 struct mdx_grid_dev {
   float origin[3];
@@ -102,9 +105,9 @@ struct mdx_grid_dev {
   float dim[3];
   int32_t n[3];
   const uint8_t* ids;  // x-fastest: i + j*nx + k*nx*ny (voxel_grid.cpp:119-121)
-  double* dose;        // MDX_DOSE_F64: per voxel sum, sum^2 interleaved (nullptr = off)
+  double* dose;        // MDX_DOSE_F64: per voxel one MDX_DOSE_REC-double record (nullptr = off)
   float* dose32;       // MDX_DOSE_F32: (sum, sum^2); MDX_DOSE_F32C: (sum, its error, sum^2, its error)
-  uint32_t* dose_n;    // per voxel deposit count (nullptr = off)
+  unsigned long long* dose_n;  // FP32 modes: per voxel deposit count (MDX_DOSE_F64 keeps it in the record)
   int32_t dose_mode;   // MDX_DOSE_*
   int32_t pad;
   // Local majorants (nullptr = off): per macro cell of MDX_MACRO_EDGE^3

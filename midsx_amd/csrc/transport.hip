@@ -119,10 +119,13 @@ __device__ __forceinline__ void voxel_deposit(const mdx_grid_dev& G, uint64_t li
       if (ql != 0.0f) gadd(G.dose32 + 4 * lin + 3, ql);
     }
   } else if (G.dose) {
-    gadd(G.dose + 2 * lin, (double)dep);
-    gadd(G.dose + 2 * lin + 1, (double)dep * (double)dep);
+    double* r = G.dose + MDX_DOSE_REC * lin;
+    gadd(r, (double)dep);
+    gadd(r + 1, (double)dep * (double)dep);
+    gadd(reinterpret_cast<unsigned long long*>(r + 2), 1ull);
+    return;
   }
-  if (G.dose_n) gadd(G.dose_n + lin, 1u);
+  if (G.dose_n) gadd(G.dose_n + lin, 1ull);
 }
 
 __device__ void process_tallies(const mdx_tallies_dev* TL, double* acc, const Step& r, uint32_t& surf_hits,
@@ -363,7 +366,7 @@ k_transport_plain(const mdx_tables_dev* __restrict__ T, const mdx_domain_dev* __
       }
       bool alive = true;
       // :180-183 delta scatter if u < 1 - sigma_tot / sigma_max
-      if (!(rng.uniform() < 1.0f - stot / smax)) {
+      if (real_collision(rng.uniform(), delta_threshold(stot, smax))) {
         ++c_real;
         r.interact = true;
         float sc, si;
@@ -942,7 +945,7 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
             cvalid |= 1u << slot;
           }
         }
-        thr = 1.0f - stot / smax;
+        thr = delta_threshold(stot, smax);
         cslot = slot;
       }
       bool replay = escaped;  // replay this step's segment for the tallies
@@ -982,7 +985,7 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
           replay = true;
           alive = false;
         }
-      } else if (!limited && !(u2 < thr)) {  // real interaction: park it for the interaction phase
+      } else if (!limited && real_collision(u2, thr)) {  // real interaction: park it for the interaction phase
         pend = true;
         qx = x0; qy = y0; qz = z0; qs = s;
         qg = g;
@@ -1010,7 +1013,7 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
             }
           }
           step_scale = 0.69314718055994531f * frcp(smax);
-          thr = 1.0f - stot / smax;
+          thr = delta_threshold(stot, smax);
         }
         dsafe = ncls == 255 ? INFINITY : (float)nr * D->grids[0].mcell;
       }
@@ -1336,6 +1339,40 @@ __global__ void k_select(int n, const float* sc, const float* si, const float* s
   type[i] = select_type(sc[i], si[i], st[i], u[i]);
 }
 
+// The delta-scatter decision of both transport kernels (limited: the step
+// was cut where a local majorant's bound ends, which takes no collision
+// test) and the raw Philox blocks, for known-answer tests.
+__global__ void k_delta(int n, const float* st, const float* sm, const float* u, const int* limited, int* real) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  real[i] = (!limited[i] && real_collision(u[i], delta_threshold(st[i], sm[i]))) ? 1 : 0;
+}
+
+// Source photons of histories [0, n) as the transport kernels draw them.
+__global__ void k_source(const mdx_source_dev* S, int n, uint64_t seed, float* dir, float* E) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Philox rng;
+  rng.init(seed, 0u, (uint64_t)i);
+  float px, py, pz, dx, dy, dz, e;
+  generate(S, rng, px, py, pz, dx, dy, dz, e);
+  dir[3 * i] = dx;
+  dir[3 * i + 1] = dy;
+  dir[3 * i + 2] = dz;
+  E[i] = e;
+}
+
+__global__ void k_philox(int n, int words, const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (words == 4) {
+    philox4x32_10(ctr[4 * i], ctr[4 * i + 1], ctr[4 * i + 2], ctr[4 * i + 3], key[2 * i], key[2 * i + 1], out[4 * i],
+                  out[4 * i + 1], out[4 * i + 2], out[4 * i + 3]);
+  } else {
+    philox2x32_10(ctr[2 * i], ctr[2 * i + 1], key[i], out[2 * i], out[2 * i + 1]);
+  }
+}
+
 __global__ void k_kahn(const mdx_tables_dev* T, int slot, int n, const float* E, uint64_t seed, float* mu,
                        float* Ep) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1383,11 +1420,12 @@ __global__ void k_zero_u64(unsigned long long* p, size_t n) {
 
 // Per voxel: the mean deposit per history m = S / N and the relative
 // standard error of that mean, sqrt((S2 / N - m^2) / (N - 1)) / m (0 where
-// nothing was deposited), from the FP64 (S, S2) pairs, as FP32 pairs.
-__global__ void k_dose_f32(const double2* __restrict__ ss, size_t nv, double n, float2* __restrict__ out) {
+// nothing was deposited), from the (S, S2) of the FP64 voxel records, as
+// FP32 pairs.
+__global__ void k_dose_f32(const double* __restrict__ rec, size_t nv, double n, float2* __restrict__ out) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nv) return;
-  const double2 v = ss[i];
+  const double2 v = *reinterpret_cast<const double2*>(rec + MDX_DOSE_REC * i);
   const double m = v.x / n;
   const double var = fmax(v.y / n - m * m, 0.0) / fmax(n - 1.0, 1.0);
   out[i] = make_float2((float)m, m > 0.0 ? (float)(sqrt(var) / m) : 0.0f);
@@ -1481,6 +1519,22 @@ hipError_t select(int n, const float* sc, const float* si, const float* stt, con
   return hipGetLastError();
 }
 
+hipError_t delta(int n, const float* stt, const float* sm, const float* u, const int* limited, int* real,
+                 hipStream_t st) {
+  hipLaunchKernelGGL(k_delta, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, stt, sm, u, limited, real);
+  return hipGetLastError();
+}
+
+hipError_t source(const mdx_source_dev* S, int n, uint64_t seed, float* dir, float* E, hipStream_t st) {
+  hipLaunchKernelGGL(k_source, dim3(blocks_for(n, 256)), dim3(256), 0, st, S, n, seed, dir, E);
+  return hipGetLastError();
+}
+
+hipError_t philox(int n, int words, const uint32_t* ctr, const uint32_t* key, uint32_t* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_philox, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, words, ctr, key, out);
+  return hipGetLastError();
+}
+
 hipError_t kahn(const mdx_tables_dev* T, int slot, int n, const float* E, uint64_t seed, float* mu, float* Ep,
                 hipStream_t st) {
   hipLaunchKernelGGL(k_kahn, dim3(blocks_for(n, 256)), dim3(256), 0, st, T, slot, n, E, seed, mu, Ep);
@@ -1506,8 +1560,8 @@ hipError_t zero_u32(uint32_t* p, size_t n, hipStream_t st) {
 
 hipError_t dose_f32(const double* ss, size_t nv, double n, float* out, hipStream_t st) {
   if (nv == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_dose_f32, dim3(blocks_for(nv, 256)), dim3(256), 0, st, reinterpret_cast<const double2*>(ss),
-                     nv, n, reinterpret_cast<float2*>(out));
+  hipLaunchKernelGGL(k_dose_f32, dim3(blocks_for(nv, 256)), dim3(256), 0, st, ss, nv, n,
+                     reinterpret_cast<float2*>(out));
   return hipGetLastError();
 }
 

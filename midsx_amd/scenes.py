@@ -169,12 +169,17 @@ def setup_dose_job(sim, job, ids=None):
     sim.enable_voxel_dose(True)
 
 
-def dose_job_bytes(job):
-    """Device bytes a dose job holds: material IDs, per-voxel (sum, sum^2,
-    count), the image's (sum, sum^2, count) per pixel, plus small records."""
-    nv = int(np.prod(job["shape"]))
+def dose_job_bytes(job, dose="f32"):
+    """Device bytes a dose job holds at most: material IDs (1 B per voxel),
+    the FP64 voxel records (sum, sum^2, count, 32 B), the macro-cell table
+    (2 B per 8^3 voxels), with dose "f32" the staged FP32 (mean, relative
+    error) pairs read back at the end (8 B per voxel), the image's (sum,
+    sum^2, count) per pixel, plus small records."""
+    nx, ny, nz = job["shape"]
+    nv = nx * ny * nz
+    cells = -(-nx // 8) * -(-ny // 8) * -(-nz // 8)
     ny_i, nx_i = job["image"]
-    return nv * (1 + 16 + 4) + 24 * nx_i * ny_i + (1 << 16)
+    return nv * (1 + 32 + (8 if dose == "f32" else 0)) + 2 * cells + 24 * nx_i * ny_i + (1 << 16)
 
 
 # --------------------------------------------------------------------------- CT

@@ -147,7 +147,10 @@ class Server:
         slot = ctx.job_create()
         scenes.setup_dose_job(ctx, job, ids=self.phantoms(job))
         res.admitted = now
-        return _Active(job, res, slot, ctx.job_device_bytes(slot))
+        # what the job holds now, plus the FP32 dose it stages for its read
+        # (held from its last round until the copy is done)
+        staged = 8 * int(np.prod(job["shape"])) if self.dose == "f32" else 0
+        return _Active(job, res, slot, ctx.job_device_bytes(slot) + staged)
 
     def _read(self, a):
         """Copies a finished job's results back (overlaps a round in flight)."""
@@ -185,7 +188,7 @@ class Server:
         n = len(jobs)
         arrivals = np.zeros(n) if arrivals is None else np.asarray(arrivals, dtype=float)
         for j in jobs:
-            if scenes.dose_job_bytes(j) > self.budget:
+            if scenes.dose_job_bytes(j, self.dose) > self.budget:
                 raise ValueError("a job's footprint exceeds the memory budget")
         order = deque(sorted(range(n), key=lambda i: arrivals[i]))
         results = [JobResult(i, float(arrivals[i])) for i in range(n)]
@@ -235,7 +238,7 @@ class Server:
                 est = float(np.prod(jobs[i]["shape"])) / self.admit_voxels_per_s if admitted else 0.0
                 if (active or admitted) and clock() - t2 + est > self.admit_slice:
                     break  # leave the rest for the next rounds: the device waits for this loop
-                need = scenes.dose_job_bytes(jobs[i])
+                need = scenes.dose_job_bytes(jobs[i], self.dose)
                 if resident + need > self.budget and (active or to_read or admitted or copying):
                     break
                 waiting.popleft()

@@ -38,7 +38,8 @@ def lib():
         for name in ["orc_destroy", "orc_set_majorant", "orc_add_material", "orc_set_domain", "orc_add_grid",
                      "orc_set_source", "orc_set_tallies", "orc_enable_voxel_dose", "orc_run", "orc_get_tally",
                      "orc_get_voxel_dose", "orc_get_background_dose", "orc_get_counters", "orc_reset",
-                     "orc_rita_sample", "orc_interp", "orc_kahn_sample", "orc_rotate"]:
+                     "orc_rita_sample", "orc_interp", "orc_kahn_sample", "orc_rotate", "orc_aacuboid_traversal",
+                     "orc_sample_source", "orc_max_threads"]:
             getattr(L, name).argtypes = None
         _lib = L
     return _lib
@@ -215,6 +216,16 @@ class Simulation:
                                ep.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
         return mu, ep
 
+    def sample_source(self, n, seed=1):
+        """n photons of the source as generatePhoton draws them: directions
+        (n x 3) and energies."""
+        d = np.zeros(3 * n)
+        e = np.zeros(n)
+        self.L.orc_sample_source(self.h, ctypes.c_int(n), ctypes.c_uint64(seed),
+                                 d.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                 e.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return d.reshape(n, 3), e
+
     def interp(self, mid, which, x):
         return self.L.orc_interp(self.h, ctypes.c_int(mid), ctypes.c_int(which), ctypes.c_double(x))
 
@@ -223,6 +234,29 @@ class Simulation:
             self.L.orc_destroy(self.h)
         except Exception:
             pass
+
+
+TRAVERSAL = ("misses", "lands_inside", "passes_through", "starts_inside_exits", "starts_inside_stays")
+
+
+def aacuboid_traversal(mn, mx, pos, direction, free_path):
+    """AACuboidVolumeTally::determineVolumeTraversal for n steps; returns
+    indices into TRAVERSAL."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    d = np.ascontiguousarray(direction, dtype=np.float64).reshape(-1, 3)
+    fp = np.ascontiguousarray(free_path, dtype=np.float64).ravel()
+    out = np.zeros(len(fp), dtype=np.int32)
+    lib().orc_aacuboid_traversal(_v3(mn), _v3(mx), ctypes.c_int(len(fp)),
+                                 pos.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                 d.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                 fp.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                 out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    return out
+
+
+def max_threads():
+    """The OpenMP team size of a run with threads=0 (omp_get_max_threads)."""
+    return int(lib().orc_max_threads())
 
 
 def rotate(d, theta, phi):

@@ -242,6 +242,31 @@ V3 rotate(const V3& v, double theta, double phi) {  // particle.cpp:7-45 (quater
   return {r.x, r.y, r.z};
 }
 
+// AACuboidVolumeTally::determineVolumeTraversal (volume_tally.cpp:38-91):
+// the slab lengths along the step's line to the box's planes (an axis along
+// which the direction is below EPSILON constrains nothing), entering length
+// = the largest of the near ones, exiting = the smallest of the far ones.
+enum { kMisses = 0, kLandsInside = 1, kPassesThrough = 2, kStartsInsideExits = 3, kStartsInsideStays = 4 };
+int aacuboid_traversal(const double* mn, const double* mx, const V3& pos, const V3& dir, double fp) {
+  double lo[3], hi[3];
+  for (int i = 0; i < 3; ++i) {
+    const double di = dir[i];
+    if (std::abs(di) < EPSILON) {
+      lo[i] = -INF;
+      hi[i] = INF;
+      continue;
+    }
+    lo[i] = (mn[i] - pos[i]) / di;
+    hi[i] = (mx[i] - pos[i]) / di;
+    if (lo[i] > hi[i]) std::swap(lo[i], hi[i]);
+  }
+  const double enter = std::max(lo[0], std::max(lo[1], lo[2]));
+  const double exit = std::min(hi[0], std::min(hi[1], hi[2]));
+  if (enter > exit || exit < 0 || enter > fp) return kMisses;
+  if (enter <= 0) return exit >= fp ? kStartsInsideStays : kStartsInsideExits;
+  return exit < fp ? kPassesThrough : kLandsInside;
+}
+
 struct Engine {
   Oracle& O;
   explicit Engine(Oracle& o) : O(o) {}
@@ -427,30 +452,11 @@ struct Engine {
     }
   }
 
-  void volume(const TallyDesc& T, TallyAcc& A, TempVolume d) {  // volume_tally.cpp:5-91
-    V3 mn(T.p0[0], T.p0[1], T.p0[2]), mx(T.p1[0], T.p1[1], T.p1[2]);
-    double lo[3], hi[3];
-    for (int i = 0; i < 3; ++i) {
-      double di = d.initial.dir[i];
-      if (std::abs(di) < EPSILON) {
-        lo[i] = -INF;
-        hi[i] = INF;
-        continue;
-      }
-      lo[i] = (mn[i] - d.initial.pos[i]) / di;
-      hi[i] = (mx[i] - d.initial.pos[i]) / di;
-      if (lo[i] > hi[i]) std::swap(lo[i], hi[i]);
-    }
-    double enter = std::max(lo[0], std::max(lo[1], lo[2]));
-    double exit = std::min(hi[0], std::min(hi[1], hi[2]));
-    double fp = d.free_path;
-    if (enter > exit || exit < 0 || enter > fp) return;
-    if (enter <= 0) {
-      d.already_counted = true;
-      if (!(exit >= fp)) d.energy_deposited = 0;
-    } else if (exit < fp) {
-      d.energy_deposited = 0;
-    }
+  void volume(const TallyDesc& T, TallyAcc& A, TempVolume d) {  // volume_tally.cpp:5-27
+    const int tr = aacuboid_traversal(T.p0, T.p1, d.initial.pos, d.initial.dir, d.free_path);
+    if (tr == kMisses) return;
+    if (tr == kStartsInsideExits || tr == kStartsInsideStays) d.already_counted = true;
+    if (tr == kPassesThrough || tr == kStartsInsideExits) d.energy_deposited = 0;
     int b = bucket(d.final_);
     if (d.energy_deposited != 0) event(A.acc, 0, b, d.energy_deposited);
     if (!d.already_counted && d.initial.E != 0) event(A.acc, 1, b, d.initial.E);
@@ -725,6 +731,35 @@ void orc_kahn_sample(void* h, int id, double E, int n, uint64_t seed, double* mu
     Ep[i] = ph.E;
   }
 }
+
+// The traversal class of n steps (position, direction, free path) against
+// the box [mn, mx]: 0 misses, 1 lands inside, 2 passes through, 3 starts
+// inside and exits, 4 starts inside and stays.
+void orc_aacuboid_traversal(const double* mn, const double* mx, int n, const double* pos, const double* dir,
+                            const double* fp, int* out) {
+  for (int i = 0; i < n; ++i)
+    out[i] = aacuboid_traversal(mn, mx, V3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]),
+                                V3(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]), fp[i]);
+}
+
+// n source photons (PhotonSource::generatePhoton, source.cpp:150-155) from
+// one generator seeded with seed: directions (3 per photon) and energies.
+void orc_sample_source(void* h, int n, uint64_t seed, double* dir, double* E) {
+  Oracle& O = *static_cast<Oracle*>(h);
+  Rng rng;
+  rng.gen.seed((uint32_t)(seed * 1000003ull + 1ull));
+  Engine eng(O);
+  for (int i = 0; i < n; ++i) {
+    const Photon p = eng.generate(rng);
+    dir[3 * i] = p.dir.x;
+    dir[3 * i + 1] = p.dir.y;
+    dir[3 * i + 2] = p.dir.z;
+    E[i] = p.E;
+  }
+}
+
+// The OpenMP team size a run with threads = 0 uses.
+int orc_max_threads() { return omp_get_max_threads(); }
 
 void orc_rotate(const double* d, double theta, double phi, double* out) {
   V3 r = rotate(V3(d[0], d[1], d[2]), theta, phi);

@@ -128,3 +128,21 @@ def check_macro_table(ctx, ids, materials, background):
                 if r < 255:  # one more cell would hold a material c does not bound
                     assert not bounds[c][held(lo - 1, hi + 1)].all(), (k, j, i, r)
     return local
+
+
+def uniform_chi2_p(x, lo, hi, bins=40):
+    """p-value of a chi^2 test that x is uniform on [lo, hi] (bins equal bins)."""
+    counts, _ = np.histogram(x, np.linspace(lo, hi, bins + 1))
+    e = len(x) / bins
+    return float(stats.chi2.sf(np.sum((counts - e) ** 2 / e), bins - 1))
+
+
+def disc_basis(normal):
+    """DiscIsotropicDirectionality::setUAndV (source.cpp:117-128): u = n x x^
+    (n x y^ when n is perpendicular to x^), v = n x u, both normalised."""
+    n = np.asarray(normal, float)
+    x, y = np.array([1.0, 0, 0]), np.array([0, 1.0, 0])
+    u = np.cross(n, y) if abs(np.dot(n, x)) < 1e-9 else np.cross(n, x)
+    u /= np.linalg.norm(u)
+    v = np.cross(n, u)
+    return u, v / np.linalg.norm(v)

@@ -174,6 +174,41 @@ def test_interaction_type_bit_exact(ctx):
     assert np.array_equal(got, expect)
 
 
+def test_delta_decision_bit_exact(ctx):
+    """physics_engine.cpp:180-183 as both transport kernels take it: real
+    collision unless u < 1 - sigma_tot / sigma_max, the quotient and the
+    difference in IEEE FP32, with the uniforms the device draws (24-bit
+    grid points in (0, 1]) and u placed exactly on the threshold and one
+    FP32 step either side of it; a step cut at a local majorant's bound
+    takes no collision test."""
+    rng = np.random.default_rng(9)
+    n = 400000
+    sm = np.exp(rng.uniform(np.log(0.05), np.log(200.0), n)).astype(np.float32)
+    st = (sm * rng.uniform(0.0, 1.0, n)).astype(np.float32)
+    st[:1000] = sm[:1000]  # sigma_tot equal to the majorant: never a delta collision
+    u = ((rng.integers(0, 1 << 24, n) + 1) / float(1 << 24)).astype(np.float32)
+    thr = np.float32(1.0) - st / sm  # numpy float32: correctly rounded quotient and difference
+    assert thr.dtype == np.float32
+    q = n // 8
+    u[q:2 * q] = thr[q:2 * q]
+    u[2 * q:3 * q] = np.nextafter(thr[2 * q:3 * q], np.float32(0))
+    u[3 * q:4 * q] = np.nextafter(thr[3 * q:4 * q], np.float32(2))
+    limited = np.zeros(n, np.int32)
+    limited[4 * q:5 * q] = 1
+    expect = ((~(u < thr)) & (limited == 0)).astype(np.int32)
+    got = ctx.delta_decision(st, sm, u, limited)
+    assert np.array_equal(got, expect), np.nonzero(got != expect)[0][:10]
+    assert got[q:2 * q].all() and not got[2 * q:3 * q].any()  # u == thr is real; one step below is delta
+    assert not got[4 * q:5 * q].any() and got[:1000].all()
+    # against the FP64 decision on the same FP32 inputs: they differ only
+    # where u lies within the FP32 rounding of the threshold
+    thr64 = 1.0 - st.astype(np.float64) / sm.astype(np.float64)
+    rnd = slice(4 * q + q, n)
+    real64 = ~(u[rnd].astype(np.float64) < thr64[rnd])
+    differ = real64 != got[rnd].astype(bool)
+    assert np.all(np.abs(u[rnd][differ] - thr64[rnd][differ]) <= 2 * np.spacing(np.float32(1.0)))
+
+
 def kn_s(m, E, mu):
     k = E / T.ELECTRON_REST_MASS
     r = 1.0 / (1 + k * (1 - mu))

@@ -206,6 +206,33 @@ def test_alias_domain_matches_own_copy():
     ctx.close()
 
 
+def test_autotune_on_aliased_job_clears_shared_dose():
+    """The autotuner's probes on a job that uses another job's phantom
+    deposit in that phantom's voxels and background voxel: after it, the
+    owner's voxel dose (sums and counts) and background dose read zero, as
+    do the tuned job's own tallies."""
+    ctx = M.Context(0)
+    ctx.build_tables(list(scenes.C2_MATERIALS))
+    kw = dict(shape=(32, 32, 24), voxel_cm=0.5, seed=7, image=(16, 16))
+    scenes.setup_radiography(ctx, **kw)  # job 0 owns the phantom
+    view = ctx.job_create()
+    ctx.job_alias_domain(0)
+    scenes.setup_radiography(ctx, **kw, with_domain=False)
+    ctx.run_jobs([view], [50000], [0], [3])
+    ctx.job_select(0)
+    assert ctx.voxel_dose(0)[0].sum() > 0
+    ctx.job_select(view)
+    ctx.autotune(probe_histories=20000)
+    assert not ctx.tally(0)["image"].any()
+    s, s2, n = ctx.voxel_dose(0)
+    assert not s.any() and not s2.any() and not n.any()
+    assert ctx.background_dose() == (0.0, 0.0, 0.0)
+    ctx.job_select(0)
+    assert not ctx.voxel_dose(0)[0].any()
+    ctx.job_release(view)
+    ctx.close()
+
+
 def test_free_flights_match_majorant_steps():
     """Background free flights change only the number of rejected steps: the
     image, dose and interaction counts of a radiography job agree with the

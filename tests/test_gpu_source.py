@@ -50,3 +50,49 @@ def test_spectrum_sampling_chi2(kernel, nb):
     chi2 = float(np.sum((h[keep] - exp) ** 2 / exp))
     pval = stats.chi2.sf(chi2, df=int(keep.sum()) - 1)
     assert pval > 0.01, (pval, chi2, keep.sum())
+
+
+def test_device_source_directions_closed_forms():
+    """The device's generate() (source.cpp:64-115) against the closed forms
+    the oracle's samplers are pinned to: isotropic theta ~ U[0, pi] and phi
+    ~ U[0, 2 pi] (Q4); disc aim point r ~ U[0, R] and angle uniform (Q5);
+    rectangle aim point a1, a2 ~ U[0, 1]; beam along the pass-through
+    point.  Directions are unit vectors to FP32 rounding."""
+    from helpers import disc_basis, uniform_chi2_p
+    ctx = M.Context(0)
+    ctx.build_tables([scenes.AIR])
+    ctx.set_domain((40.0, 40.0, 40.0), scenes.AIR)
+    n = 400000
+    # isotropic
+    ctx.set_source(direction=M.DIR_ISOTROPIC, position=(20, 20, 20), energy=50e3)
+    d, e = ctx.sample_source(n, seed=3)
+    d = d.astype(np.float64)
+    assert np.all(np.abs(np.linalg.norm(d, axis=1) - 1) < 1e-6) and np.all(e == np.float32(50e3))
+    assert uniform_chi2_p(np.arccos(np.clip(d[:, 2], -1, 1)), 0, np.pi) > 1e-3
+    assert uniform_chi2_p(np.mod(np.arctan2(d[:, 1], d[:, 0]), 2 * np.pi), 0, 2 * np.pi) > 1e-3
+    # disc
+    P, C, nrm, R = np.array([20.0, 20.0, 1.0]), np.array([20.5, 19.5, 11.0]), np.array([0.0, 0.0, 1.0]), 0.7
+    ctx.set_source(direction=M.DIR_DISC_ISOTROPIC, position=P, a=C, b=nrm, radius=R, energy=30e3)
+    d, _ = ctx.sample_source(n, seed=4)
+    d = d.astype(np.float64)
+    u, v = disc_basis(nrm)
+    X = P + (((C - P) @ nrm) / (d @ nrm))[:, None] * d
+    r = np.linalg.norm(X - C, axis=1)
+    assert np.all(r <= R * (1 + 1e-5))
+    assert uniform_chi2_p(np.minimum(r, R), 0, R) > 1e-3
+    assert uniform_chi2_p(np.mod(np.arctan2((X - C) @ v, (X - C) @ u), 2 * np.pi), 0, 2 * np.pi) > 1e-3
+    # rectangle
+    P = np.array([21.0, 22.0, 30.0])
+    corner, e1, e2 = np.array([18.0, 18.0, 1.0]), np.array([4.0, 0.0, 0.0]), np.array([0.0, 3.0, 0.0])
+    ctx.set_source(direction=M.DIR_RECT_ISOTROPIC, position=P, a=corner, b=e1, c=e2, energy=30e3)
+    d, _ = ctx.sample_source(n, seed=5)
+    d = d.astype(np.float64)
+    X = P + ((corner[2] - P[2]) / d[:, 2])[:, None] * d
+    a1, a2 = (X - corner) @ e1 / 16.0, (X - corner) @ e2 / 9.0
+    assert uniform_chi2_p(np.clip(a1, 0, 1), 0, 1) > 1e-3 and uniform_chi2_p(np.clip(a2, 0, 1), 0, 1) > 1e-3
+    # beam
+    ctx.set_source(direction=M.DIR_BEAM, position=(20, 20, 0), a=(23, 24, 20), energy=30e3)
+    d, _ = ctx.sample_source(1000, seed=6)
+    want = np.array([3.0, 4.0, 20.0]) / np.linalg.norm([3.0, 4.0, 20.0])
+    assert np.allclose(d, want, atol=1e-6)
+    ctx.close()

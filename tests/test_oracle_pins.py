@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 from scipy import integrate
 
+from helpers import disc_basis, uniform_chi2_p
 import oracle
 from oracle import tables as T
 
@@ -308,3 +309,186 @@ def test_primary_transmission_closed_form():
     p = math.exp(-w.total_i(60e3) * 1.0 - a.total_i(60e3) * 99.0)
     got = sim.tally(1)["acc"][2, 0, 0] / n
     assert abs(got - p) < 4 * math.sqrt(p * (1 - p) / n)
+
+
+# --------------------------------------------------------------------- AACuboid traversal
+def _sat_segment_box(p, d, fp, mn, mx):
+    """Separating-axis test of the segment p + t d, t in [0, fp], against the
+    closed box [mn, mx] (an algorithm independent of the slab lengths the
+    reference uses): the three box axes and the three cross products of d
+    with them.  Returns (overlap, the smallest margin over the axes)."""
+    c = p + 0.5 * fp * d
+    h = 0.5 * fp * d
+    bc = 0.5 * (mn + mx)
+    e = 0.5 * (mx - mn)
+    r = c - bc
+    margins = list(e + np.abs(h) - np.abs(r))
+    for i in range(3):
+        ax = np.cross(d, np.eye(3)[i])
+        margins.append(float(np.sum(e * np.abs(ax)) - abs(np.dot(r, ax))))
+    m = min(margins)
+    return m >= 0, m
+
+
+def test_aacuboid_traversal_matches_geometry():
+    """volume_tally.cpp:38-91 classifies a step by where its segment lies
+    relative to the box; the classes are fixed by the geometry alone:
+    misses = the segment and the closed box are disjoint, starts inside /
+    lands inside = the start / end point in the box, passes through = both
+    ends outside and the segment crosses the box.  Random boxes and steps
+    (the oracle's class) against endpoint tests and a separating-axis test;
+    steps within 1e-7 of a face or an edge case are left out, where the two
+    computations may round differently."""
+    rng = np.random.default_rng(12)
+    n = 60000
+    mn = rng.uniform(-2, 0, (n, 3))
+    mx = mn + rng.uniform(0.1, 3, (n, 3))
+    p = rng.uniform(-4, 4, (n, 3))
+    half = n // 2  # half the steps start in the box
+    p[:half] = mn[:half] + rng.random((half, 3)) * (mx[:half] - mn[:half])
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    fp = rng.exponential(1.5, n)
+    # a quarter start outside aimed at a point of the box, reaching short of
+    # it, into it or beyond it
+    q = slice(half, half + n // 4)
+    tgt = mn[q] + rng.random((n // 4, 3)) * (mx[q] - mn[q])
+    d[q] = tgt - p[q]
+    dist = np.linalg.norm(d[q], axis=1)
+    d[q] /= dist[:, None]
+    fp[q] = dist * rng.uniform(0.3, 3.0, n // 4)
+    got = np.array([oracle.aacuboid_traversal(mn[i], mx[i], p[i:i + 1], d[i:i + 1], fp[i:i + 1])[0]
+                    for i in range(n)])
+    seen = np.zeros(5, int)
+    checked = 0
+    for i in range(n):
+        if np.min(np.abs(d[i])) < 1e-6:
+            continue
+        end = p[i] + fp[i] * d[i]
+        dist_s = np.min(np.concatenate([np.abs(p[i] - mn[i]), np.abs(p[i] - mx[i])]))
+        dist_e = np.min(np.concatenate([np.abs(end - mn[i]), np.abs(end - mx[i])]))
+        hit, margin = _sat_segment_box(p[i], d[i], fp[i], mn[i], mx[i])
+        if min(dist_s, dist_e, abs(margin)) < 1e-7:
+            continue
+        s_in = np.all((p[i] >= mn[i]) & (p[i] <= mx[i]))
+        e_in = np.all((end >= mn[i]) & (end <= mx[i]))
+        if not hit:
+            want = 0
+        elif s_in:
+            want = 4 if e_in else 3
+        else:
+            want = 1 if e_in else 2
+        assert got[i] == want, (i, got[i], want)
+        seen[want] += 1
+        checked += 1
+    assert checked > 0.95 * n and np.all(seen > 1000), seen
+
+
+def test_aacuboid_traversal_parallel_axis_quirk():
+    """An axis along which |d_i| < 1e-9 constrains nothing
+    (volume_tally.cpp:75-79): a step parallel to the x planes but beside the
+    box in x is still classified by y and z alone, as the reference does."""
+    mn, mx = (0.0, 0.0, 0.0), (1.0, 1.0, 1.0)
+    p = [[5.0, 0.5, -1.0], [0.5, 0.5, -1.0], [5.0, 0.5, 0.5]]
+    d = [[0.0, 0.0, 1.0]] * 3
+    got = oracle.aacuboid_traversal(mn, mx, p, d, [3.0, 3.0, 0.1])
+    assert list(got) == [2, 2, 4]
+
+
+def test_volume_deposit_equals_voxel_dose_in_box():
+    """Per history, EnergyDeposition of a volume tally counts the deposits
+    of the steps that land inside or start and stay inside its box, i.e.
+    the deposits at interaction points in the box; the voxel dose holds the
+    same deposits by voxel.  For boxes on voxel boundaries the two sums agree
+    to FP64 summation order, for every history set; a misclassified step
+    (a deposit counted when passing through or after exiting, or a landing
+    step missed) would break the equality."""
+    sim = oracle.Simulation([3, 48, 169])
+    sim.set_domain((4.0, 4.0, 30.0), 3)
+    ids = np.full((40, 40, 20), 48, np.uint8)
+    ids[10:30, 5:20, 4:12] = 169
+    sim.add_grid(ids, (0.1, 0.1, 0.1), (0, 0, 10))
+    boxes = [((0, 0, 10), (4, 4, 12), (slice(0, 40), slice(0, 40), slice(0, 20))),
+             ((1.0, 0.5, 10.4), (3.0, 2.0, 11.2), (slice(10, 30), slice(5, 20), slice(4, 12))),
+             ((0.3, 2.2, 11.5), (3.9, 3.3, 11.9), (slice(3, 39), slice(22, 33), slice(15, 19)))]
+    sim.set_source(direction=1, position=(2, 2, 11), energy=60e3)  # isotropic, inside the grid
+    sim.set_tallies([dict(kind=0, p0=b[0], p1=b[1]) for b in boxes])
+    sim.enable_voxel_dose(True)
+    sim.run(40000, seed=21, threads=4)
+    dose = sim.voxel_dose(0)[0].reshape(20, 40, 40)  # [z][y][x]
+    for t, (_, _, (sx, sy, sz)) in enumerate(boxes):
+        vol = sim.tally(t)["acc"][0, :, 1].sum()  # EnergyDeposition, all buckets
+        vox = dose[sz, sy, sx].sum()
+        assert vox > 0 and abs(vol - vox) <= 1e-12 * vox, (t, vol, vox)
+
+
+# --------------------------------------------------------------------- source samplers
+def test_isotropic_source_theta_uniform():
+    """IsotropicDirectionality (source.cpp:64-70): theta ~ U[0, pi] and phi ~
+    U[0, 2 pi] (Q4: not uniform on the sphere); a sampler uniform in
+    cos theta fails the theta test."""
+    sim = oracle.Simulation([3])
+    sim.set_source(direction=1, position=(1, 1, 1), energy=50e3)
+    d, e = sim.sample_source(200000, seed=3)
+    assert np.allclose(np.linalg.norm(d, axis=1), 1, atol=1e-12) and np.all(e == 50e3)
+    th = np.arccos(np.clip(d[:, 2], -1, 1))
+    ph = np.mod(np.arctan2(d[:, 1], d[:, 0]), 2 * np.pi)
+    assert uniform_chi2_p(th, 0, np.pi) > 1e-3
+    assert uniform_chi2_p(ph, 0, 2 * np.pi) > 1e-3
+    sphere = np.arccos(np.random.default_rng(1).uniform(-1, 1, 200000))
+    assert uniform_chi2_p(sphere, 0, np.pi) < 1e-10  # the test tells the two apart
+
+
+def test_disc_source_radius_uniform():
+    """DiscIsotropicDirectionality (source.cpp:108-115): the aim point on the
+    disc has r ~ U[0, R] (Q5: not area-uniform) and angle ~ U[0, 2 pi]; the
+    point is recovered by intersecting each direction with the disc's
+    plane.  An area-uniform sampler (r = R sqrt(u)) fails the r test."""
+    P, C, nrm, R = np.array([2.0, 2.0, 0.0]), np.array([2.5, 1.5, 10.0]), np.array([0.0, 0.0, 1.0]), 0.7
+    u, v = disc_basis(nrm)
+    sim = oracle.Simulation([3])
+    sim.set_source(direction=3, position=P, a=C, b=nrm, radius=R, energy=30e3)  # b: the disc normal
+    d, _ = sim.sample_source(200000, seed=4)
+    t = ((C - P) @ nrm) / (d @ nrm)
+    X = P + t[:, None] * d
+    r = np.linalg.norm(X - C, axis=1)
+    ang = np.mod(np.arctan2((X - C) @ v, (X - C) @ u), 2 * np.pi)
+    assert np.all(r <= R * (1 + 1e-9))
+    assert uniform_chi2_p(r, 0, R) > 1e-3
+    assert uniform_chi2_p(ang, 0, 2 * np.pi) > 1e-3
+    area = R * np.sqrt(np.random.default_rng(2).random(200000))
+    assert uniform_chi2_p(area, 0, R) < 1e-10
+
+
+def test_rect_source_aim_point_uniform():
+    """RectangularIsotropicDirectionality (source.cpp:85-90): the aim point
+    corner + a1 edge1 + a2 edge2 with a1, a2 ~ U[0, 1]."""
+    P = np.array([1.0, 2.0, 30.0])
+    corner, e1, e2 = np.array([0.0, 0.0, 0.0]), np.array([4.0, 0.0, 0.0]), np.array([0.0, 3.0, 0.0])
+    sim = oracle.Simulation([3])
+    sim.set_source(direction=2, position=P, a=corner, b=e1, c=e2, energy=30e3)
+    d, _ = sim.sample_source(200000, seed=5)
+    t = (corner[2] - P[2]) / d[:, 2]
+    X = P + t[:, None] * d
+    a1, a2 = (X - corner) @ e1 / 16.0, (X - corner) @ e2 / 9.0
+    assert uniform_chi2_p(a1, 0, 1) > 1e-3 and uniform_chi2_p(a2, 0, 1) > 1e-3
+    assert abs(np.corrcoef(a1, a2)[0, 1]) < 0.01
+
+
+def test_poly_spectrum_sampling():
+    """DiscreteInversion::sample (probability_dist.cpp:18-28): bin k with
+    probability w_k / sum w; zero-weight bins never drawn."""
+    e = np.arange(20.0, 60.0) * 1e3
+    w = np.random.default_rng(6).uniform(0, 1, e.size) ** 2
+    w[[3, 17]] = 0.0
+    sim = oracle.Simulation([3])
+    sim.set_source(direction=0, position=(0, 0, 0), a=(0, 0, 1), spectrum=(e, w))
+    _, E = sim.sample_source(300000, seed=6)
+    k = np.rint(E / 1e3 - 20).astype(int)
+    counts = np.bincount(k, minlength=e.size)
+    assert counts[3] == 0 and counts[17] == 0
+    p = w / w.sum()
+    keep = p > 0
+    from scipy import stats
+    chi2 = np.sum((counts[keep] - p[keep] * len(E)) ** 2 / (p[keep] * len(E)))
+    assert stats.chi2.sf(chi2, keep.sum() - 1) > 1e-3

@@ -48,7 +48,9 @@ class FakeCtx:
 
     def add_grid(self, ids, spacing, origin):
         self.slots[self.cur]["shape"] = ids.shape
-        self.bytes[self.cur] = ids.size * 21 + 24 * 128 * 128 + (1 << 16)
+        # the library's layout: IDs, 32-byte FP64 voxel records, macro cells
+        cells = int(np.prod([-(-n // 8) for n in ids.shape]))
+        self.bytes[self.cur] = ids.size * 33 + 2 * cells + 24 * 128 * 128 + (1 << 16)
         self.resident += self.bytes[self.cur]
         self.peak = max(self.peak, self.resident)
 
@@ -78,7 +80,11 @@ class FakeCtx:
         return np.ones(n), np.ones(n), None
 
     def stage_voxel_dose_f32(self, g, n_histories):
-        self.slots[self.cur]["staged"] = True
+        # the staged FP32 pairs are device memory until the copy is done
+        n = int(np.prod(self.slots[self.cur]["shape"]))
+        self.slots[self.cur]["staged"] = 8 * n
+        self.resident += 8 * n
+        self.peak = max(self.peak, self.resident)
 
     def read_staged_dose_f32_async(self, out):
         assert self.slots[self.cur].get("staged")
@@ -89,6 +95,9 @@ class FakeCtx:
         if wait:
             s["copy"] = 0
         s["copy"] = max(0, s.get("copy", 0) - 1)
+        if s["copy"] == 0 and s.get("staged"):
+            self.resident -= s["staged"]
+            s["staged"] = 0
         return s["copy"] == 0
 
     def voxel_dose_f32(self, g, n_histories, out=None):
@@ -143,6 +152,23 @@ def test_admission_respects_budget_and_order():
     adm = sorted(range(len(jobs)), key=lambda i: (res[i].admitted, i))
     assert adm == sorted(adm)  # first come, first admitted (ties in index order)
     assert ctx.resident == 0
+
+
+def test_budget_holds_with_staged_reads_in_flight():
+    """Jobs whose FP32 dose is staged and being copied still hold device
+    memory: with a budget of two jobs' footprints (staging included) the
+    peak never exceeds it, while a budget that leaves the staging out would
+    be overrun by the same stream."""
+    ctx = FakeCtx()
+    jobs = [dict(j, shape=(16, 16, 16)) for j in small_jobs(16, 4)]
+    full = scenes.dose_job_bytes(jobs[0], "f32")
+    budget = 2 * full
+    srv = Server(ctx, budget_bytes=budget, round_histories=50_000, min_slice=5_000, on_result=None)
+    srv.run(jobs)
+    assert ctx.peak <= budget, (ctx.peak, budget)
+    assert ctx.resident == 0
+    no_stage = scenes.dose_job_bytes(jobs[0], "f64")
+    assert full - no_stage == 8 * 16 ** 3 and ctx.peak > 2 * no_stage
 
 
 def test_round_plan_shares_equally():
