@@ -227,6 +227,18 @@ int mdx_tables_device_bytes(mdx_ctx* ctx, uint64_t* bytes);
  * its job slot, so give different jobs different seeds.  Blocking. */
 int mdx_run_jobs(mdx_ctx* ctx, int n_jobs, const int* jobs, const uint64_t* first, const uint64_t* n,
                  const uint64_t* seed, mdx_stats* stats);
+/* mdx_run_jobs that allocates nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory while they run transport
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The launch's job records, prefix offsets, work counters and
+ * event counters live in a static device array of the library (about 19 KB
+ * per device, held until the process ends) instead of context buffers, so at
+ * most 256 jobs per call (more: MDX_ERR_INVALID).  Every job must be set up
+ * before (domain, source, tallies; also the tables), and results are the
+ * same as mdx_run_jobs's for the same arguments.  Calls that overlap on one
+ * device share that array and race: serialise them.  Blocking. */
+int mdx_run_jobs_noalloc(mdx_ctx* ctx, int n_jobs, const int* jobs, const uint64_t* first, const uint64_t* n,
+                         const uint64_t* seed, mdx_stats* stats);
 /* Background free flights for the selected job's queue-kernel runs (default
  * on).  Through the background material outside every grid the photon flies
  * with that material's own cross section instead of the global majorant

@@ -46,7 +46,7 @@ EXPORTS = [
     "mdx_set_flights", "mdx_set_rita_builder", "mdx_set_dose_precision", "mdx_get_voxel_dose_f32", "mdx_stage_voxel_dose_f32",
     "mdx_read_staged_dose_f32_async", "mdx_read_done", "mdx_copy_probe", "mdx_air_muen", "mdx_set_air_muen_table",
     "mdx_set_local_majorants", "mdx_get_macro_cells", "mdx_delta_decision", "mdx_philox",
-    "mdx_sample_source",
+    "mdx_sample_source", "mdx_run_jobs_noalloc",
 ]
 
 
@@ -452,6 +452,22 @@ class Context:
         ss = np.ascontiguousarray(seeds if seeds is not None else np.ones(k), dtype=np.uint64)
         st = _Stats()
         self._call(self.lib.mdx_run_jobs, self._h, ctypes.c_int(k), _ptr(ids, ctypes.c_int),
+                   _ptr(ff, ctypes.c_uint64), _ptr(nn, ctypes.c_uint64), _ptr(ss, ctypes.c_uint64), ctypes.byref(st))
+        out = {name: int(st.counters[i]) for i, name in enumerate(STAT_NAMES)}
+        out["kernel_ms"] = st.kernel_ms
+        return out
+
+    def run_jobs_noalloc(self, jobs, n, first=None, seeds=None):
+        """run_jobs without device allocation (mdx_run_jobs_noalloc): the
+        launch's records and counters in the library's static scratch; at
+        most 256 jobs."""
+        k = len(jobs)
+        ids = np.ascontiguousarray(jobs, dtype=np.int32)
+        nn = np.ascontiguousarray(n, dtype=np.uint64)
+        ff = np.ascontiguousarray(first if first is not None else np.zeros(k), dtype=np.uint64)
+        ss = np.ascontiguousarray(seeds if seeds is not None else np.ones(k), dtype=np.uint64)
+        st = _Stats()
+        self._call(self.lib.mdx_run_jobs_noalloc, self._h, ctypes.c_int(k), _ptr(ids, ctypes.c_int),
                    _ptr(ff, ctypes.c_uint64), _ptr(nn, ctypes.c_uint64), _ptr(ss, ctypes.c_uint64), ctypes.byref(st))
         out = {name: int(st.counters[i]) for i, name in enumerate(STAT_NAMES)}
         out["kernel_ms"] = st.kernel_ms

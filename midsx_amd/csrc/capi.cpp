@@ -287,6 +287,8 @@ struct mdx_ctx {
   }
 
   DBuf d_counters;
+  // this device's static scratch (mdx_run_jobs_noalloc), looked up at creation
+  void* scratch = nullptr;
   // queue-kernel launch parameters (mdx_set_queue_params)
   int queue_blocks_per_cu = 4;
   int refill_min = 32;
@@ -1063,6 +1065,7 @@ int mdx_create(int device, mdx_ctx** out) {
     c->lds_optin = std::max(prop.sharedMemPerBlockOptin, prop.sharedMemPerBlock);
     c->d_counters.alloc(MDX_NCOUNTER * sizeof(unsigned long long));
     c->d_work.alloc(kRoundMax * sizeof(unsigned long long));
+    hip_check(mdx_launch::scratch_address(device, &c->scratch), "hipGetSymbolAddress(scratch)");
     *out = c.release();
   });
 }
@@ -1734,6 +1737,93 @@ int mdx_run_jobs(mdx_ctx* c, int n_jobs, const int* jobs, const uint64_t* first,
   return guard([&] {
     need(c && n_jobs > 0 && jobs && first && n && seed, "bad argument");
     run_jobs(c, n_jobs, jobs, first, n, seed, stats);
+  });
+}
+
+// mdx_run_jobs without allocating: the job records, prefix offsets, work
+// counters and event counters of the launch go to this device's static
+// scratch instead of context buffers; every job must be fully set up
+// (domain, source, tallies) before.  No device or host heap allocation on
+// this path: records are built on the stack, queued clears are flushed in
+// place, statistics are read into a local array.
+int mdx_run_jobs_noalloc(mdx_ctx* c, int n_jobs, const int* ids, const uint64_t* first, const uint64_t* n,
+                         const uint64_t* seed, mdx_stats* stats) {
+  return guard([&] {
+    need(c && ids && first && n && seed, "NULL argument");
+    need(n_jobs >= 1 && n_jobs <= MDX_NOALLOC_MAX_JOBS,
+         "n_jobs must be in [1, 256]: the static scratch holds 256 job records");
+    need(c->device >= 0 && c->scratch, "host-only context (device -1) cannot run device work");
+    need(c->tables_ready, "physics tables not built (mdx_build_tables)");
+    need(!c->inflight, "a round is in flight (mdx_wait first)");
+    set_device(c);
+    mdx_job_dev jd[MDX_NOALLOC_MAX_JOBS];
+    uint64_t prefix[MDX_NOALLOC_MAX_JOBS + 1];
+    prefix[0] = 0;
+    uint32_t lds_voxels = 0;
+    {
+      KeepCur kc(c);
+      for (int i = 0; i < n_jobs; ++i) {
+        job_at(c, ids[i]);
+        c->cur = domain_owner(c, ids[i]);
+        check_materials(c);
+        const Job& O = c->J();
+        need(O.d_domain.p != nullptr, "domain not uploaded");
+        if (n_jobs == 1 && O.voxel_dose && O.dose_mode == MDX_DOSE_F32C_LDS && !O.grids.empty())
+          lds_voxels = (uint32_t)O.grids[0]->voxels();
+        jd[i].D = O.d_domain.as<mdx_domain_dev>();
+        c->cur = ids[i];
+        need(c->J().source_set, "source not set (mdx_set_source)");
+        need(c->J().d_tallies.p != nullptr, "tallies not set (mdx_set_tallies)");
+        jd[i].S = c->J().d_source.as<mdx_source_dev>();
+        jd[i].TL = c->J().d_tallies.as<mdx_tallies_dev>();
+        jd[i].first = first[i];
+        jd[i].n = n[i];
+        jd[i].seed = seed[i];
+        jd[i].stream = 0;
+        bool volume = false;
+        for (auto& t : c->J().tallies) volume = volume || t->t.kind == MDX_TALLY_AACUBOID;
+        jd[i].flights = c->J().flights && !volume ? 1u : 0u;
+        prefix[i + 1] = prefix[i] + n[i];
+      }
+    }
+    // clears queued by job set-up while a graph round ran, done in place
+    {
+      std::lock_guard<std::mutex> lk(g_clear_mu);
+      for (auto& r : g_clears)
+        if (r.ctx == c) clear_now(r.p, r.bytes, c->stream);
+      g_clears.erase(std::remove_if(g_clears.begin(), g_clears.end(), [&](const ClearReq& r) { return r.ctx == c; }),
+                     g_clears.end());
+    }
+    const mdx_launch::ScratchLayout L = mdx_launch::scratch_layout();
+    char* base = static_cast<char*>(c->scratch);
+    auto* cnt = reinterpret_cast<unsigned long long*>(base + L.counters);
+    auto* work = reinterpret_cast<unsigned long long*>(base + L.work);
+    auto* dprefix = reinterpret_cast<uint64_t*>(base + L.prefix);
+    auto* djobs = reinterpret_cast<mdx_job_dev*>(base + L.jobs);
+    hip_check(hipMemcpyAsync(djobs, jd, n_jobs * sizeof(mdx_job_dev), hipMemcpyHostToDevice, c->stream), "jobs");
+    hip_check(hipMemcpyAsync(dprefix, prefix, (n_jobs + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream),
+              "prefix");
+    // the scratch keeps the last call's contents: the counters start at zero
+    hip_check(mdx_launch::zero_u64(cnt, MDX_NCOUNTER, c->stream), "zero counters");
+    hip_check(mdx_launch::zero_u64(work, n_jobs, c->stream), "zero work");
+    const uint64_t total = prefix[n_jobs];
+    hip_check(hipEventRecord(c->ev0, c->stream), "hipEventRecord");
+    if (total > 0)
+      hip_check(mdx_launch::transport_queue(c->d_tables.as<mdx_tables_dev>(), djobs, dprefix, n_jobs, total, work, cnt,
+                                            c->num_cu * c->queue_blocks_per_cu, c->refill_min, c->interact_min,
+                                            lds_voxels, c->stream),
+                "queue launch");
+    hip_check(hipEventRecord(c->ev1, c->stream), "hipEventRecord");
+    unsigned long long h[MDX_NCOUNTER];
+    hip_check(hipMemcpyAsync(h, cnt, sizeof h, hipMemcpyDeviceToHost, c->stream), "counters");
+    hip_check(hipStreamSynchronize(c->stream), "transport");
+    if (stats) {
+      std::memset(stats, 0, sizeof *stats);
+      for (int i = 0; i < MDX_NCOUNTER && i < MDX_STAT_NCOUNTERS; ++i) stats->counters[i] = h[i];
+      float ms = 0;
+      hip_check(hipEventElapsedTime(&ms, c->ev0, c->ev1), "hipEventElapsedTime");
+      stats->kernel_ms = ms;
+    }
   });
 }
 

@@ -6,6 +6,16 @@
 
 namespace mdx_launch {
 
+// Byte offsets of the regions of the static scratch (mdx_run_jobs_noalloc):
+// event counters, per-job work counters, prefix offsets, job records; total
+// size.  At most MDX_NOALLOC_MAX_JOBS jobs.
+struct ScratchLayout {
+  size_t counters, work, prefix, jobs, bytes;
+};
+ScratchLayout scratch_layout();
+// The calling thread's current device must be `device`.
+hipError_t scratch_address(int device, void** out);
+
 hipError_t transport_plain(const mdx_tables_dev* T, const mdx_domain_dev* D, const mdx_source_dev* S,
                            const mdx_tallies_dev* TL, uint64_t first, uint64_t n, uint64_t seed, uint32_t job,
                            unsigned long long* counters, int grid_blocks, hipStream_t st);

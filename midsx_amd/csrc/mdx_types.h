@@ -198,6 +198,11 @@ struct mdx_job_dev {
   uint32_t flights;  // 1: free flights through the background (no volume tallies)
 };
 
+// mdx_run_jobs_noalloc: job records, prefix offsets, work counters and event
+// counters of one launch live in a static device array (see transport.hip),
+// which holds at most this many jobs.
+#define MDX_NOALLOC_MAX_JOBS 256
+
 // Event counters, one int64 each; indexes into the counter array.
 #define MDX_C_HIST 0
 #define MDX_C_STEP 1

@@ -9,6 +9,8 @@
 // dose, images and spectra go straight to FP64 global atomics.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "device.h"
 #include "kernels.h"
 
@@ -1440,9 +1442,46 @@ __global__ void k_copy16(const float4* __restrict__ src, float4* __restrict__ ds
 
 inline unsigned blocks_for(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
+// Static scratch of mdx_run_jobs_noalloc: the launch's event counters, the
+// jobs' work counters, prefix offsets and job records, at the offsets of
+// mdx_launch::ScratchLayout (each region 256-byte aligned).  One copy per
+// device, held until the process ends; calls that overlap on one device
+// share it and race.
+constexpr size_t kAlign = 256;
+constexpr size_t round_up(size_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
+constexpr size_t kScratchCounters = 0;
+constexpr size_t kScratchWork = round_up(MDX_NCOUNTER * sizeof(unsigned long long));
+constexpr size_t kScratchPrefix = kScratchWork + round_up(MDX_NOALLOC_MAX_JOBS * sizeof(unsigned long long));
+constexpr size_t kScratchJobs = kScratchPrefix + round_up((MDX_NOALLOC_MAX_JOBS + 1) * sizeof(uint64_t));
+constexpr size_t kScratchBytes = kScratchJobs + round_up(MDX_NOALLOC_MAX_JOBS * sizeof(mdx_job_dev));
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 }  // namespace
 
 namespace mdx_launch {
+
+ScratchLayout scratch_layout() {
+  return ScratchLayout{kScratchCounters, kScratchWork, kScratchPrefix, kScratchJobs, kScratchBytes};
+}
+
+// The device address of this device's scratch array, looked up once per
+// device (with that device current) and kept: in host code the array's name
+// is a host-side stand-in, never a device address.
+hipError_t scratch_address(int device, void** out) {
+  static std::mutex mu;
+  static void* addr[64] = {nullptr};
+  if (device < 0 || device >= 64) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!addr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[device] = p;
+  }
+  *out = addr[device];
+  return hipSuccess;
+}
 
 hipError_t transport_plain(const mdx_tables_dev* T, const mdx_domain_dev* D, const mdx_source_dev* S,
                            const mdx_tallies_dev* TL, uint64_t first, uint64_t n, uint64_t seed, uint32_t job,
