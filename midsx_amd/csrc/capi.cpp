@@ -766,7 +766,7 @@ void build_and_upload(mdx_ctx* c, const std::vector<int>& ids) {
   c->d_nodes.upload(P.nodes.data(), P.nodes.size(), c->stream);
   c->d_hash.upload(P.hash.data(), P.hash.size(), c->stream);
   c->d_rE.upload(P.rita_E.data(), P.rita_E.size(), c->stream);
-  c->d_rinv.upload(P.rita_inv_dlnE.data(), P.rita_inv_dlnE.size(), c->stream);
+  c->d_rinv.upload(P.rita_escale.data(), P.rita_escale.size(), c->stream);
   c->d_roff.upload(P.rita_row_off.data(), P.rita_row_off.size(), c->stream);
   c->d_rrows.upload(P.rita_rows.data(), P.rita_rows.size(), c->stream);
   c->d_rhash.upload(P.rita_ehash.data(), P.rita_ehash.size(), c->stream);
@@ -776,9 +776,9 @@ void build_and_upload(mdx_ctx* c, const std::vector<int>& ids) {
   T.nodes = c->d_nodes.as<mdx_node>();
   T.hash = c->d_hash.as<int32_t>();
   T.rita_E = c->d_rE.as<float>();
-  T.rita_inv_dlnE = c->d_rinv.as<float>();
+  T.rita_escale = c->d_rinv.as<float>();
   T.rita_row_off = c->d_roff.as<int32_t>();
-  T.rita_rows = c->d_rrows.as<mdx_f4>();
+  T.rita_rows = c->d_rrows.as<mdx_rita_row>();
   T.rita_ehash = c->d_rhash.as<int32_t>();
   T.rita_guide = c->d_rguide.as<uint8_t>();
   T.majorant = P.majorant;

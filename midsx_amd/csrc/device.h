@@ -342,13 +342,29 @@ __device__ __forceinline__ void rotate(float& dx, float& dy, float& dz, float mu
 
 // ------------------------------------------------------------------ RITA
 // probability_dist.cpp:90-119 on one energy row, in w = 1 + x: the largest
-// node i with y_i < y (lower_bound - 1), then the rational interpolation on
-// [y_i, y_{i+1}].  The row's guide entry for y gives a start index with
-// y_i < y; the node pair (i, i + 1) brackets y unless the guide bin holds
-// further nodes, and a short walk follows then.
+// node i with y_i < y (lower_bound - 1, the CDF values taken in FP64), then
+// the rational interpolation on [y_i, y_{i+1}].  The row's guide entry for y
+// gives a start node with y_i < y; the next node's CDF value, loaded beside
+// the start node, tells whether it brackets y, and a short walk follows when
+// it does not.
+__device__ __forceinline__ mdx_rita_row gld(const mdx_rita_row* p) {
+  const mdx_v4 a = *(const MDX_GLOBAL mdx_v4*)p;
+  const mdx_v4 b = *(const MDX_GLOBAL mdx_v4*)((const MDX_GLOBAL float*)p + 4);
+  mdx_rita_row r;
+  r.w_hi = a.x; r.w_lo = a.y; r.y_hi = a.z; r.y_lo = a.w;
+  r.a = b.x; r.b = b.y; r.dw = b.z; r.d = b.w;
+  return r;
+}
+
+// y > y_k for the FP64 CDF value y_k = yh + yl (yh = y_k rounded to
+// nearest): exact for an FP32 y, since no FP32 value lies strictly between
+// yh and y_k.
+__device__ __forceinline__ bool rita_above(float y, float yh, float yl) { return y > yh || (y == yh && yl < 0.0f); }
+
 struct RitaAt {
   int i;
-  mdx_f4 a, b;  // nodes i and i + 1 (clamped to the row)
+  mdx_rita_row r;  // node i (clamped to the row)
+  float yn_hi, yn_lo;  // CDF value of node i + 1 (clamped)
 };
 
 __device__ __forceinline__ int rita_guide(const mdx_tables_dev* T, int grow, float y) {
@@ -358,34 +374,49 @@ __device__ __forceinline__ int rita_guide(const mdx_tables_dev* T, int grow, flo
 }
 
 __device__ __forceinline__ RitaAt rita_nodes(const mdx_tables_dev* T, int row0, int m, int i) {
-  const mdx_f4* R = T->rita_rows + row0;
+  const mdx_rita_row* R = T->rita_rows + row0;
   const int ia = i < 0 ? 0 : i, ib = i + 1 > m - 1 ? m - 1 : i + 1;
-  return RitaAt{i, gld(R + ia), gld(R + ib)};
+  const mdx_v2 yn = *(const MDX_GLOBAL mdx_v2*)(&R[ib].y_hi);
+  return RitaAt{i, gld(R + ia), yn.x, yn.y};
 }
 
-__device__ __forceinline__ float rita_finish(const mdx_tables_dev* T, int row0, int m, float y, RitaAt r) {
-  if (r.i + 1 < m && r.b.y < y) {  // the guide bin holds more nodes below y
-    const mdx_f4* R = T->rita_rows + row0;
+// Returns w as an unevaluated pair hi + lo (TwoSum of the last addition).
+__device__ __forceinline__ mdx_v2 rita_finish(const mdx_tables_dev* T, int row0, int m, float y, RitaAt r) {
+  if (r.i + 1 < m && rita_above(y, r.yn_hi, r.yn_lo)) {  // y lies beyond node i + 1: walk on
+    const mdx_rita_row* R = T->rita_rows + row0;
     int i = r.i + 1;
-    while (i + 1 < m && gld(&R[i + 1].y) < y) ++i;
+    while (i + 1 < m && rita_above(y, gld(&R[i + 1].y_hi), gld(&R[i + 1].y_lo))) ++i;
     r = rita_nodes(T, row0, m, i);
   }
-  if (r.i == m - 1) return 2.0f;
-  if (r.i < 0) return 0.0f;
-  const mdx_f4& a = r.a;
-  const mdx_f4& b = r.b;
-  const float nu = y - a.y;
-  const float d = b.y - a.y;
-  return a.x + (1.0f + a.z + a.w) * d * nu / (d * d + a.z * d * nu + a.w * nu * nu) * (b.x - a.x);
+  if (r.i == m - 1) return mdx_v2{2.0f, 0.0f};  // above the last CDF value: x_max (probability_dist.cpp:97-100)
+  if (r.i < 0) return mdx_v2{0.0f, 0.0f};       // x_min
+  // nu = y - y_i with y_i's low part, rho = nu / d; the interval's fraction
+  // (1 + a + b) d nu / (d^2 + a d nu + b nu^2) of dw, divided through by d^2
+  // (the CDF widths of high-energy rows are ~1e-26: d^2 would underflow),
+  // then w_i + fraction * dw with w_i's low part added before the last rounding
+  const mdx_rita_row& q = r.r;
+  const float rho = ((y - q.y_hi) - q.y_lo) / q.d;
+  const float t = (1.0f + q.a + q.b) * rho / (1.0f + rho * (q.a + q.b * rho));
+  const float c = __builtin_fmaf(t, q.dw, q.w_lo);
+  const float sum = __fadd_rn(q.w_hi, c);  // TwoSum: sum + err = w_hi + c exactly
+  const float bb = __fsub_rn(sum, q.w_hi);
+  const float err = __fadd_rn(__fsub_rn(q.w_hi, __fsub_rn(sum, bb)), __fsub_rn(c, bb));
+  return mdx_v2{sum, err};
 }
 
 // probability_dist.cpp:73-88: energy bracket by lower_bound - 1 on the
 // incoherent-CS grid (the largest i with E_i < E, clamped to the last
 // interval), x at both energies from the same y, then log-log interpolation
-// of (x + 1) in E.  The energy hash gives the bracket, which holds unless its
-// bin contains more grid energies; everything that depends only on the
-// bracket (the two energies, the row offsets, both rows' guide entries) is
-// loaded at once on that assumption and reloaded when the check fails.
+// of (x + 1) in E, here w = w_b + w_b expm1(g ln(w_u / w_l)) from the
+// nearer energy b (g = f or f - 1) with w_l and w_u as hi + lo pairs (their
+// low parts in the ratio and in the sum), so the result takes one rounding
+// beyond the small correction's.  The stored energies
+// are rounded down, so E > stored <=> E > E_i and the bracket is the FP64
+// one; f = (ln(E / stored) + ln(stored / E_i)) / ln(E_{i+1} / E_i).  The
+// energy hash gives the bracket, which holds unless its bin contains more
+// grid energies; everything that depends only on the bracket (the two
+// energies, the row offsets, both rows' guide entries) is loaded at once on
+// that assumption and reloaded when the check fails.
 __device__ __forceinline__ float rita_sample(const mdx_tables_dev* T, const mdx_material_dev& M, float E, float y) {
   const float* Eg = T->rita_E + M.rita_eoff;
   const int* ro = T->rita_row_off + M.rita_roff;
@@ -410,10 +441,19 @@ __device__ __forceinline__ float rita_sample(const mdx_tables_dev* T, const mdx_
     ju = rita_guide(T, M.rita_eoff + il + 1, y);
   }
   const int ml = r1 - r0, mu = r2 - r1;
-  const float wl = rita_finish(T, r0, ml, y, rita_nodes(T, r0, ml, jl));
-  const float wu = rita_finish(T, r1, mu, y, rita_nodes(T, r1, mu, ju));
-  const float f = log_ratio(E, e0) * gld(T->rita_inv_dlnE + M.rita_eoff + il);
-  const float w = (wl > 0.0f && wu > 0.0f) ? wl * expf(f * log_ratio(wu, wl)) : wl;
+  const mdx_v2 wl = rita_finish(T, r0, ml, y, rita_nodes(T, r0, ml, jl));
+  const mdx_v2 wu = rita_finish(T, r1, mu, y, rita_nodes(T, r1, mu, ju));
+  if (!(wl.x > 0.0f && wu.x > 0.0f)) return wl.x - 1.0f;
+  const mdx_v2 es = *(const MDX_GLOBAL mdx_v2*)(T->rita_escale + 2 * (M.rita_eoff + il));
+  const float f = (log_ratio(E, e0) + es.y) * es.x;
+  const float lr = log_ratio(wu.x, wl.x) + (wu.y * frcp(wu.x) - wl.y * frcp(wl.x));  // ln(w_u / w_l)
+  // from the nearer energy's w: w = w_u exp((f - 1) ln(w_u / w_l)) above the
+  // middle (f - 1 exact there), so that expm1's argument stays within half
+  // the interval's ln ratio and its last-place error stays small against w
+  const bool upper = f > 0.5f;
+  const mdx_v2 wb = upper ? wu : wl;
+  const float e = expm1f((upper ? f - 1.0f : f) * lr);
+  const float w = wb.x + __builtin_fmaf(wb.x, e, __builtin_fmaf(wb.y, e, wb.y));
   return w - 1.0f;
 }
 

@@ -57,7 +57,7 @@ struct mdx_curve {
 struct mdx_material_dev {
   mdx_curve total, coh, incoh, photo, sf;
   int32_t rita_ne;     // energies in the RITA grid
-  int32_t rita_eoff;   // offset into rita_E / rita_inv_dlnE (and row index for rita_guide)
+  int32_t rita_eoff;   // offset into rita_E (and row index for rita_guide; 2 floats each in rita_escale)
   int32_t rita_roff;   // offset into rita_row_off (rita_ne + 1 entries)
   int32_t material_id;
   int32_t rita_hoff;   // first entry of the energy hash in rita_ehash
@@ -70,16 +70,26 @@ struct alignas(16) mdx_f4 {
   float x, y, z, w;
 };
 
-// RITA row (PENELOPE inverse-CDF interpolation) stored as w = 1 + x so the
-// backward-scatter end keeps its precision: (w_i, y_i, a_i, b_i).
+// One node i of a RITA row (PENELOPE inverse-CDF interpolation,
+// probability_dist.cpp:90-119) in w = 1 + x, which keeps the backward end's
+// precision: w_i = w_hi + w_lo and the CDF value y_i = y_hi + y_lo as FP32
+// pairs (the FP64 values to about 2^-48; y_hi is y_i rounded to nearest, so
+// |y_lo| <= half its ulp), the interval's parameters a_i, b_i and widths
+// dw = w_{i+1} - w_i, d = y_{i+1} - y_i, each rounded once from FP64 (0 for
+// the row's last node).  32 bytes: one aligned sector per node.
+struct alignas(32) mdx_rita_row {
+  float w_hi, w_lo, y_hi, y_lo;
+  float a, b, dw, d;
+};
+
 struct mdx_tables_dev {
   const float* xs;          // node abscissae (all curves)
   const mdx_node* nodes;    // node records (all curves)
   const int32_t* hash;      // bracket hash (all curves)
-  const float* rita_E;      // RITA energies (all materials)
-  const float* rita_inv_dlnE;   // 1 / ln(E_{i+1}/E_i)
+  const float* rita_E;      // RITA energies (all materials), rounded down to FP32
+  const float* rita_escale; // per energy i: 1 / ln(E_{i+1}/E_i) and ln(rita_E[i] / E_i)
   const int32_t* rita_row_off;  // per energy: first row; one extra entry per material
-  const mdx_f4* rita_rows;
+  const mdx_rita_row* rita_rows;
   const int32_t* rita_ehash;    // energy hash (all materials)
   const uint8_t* rita_guide;    // MDX_RITA_GUIDE entries per energy row (all materials)
   mdx_curve majorant;

@@ -291,8 +291,14 @@ double Rita::x_from_y(int ie, double yu) const {
   return X[i] + (1 + ai + bi) * d * nu / (d * d + ai * d * nu + bi * nu * nu) * (X[i + 1] - X[i]);
 }
 
+// probability_dist.cpp:73-88.  The bracket lower_bound - 1 is clamped to the
+// first / last interval (the reference reads outside its arrays for an energy
+// at or below the grid's first energy or above its last, as its comment at
+// :76 warns; here the end interval's log-log line is extended, as the device
+// does).
 double Rita::sample(double E, double yu) const {
   int lo = (int)(std::lower_bound(energies.begin(), energies.end(), E) - energies.begin()) - 1;
+  lo = std::max(0, std::min(lo, (int)energies.size() - 2));
   int up = lo + 1;
   double xl = x_from_y(lo, yu), xu = x_from_y(up, yu);
   return std::exp(std::log(xl + 1) + (std::log(xu + 1) - std::log(xl + 1)) * (std::log(E) - std::log(energies[lo])) /
@@ -637,28 +643,40 @@ void pack_all(Packed& p, const std::vector<const MaterialTables*>& mats, const T
     d.rita_eoff = (int)p.rita_E.size();
     d.rita_roff = (int)p.rita_row_off.size();
     for (size_t i = 0; i < R.energies.size(); ++i) {
-      p.rita_E.push_back((float)R.energies[i]);
-      p.rita_inv_dlnE.push_back(i + 1 < R.energies.size()
-                                    ? (float)(1.0 / std::log(R.energies[i + 1] / R.energies[i]))
-                                    : 0.0f);
+      // energies rounded toward -inf: for an FP32 energy E, E > stored <=>
+      // E > E_i, so the device brackets as the FP64 lower_bound - 1 does;
+      // ln(stored / E_i) corrects the interpolation weight for the rounding
+      const double e = R.energies[i];
+      float e32 = (float)e;
+      if ((double)e32 > e) e32 = std::nextafter(e32, 0.0f);
+      p.rita_E.push_back(e32);
+      p.rita_escale.push_back(i + 1 < R.energies.size() ? (float)(1.0 / std::log(R.energies[i + 1] / e)) : 0.0f);
+      p.rita_escale.push_back((float)std::log((double)e32 / e));
       p.rita_row_off.push_back((int)p.rita_rows.size());
       if (R.x[i].size() > 255) throw std::runtime_error("a RITA row has more than 255 points");
-      std::vector<float> ys;
-      for (size_t r = 0; r < R.x[i].size(); ++r) {
-        mdx_f4 q;
-        q.x = (float)(1.0 + R.x[i][r]);
-        q.y = (float)R.y[i][r];
-        q.z = r < R.a[i].size() ? (float)R.a[i][r] : 0.0f;
-        q.w = r < R.b[i].size() ? (float)R.b[i][r] : 0.0f;
+      const size_t m = R.x[i].size();
+      for (size_t r = 0; r < m; ++r) {
+        mdx_rita_row q{};
+        const double w = 1.0 + R.x[i][r], y = R.y[i][r];
+        q.w_hi = (float)w;
+        q.w_lo = (float)(w - (double)q.w_hi);
+        q.y_hi = (float)y;
+        q.y_lo = (float)(y - (double)q.y_hi);
+        if (r + 1 < m) {
+          q.a = r < R.a[i].size() ? (float)R.a[i][r] : 0.0f;
+          q.b = r < R.b[i].size() ? (float)R.b[i][r] : 0.0f;
+          q.dw = (float)((1.0 + R.x[i][r + 1]) - w);
+          q.d = (float)(R.y[i][r + 1] - y);
+        }
         p.rita_rows.push_back(q);
-        ys.push_back(q.y);
       }
-      // guide: entry j = number of CDF values (as stored) below j / G; the
-      // device scales y by G (a power of two, so exactly) to find j
+      // guide: entry j = number of CDF values (FP64) below j / G; the
+      // device scales y by G (a power of two, so exactly) to find j, and the
+      // node before the entry has y_i < y
       for (int j = 0; j < MDX_RITA_GUIDE; ++j) {
-        const float t = (float)j / (float)MDX_RITA_GUIDE;
+        const double t = (double)j / (double)MDX_RITA_GUIDE;
         int cnt = 0;
-        while (cnt < (int)ys.size() && ys[cnt] < t) ++cnt;
+        while (cnt < (int)m && R.y[i][cnt] < t) ++cnt;
         p.rita_guide.push_back((uint8_t)cnt);
       }
     }

@@ -90,9 +90,9 @@ struct Packed {
   std::vector<float> xs;
   std::vector<mdx_node> nodes;
   std::vector<int32_t> hash;
-  std::vector<float> rita_E, rita_inv_dlnE;
+  std::vector<float> rita_E, rita_escale;
   std::vector<int32_t> rita_row_off;
-  std::vector<mdx_f4> rita_rows;
+  std::vector<mdx_rita_row> rita_rows;
   std::vector<int32_t> rita_ehash;
   std::vector<uint8_t> rita_guide;
   mdx_curve majorant{};

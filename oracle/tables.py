@@ -307,7 +307,9 @@ class Rita:
 
     def sample(self, E, y):
         """probability_dist.cpp:73-88 with the uniform y given."""
-        lo = np.searchsorted(self.energies, E, side="left") - 1
+        # lower_bound - 1, clamped to the end intervals (the reference reads
+        # out of its arrays for energies at or beyond the grid's ends, :76)
+        lo = min(max(int(np.searchsorted(self.energies, E, side="left")) - 1, 0), len(self.energies) - 2)
         up = lo + 1
         xl, xu = self.x_from_y(lo, y), self.x_from_y(up, y)
         El, Eu = self.energies[lo], self.energies[up]

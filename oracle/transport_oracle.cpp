@@ -128,7 +128,9 @@ struct Rita {  // probability_dist.cpp:73-119
     return xi + (1 + ai + bi) * d * nu / (d * d + ai * d * nu + bi * nu * nu) * (xi1 - xi);
   }
   double sample(double e, double yy) const {
-    int lo = lower_minus_one(E, e), up = lo + 1;
+    // lower_bound - 1 clamped to the end intervals: the reference reads out of
+    // its arrays for energies at or beyond the grid's ends (probability_dist.cpp:76)
+    int lo = std::max(0, std::min(lower_minus_one(E, e), (int)E.size() - 2)), up = lo + 1;
     double xl = x_from_y(lo, yy), xu = x_from_y(up, yy);
     return std::exp(std::log(xl + 1) + (std::log(xu + 1) - std::log(xl + 1)) * (std::log(e) - std::log(E[lo])) /
                                            (std::log(E[up]) - std::log(E[lo]))) -

@@ -146,3 +146,43 @@ def disc_basis(normal):
     u /= np.linalg.norm(u)
     v = np.cross(n, u)
     return u, v / np.linalg.norm(v)
+
+
+def binwise(g, o, events, min_events=200, screen_alpha=1e-3):
+    """Every bin of one tally, two codes compared: g (k_g x n) and o (k_o x n)
+    per-batch values, events the events behind each bin summed over both
+    sides' batches (a selection symmetric in the two samples).  Bins with at
+    least min_events stand alone; the others are summed into one pooled bin,
+    so that every bin takes part.  Per bin a Welch t of the two batch means, mapped to a
+    normal score (welch_normal_scores); the tally's chi^2 = sum of the
+    squared scores over its bins, with as many degrees of freedom (the bins
+    of a voxel-dose, image or spectrum tally are close to independent: a
+    history scores in few of them).  The screen at 3 combined standard
+    errors takes the number of bins: the count of bins beyond 3 sigma may
+    not exceed the (1 - screen_alpha) quantile of Binomial(n, P(|Z| > 3)),
+    and no bin may lie beyond the Sidak bound of family-wise level
+    screen_alpha.  Returns a dict; `ok` holds when p > 0.01 and the screen
+    passes."""
+    g = np.asarray(g, dtype=np.float64)
+    o = np.asarray(o, dtype=np.float64)
+    alone = np.asarray(events, dtype=np.float64).ravel() >= min_events
+    G, O = [g[:, alone]], [o[:, alone]]
+    rest = ~alone
+    if rest.any():
+        G.append(g[:, rest].sum(axis=1, keepdims=True))
+        O.append(o[:, rest].sum(axis=1, keepdims=True))
+    G, O = np.concatenate(G, axis=1), np.concatenate(O, axis=1)
+    gm, gs = batch_mean_sigma(G)
+    om, os_ = batch_mean_sigma(O)
+    live = (gs > 0) | (os_ > 0)  # a bin empty on both sides carries no test
+    z = welch_normal_scores(gm, gs, len(G), om, os_, len(O))[live]
+    n = len(z)
+    chi2 = float(np.sum(z * z))
+    p = float(stats.chi2.sf(chi2, n))
+    over3 = int(np.sum(np.abs(z) > 3.0))
+    allow3 = int(stats.binom.ppf(1 - screen_alpha, n, 2 * stats.norm.sf(3.0)))
+    zmax = float(np.max(np.abs(z))) if n else 0.0
+    zcap = float(stats.norm.isf(-np.expm1(np.log1p(-screen_alpha) / n) / 2)) if n else np.inf
+    return dict(ok=bool(p > 0.01 and over3 <= allow3 and zmax <= zcap), p=p, chi2=chi2, bins=n,
+                standalone=int(alone.sum()), pooled=int(rest.sum()), over3=over3, allow3=allow3, zmax=zmax,
+                zcap=zcap)

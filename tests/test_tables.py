@@ -94,3 +94,24 @@ def test_packing_is_exact_on_host():
     with pytest.raises(M.MdxError):
         c.material_id("Unobtainium")
     c.close()
+
+
+def test_rita_energy_bracket_clamped_at_grid_ends():
+    """An energy at or below the RITA grid's first energy (or above its last)
+    has no lower_bound - 1 bracket: the reference reads outside its arrays
+    there (probability_dist.cpp:76); the host sampler, the device and the
+    oracle extend the end interval instead.  At the first grid energy the
+    end interval's lower row is taken exactly (f = 0)."""
+    import midsx_amd as M
+    ctx = M.Context(-1)
+    ctx.build_tables([48])
+    e0, x0, y0, a0, b0 = ctx.rita_row(48, 0)
+    y = np.array([0.1, 0.5, 0.9])
+    first = ctx.sample_rayleigh_host(48, np.full(3, e0), y)
+    below = ctx.sample_rayleigh_host(48, np.full(3, e0 * 0.999), y)
+    above = ctx.sample_rayleigh_host(48, np.full(3, 1e12), y)
+    assert np.all(np.isfinite(first)) and np.all(np.isfinite(below)) and np.all(np.isfinite(above))
+    r = T.Material(48).rita
+    assert np.allclose(first, [r.x_from_y(0, v) for v in y], rtol=0, atol=1e-12)
+    assert np.allclose(below, [r.sample(e0 * 0.999, v) for v in y], rtol=0, atol=1e-9)
+    ctx.close()
