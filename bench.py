@@ -1,29 +1,33 @@
 """Benchmark of the photon-transport hot path on one MI355X.
 
-Workload (radiography): 256 x 256 x 160 uint8 phantom of seeded ellipsoids at
-1.25 mm (water, cortical bone, adipose) in air, point source 60 cm above the
-phantom, cone beam over its top face, Kramers 15-80 keV spectrum in 1 keV
-bins, a 256 x 256 energy-fluence image at the far face and FP64 per-voxel
-deposited energy.  One step transports one batch of histories; the default
-10 steps of 1e7 histories are the 1e8 histories of the workload.  Inputs
-(tables, phantom, tallies) are resident in HBM before timing starts.
+Headline workload (the served job stream): dose jobs drawn with a fixed seed
+(cubic phantoms of 64^3 to 512^3 voxels at 0.5-2 mm, monoenergetic 20-120
+keV or Kramers 60-120 kVp sources, FP64 voxel dose plus a 128 x 128 image
+each), each asking for 1 % relative standard error on its region of
+interest: four 1 cm^3 dose voxels on the beam axis, 1-2 cm below the
+entrance face (`scenes.dose_job_roi`; the error of a voxel's summed deposits
+as the reference computes it, quantity.cpp:37-45).  The continuous-batching
+loop runs each job until every ROI voxel is at or below 1 % (checked on the
+device after every round); the plain kernel then serves the same jobs with
+the same history counts first-in first-out, one launch per job, so both
+sides reach the same sigma.  `value` is the served histories/s with every
+job queued at the start; then p50/p99 latency under Poisson arrivals at 70 %
+of each service's own capacity.  Phantoms are built before the clock starts
+(they are the clients' data, in pinned host memory); uploads, tally zeroing,
+transport, the ROI checks and the read-back of every job's dose and image
+are timed.  The roofline is that of the round kernel (k_transport_queue_round)
+over the timed rounds: algorithmic bytes from its int64 event counters over
+its HIP-event time.
 
-Prints one JSON line: throughput, the roofline of the transport kernel
-(algorithmic bytes from its int64 event counters over its HIP-event
-duration), and the FP64 CPU restatement timed on a bounded sample of the same
-workload on the host cores.
-
-Served job stream (the "served" object): dose jobs drawn with a fixed seed
-(cubic phantoms of 64^3 to 512^3 voxels at 0.5-2 mm, monoenergetic 20-120 keV
-or Kramers 60-120 kVp sources, 1e6-1e8 histories log-uniform, FP64 voxel dose
-plus a 128 x 128 image each), served by the continuous-batching loop and,
-for comparison, first-in first-out by the plain kernel one job per launch:
-throughput with every job queued at the start, then p50/p99 latency under
-Poisson arrivals at a fraction of the served capacity.  Phantoms are built
-before the clock starts (they are the clients' data, handed over in pinned
-host memory); uploads, tally zeroing, transport and the read-back of every
-job's dose and image are timed; the ROI statistics come from one more
-served pass outside the timed runs.
+Beside it: the radiography workload (256 x 256 x 160 phantom at 1.25 mm,
+15-80 keV Kramers cone beam, 256 x 256 image, FP64 voxel dose; 10 steps of
+1e7 histories) on the queue kernel, the plain kernel, and the queue kernel
+with background free flights and local majorants off (what batching alone
+gives); the CT scan (360 views of a 512 x 512 x 256 phantom in one launch;
+plain on every tenth view); the precision study (the slab at 1e9 histories
+per accumulation mode); and the FP64 CPU restatement of the reference's
+OpenMP path timed on a bounded sample of the served stream's jobs on the
+host's cores.
 """
 import argparse
 import json
@@ -37,6 +41,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+RSE_TARGET = 0.01
 
 
 def algorithmic_bytes(st):
@@ -49,36 +54,66 @@ def algorithmic_bytes(st):
             + 32 * st["deposits"] + 16 * st["surface_hits"] + 8 * st["histories"])
 
 
-def measured_traffic_per_history():
-    """HBM bytes per history of the queue kernel on this workload from the
-    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None if absent)."""
-    path = os.path.join(ROOT, "profiles", "k_transport_queue_radiography_256x256x160.md")
+def progress(msg):
+    print("bench: %s" % msg, file=sys.stderr, flush=True)
+
+
+def profile_value(name, key):
+    """A figure recorded in a committed rocprofv3 summary (None if absent)."""
     try:
-        for line in open(path):
-            if line.startswith("traffic_bytes_per_history"):
+        for line in open(os.path.join(ROOT, "profiles", name)):
+            if line.startswith(key):
                 return float(line.split("=")[1])
-    except OSError:
+    except (OSError, ValueError):
         pass
     return None
 
 
-def cpu_baseline(kw, seconds=12.0, threads=16):
+def copy_probe(ctx, mb):
+    """The library's streaming copy of mb MiB (read mb MiB, write mb MiB),
+    twice: the reference for scaling the HBM counters of a profiled run, and
+    the achievable copy bandwidth."""
+    nbytes = mb * 2 ** 20
+    ms = ctx.copy_probe(nbytes)
+    return {"bytes_read": nbytes, "bytes_written": nbytes, "ms": ms, "GB_per_s": 2 * nbytes / ms / 1e6}
+
+
+def cpu_baseline(jobs, phantoms, seconds, affinity):
+    """The FP64 C++/OpenMP restatement of the reference's CPU path on a
+    bounded sample of the served stream: one job of each phantom size, each
+    run for the histories that take about seconds / 4 on the host's cores
+    (the OpenMP team of the environment)."""
     import oracle
     from midsx_amd import scenes
-    sim = oracle.Simulation(scenes.C2_MATERIALS)
-    scenes.setup_radiography(sim, **kw)
-    n = 2000
-    dt = sim.run(n, seed=99, threads=threads)
-    while dt < 1.0 and n < 10 ** 7:
-        n *= 4
+    threads = oracle.max_threads()
+    sim = oracle.Simulation(scenes.DOSE_MATERIALS)
+    picks, seen = [], set()
+    for j in jobs:
+        if j["shape"][0] not in seen:
+            seen.add(j["shape"][0])
+            picks.append(j)
+    total_n, total_t, parts = 0, 0.0, []
+    for j in picks:
+        progress("CPU baseline: %d^3 job" % j["shape"][0])
+        scenes.setup_dose_job(sim, j, ids=phantoms(j))
+        n = 2000
         sim.reset()
-        dt = sim.run(n, seed=99, threads=threads)
-    n = int(max(n, n * seconds / max(dt, 1e-9)))
-    sim.reset()
-    dt = sim.run(n, seed=100, threads=threads)
-    return dict(value=n / dt, unit="histories/s", cores=threads, kind="port",
-                sample="%d histories of the radiography workload (full phantom, image and voxel-dose tallies), "
-                       "FP64 C++/OpenMP restatement, %.1f s wall" % (n, dt))
+        dt = sim.run(n, seed=99, threads=0)
+        while dt < 0.5 and n < 10 ** 8:
+            n *= 4
+            sim.reset()
+            dt = sim.run(n, seed=99, threads=0)
+        n = int(max(n, n * seconds / len(picks) / max(dt, 1e-9)))
+        sim.reset()
+        dt = sim.run(n, seed=100, threads=0)
+        total_n += n
+        total_t += dt
+        parts.append("%d^3: %d histories in %.1f s" % (j["shape"][0], n, dt))
+    return dict(value=total_n / total_t, unit="histories/s", cores=threads, kind="port",
+                sample="FP64 C++/OpenMP restatement of runSimulation on one dose job of each phantom size of the "
+                       "served stream (" + "; ".join(parts) + ")",
+                omp_max_threads=threads, nproc=os.cpu_count(), affinity_cpus=affinity,
+                omp_proc_bind=os.environ.get("OMP_PROC_BIND"), omp_places=os.environ.get("OMP_PLACES"))
 
 
 def timed_steps(ctx, kernel, n, steps, warmup, first):
@@ -95,37 +130,39 @@ def timed_steps(ctx, kernel, n, steps, warmup, first):
     return stats, wall, first
 
 
-def copy_probe(ctx, mb):
-    """The library's streaming copy of mb MiB (read mb MiB, write mb MiB),
-    twice: the reference for scaling the HBM counters of a profiled run, and
-    the achievable copy bandwidth."""
-    nbytes = mb * 2 ** 20
-    ms = ctx.copy_probe(nbytes)
-    return {"bytes_read": nbytes, "bytes_written": nbytes, "ms": ms, "GB_per_s": 2 * nbytes / ms / 1e6}
+def roofline(stats_list, traffic=None, source=None):
+    kern_ms = np.array([s["kernel_ms"] for s in stats_list])
+    byt = np.array([algorithmic_bytes(s) for s in stats_list], dtype=np.float64)
+    achieved = byt.sum() / (kern_ms.sum() / 1e3) / 1e9
+    out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel_ms_avg": float(kern_ms.mean()),
+           "algorithmic_bytes_per_launch": float(byt.mean())}
+    if source:
+        out["traffic_source"] = source
+    return out
 
 
-def served_stream(ctx, n_jobs, seed, load):
-    """The served job stream: saturated throughput of the continuous-batching
-    loop and of plain FIFO service, then latency under Poisson arrivals."""
+def served_stream(ctx, n_jobs, seed, load, cap, plain=True):
+    """The served job stream at 1 % ROI error: the continuous-batching loop
+    against plain FIFO on the same history counts."""
     from midsx_amd import scenes
-    from midsx_amd.serve import Server, run_plain, poisson_arrivals, latency_percentiles, roi_summary, \
-        warm_read_buffers
-    jobs = scenes.draw_dose_jobs(n_jobs, seed)
+    from midsx_amd.serve import Server, run_plain, poisson_arrivals, latency_percentiles, warm_read_buffers
+    jobs = [dict(j, histories=cap, rse_target=RSE_TARGET) for j in scenes.draw_dose_jobs(n_jobs, seed)]
     cache = {}
     for j in jobs:  # the clients' phantoms, in pinned host memory (x fastest)
-        ids = scenes.dose_phantom(j["shape"], j["voxel_cm"], j["phantom_seed"])
-        try:
-            buf = ctx.pinned(ids.size, np.uint8).reshape(ids.shape, order="F")
-            buf[...] = ids
-            ids = buf
-        except Exception:
-            pass
-        cache[id(j)] = ids
-    ph = lambda j: cache[id(j)]
-    H = sum(j["histories"] for j in jobs)
+        key = (j["shape"], j["voxel_cm"], j["phantom_seed"])
+        if key not in cache:
+            ids = scenes.dose_phantom(j["shape"], j["voxel_cm"], j["phantom_seed"])
+            try:
+                buf = ctx.pinned(ids.size, np.uint8).reshape(ids.shape, order="F")
+                buf[...] = ids
+                ids = buf
+            except Exception:
+                pass
+            cache[key] = ids
+    ph = lambda j: cache[(j["shape"], j["voxel_cm"], j["phantom_seed"])]
     # the stream's own launch parameters, tuned before timing on one job of
     # it (a 256^3 phantom if there is one), as a server does at start-up
-    keep_params = ctx.queue_params()
     keep_job = ctx.job
     probe = next((j for j in jobs if j["shape"][0] == 256), jobs[0])
     pj = ctx.job_create()
@@ -133,38 +170,51 @@ def served_stream(ctx, n_jobs, seed, load):
     tuned = ctx.autotune(probe_histories=4000000)
     ctx.job_release(pj)
     ctx.job_select(keep_job)
-    # an untimed pass of the same stream first, which also yields the dose
-    # statistics: the device memory pool and the pinned result buffers are
-    # then warm, as in a service that has been running
-    summaries = {}
-
-    def keep(job, mean, rel, image):
-        summaries[id(job)] = roi_summary(job, mean, rel, image)
-
-    Server(ctx, phantoms=ph, on_result=keep).run(jobs)
-    rse = [summaries[id(j)]["roi_median_rse"] for j in jobs]
+    # an untimed pass first: the device memory pool and the pinned result
+    # buffers are then warm, as in a service that has been running
+    progress("served stream: tuned %s; warm pass" % (tuned,))
+    Server(ctx, phantoms=ph).run(jobs)
+    progress("served stream: timed pass")
     srv = Server(ctx, phantoms=ph)
     t = time.perf_counter()
-    srv.run(jobs)
+    res = srv.run(jobs)
     served_s = time.perf_counter() - t
-    warm_read_buffers(ctx, jobs)  # the plain service's result buffers, pinned ahead as well
-    _, pinfo = run_plain(ctx, jobs, phantoms=ph)
-    plain_s = pinfo["makespan"]
-    cap = n_jobs / served_s
-    arr = poisson_arrivals(n_jobs, load * cap, seed + 1)
-    srv2 = Server(ctx, phantoms=ph)
-    lat = srv2.run(jobs, arr)
-    p50, p99 = latency_percentiles(lat)
-    plat, _ = run_plain(ctx, jobs, arr, phantoms=ph)
-    q50, q99 = latency_percentiles(plat)
-    ctx.set_queue_params(*keep_params)
-    return {
-        "workload": "%d dose jobs (seed %d): phantoms %s, histories 1e6-1e8 log-uniform, total %d" % (
-            n_jobs, seed, sorted(set(j["shape"][0] for j in jobs)), H),
+    hist = [r.histories for r in res]
+    H = sum(hist)
+    served_rse = [r.roi_rse for r in res]
+    capped = sum(1 for h in hist if h >= cap)
+    progress("served stream: %.3f s for %d histories; latency pass" % (served_s, H))
+    # latency under Poisson arrivals at `load` of each service's own capacity
+    arr = poisson_arrivals(n_jobs, load * n_jobs / served_s, seed + 1)
+    p50, p99 = latency_percentiles(Server(ctx, phantoms=ph).run(jobs, arr))
+    plain_s = q50 = q99 = None
+    plain_rse = [float("nan")]
+    pinfo = {"kernel_ms": float("nan"), "counters": {"steps": float("nan")}}
+    if plain:
+        # plain: the same jobs with the histories each needed when served
+        pjobs = [dict(j, histories=h) for j, h in zip(jobs, hist)]
+        warm_read_buffers(ctx, pjobs)  # the plain service's result buffers, pinned ahead as well
+        progress("served stream: plain FIFO")
+        pres, pinfo = run_plain(ctx, pjobs, phantoms=ph)
+        plain_s = pinfo["makespan"]
+        progress("served stream: plain %.3f s; plain latency pass" % plain_s)
+        plain_rse = [r.roi_rse for r in pres]
+        parr = poisson_arrivals(n_jobs, load * n_jobs / plain_s, seed + 2)
+        plat, _ = run_plain(ctx, pjobs, parr, phantoms=ph)
+        q50, q99 = latency_percentiles(plat)
+    traffic_ph = profile_value("k_transport_queue_round_served_dose_stream.md", "traffic_bytes_per_history")
+    rounds = srv.round_log
+    traffic = traffic_ph * float(np.mean([r["histories"] for r in rounds])) if traffic_ph else None
+    return jobs, ph, {
+        "workload": "%d dose jobs (seed %d), phantoms %s voxels at 0.5-2 mm, each run to <= %g relative "
+                    "standard error on its 4 ROI dose voxels (1 cm^3, on the beam axis 1-2 cm deep); "
+                    "histories per job min %.3g, median %.3g, max %.3g, total %d" % (
+                        n_jobs, seed, sorted(set(j["shape"][0] for j in jobs)), RSE_TARGET, min(hist),
+                        float(np.median(hist)), max(hist), H),
         "value": H / served_s,
-        "unit": "histories/s",
-        "plain_value": H / plain_s,
-        "speedup_vs_plain": plain_s / served_s,
+        "plain_value": H / plain_s if plain_s else None,
+        "speedup_vs_plain": plain_s / served_s if plain_s else None,
+        "jobs_at_cap": capped,
         "served_s": served_s,
         "plain_s": plain_s,
         "kernel_s": srv.kernel_ms / 1e3,
@@ -173,19 +223,82 @@ def served_stream(ctx, n_jobs, seed, load):
         "host_s": {k: round(v, 4) for k, v in srv.host_s.items()},
         "steps_per_history": srv.counters["steps"] / H,
         "plain_steps_per_history": pinfo["counters"]["steps"] / H,
-        "latency": {"arrival_rate_jobs_per_s": load * cap, "load": load, "p50_s": p50, "p99_s": p99,
+        "roi_rse": {"served_max": float(np.max(served_rse)), "served_median": float(np.median(served_rse)),
+                    "plain_max": float(np.max(plain_rse)), "plain_median": float(np.median(plain_rse))},
+        "latency": {"load": load, "served_arrival_rate_jobs_per_s": load * n_jobs / served_s, "p50_s": p50,
+                    "p99_s": p99, "plain_arrival_rate_jobs_per_s": load * n_jobs / plain_s if plain_s else None,
                     "plain_fifo_p50_s": q50, "plain_fifo_p99_s": q99},
-        "roi_median_rse": {"median_over_jobs": float(np.median(rse)), "max_over_jobs": float(np.max(rse))},
+        "roofline": roofline(rounds, traffic, "rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 per history of the "
+                                              "served stream (factors from the k_copy16 streaming copy), "
+                                              "profiles/k_transport_queue_round_served_dose_stream.md"
+                             if traffic else None),
         "queue_params": tuned,
     }
 
 
-def ct_scan(ctx, n_views, scale, seed=4):
+def radiography(ctx, args):
+    """Config 2 on the queue kernel (autotuned), the plain kernel, and the
+    queue kernel with free flights and local majorants off."""
+    import midsx_amd as M
+    from midsx_amd import scenes
+    kw = dict(shape=(256, 256, 160), voxel_cm=0.125, seed=2, image=(256, 256))
+    job = ctx.job_create()
+    scenes.setup_radiography(ctx, **kw)
+    n = args.histories_per_step
+    tuned = None
+    if args.queue_params:
+        ctx.set_queue_params(*[int(v) for v in args.queue_params.split(",")])
+    elif not args.no_autotune:
+        tuned = ctx.autotune(probe_histories=4000000)  # launches of ~3 ms: stable timings
+    progress("radiography: queue kernel")
+    stats, wall, first = timed_steps(ctx, M.KERNEL_QUEUE, n, args.steps, args.warmup, 0)
+    total = n * args.steps
+    out = {
+        "workload": "radiography: 256x256x160 uint8 phantom at 1.25 mm, 15-80 keV Kramers cone beam, "
+                    "256x256 image + FP64 voxel dose, %d steps of %d histories" % (args.steps, n),
+        "value": total / wall, "unit": "histories/s", "ms_per_step": wall / args.steps * 1e3,
+        "queue_params": dict(zip(("blocks_per_cu", "refill_min", "interact_min"), ctx.queue_params())),
+    }
+    agg = {k: sum(s[k] for s in stats) for k in M.STAT_NAMES}
+    out["events_per_history"] = {k: agg[k] / total for k in ("steps", "real", "rayleigh", "compton", "photo",
+                                                              "kahn_trials", "deposits", "surface_hits")}
+    tph = profile_value("k_transport_queue_radiography_256x256x160.md", "traffic_bytes_per_history")
+    out["roofline"] = roofline(stats, tph * n if tph else None,
+                               "rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 per launch (factors from the "
+                               "k_copy16 streaming copy), profiles/k_transport_queue_radiography_256x256x160.md"
+                               if tph else None)
+    if tuned:
+        out["autotune_probe_ms"] = tuned["probe_ms"]
+    if not args.no_plain:
+        progress("radiography: plain kernel")
+        pstats, pwall, first = timed_steps(ctx, M.KERNEL_PLAIN, n, args.steps, args.warmup, first)
+        out["plain"] = {"value": total / pwall, "ms_per_step": pwall / args.steps * 1e3,
+                        "steps_per_history": sum(s["steps"] for s in pstats) / total}
+        out["speedup_vs_plain"] = out["value"] / out["plain"]["value"]
+        # what batching and regrouping alone give: majorant steps everywhere
+        progress("radiography: ablation")
+        ctx.set_flights(False)
+        ctx.set_local_majorants(False)
+        astats, awall, first = timed_steps(ctx, M.KERNEL_QUEUE, n, args.steps, args.warmup, first)
+        ctx.set_flights(True)
+        ctx.set_local_majorants(True)
+        out["ablation_majorant_steps"] = {
+            "what": "queue kernel with background free flights and local majorants off (the plain kernel's "
+                    "Woodcock steps, event-regrouped and batched), same launch parameters",
+            "value": total / awall, "ms_per_step": awall / args.steps * 1e3,
+            "steps_per_history": sum(s["steps"] for s in astats) / total,
+            "speedup_vs_plain": (total / awall) / out["plain"]["value"]}
+    ctx.job_release(job)
+    return out
+
+
+def ct_scan(ctx, n_views, scale, plain_every, seed=4):
     """CT: 512 x 512 x 256 phantom at 0.8 mm (water, lung, adipose, bone),
     n_views views about its z axis, 120 kVp, a 512 x 64 detector per view,
     per-view histories U[2e6, 2e7] x scale.  All views in one multi-job launch
     of the queue kernel (views alias the phantom job; the phantom's voxel dose
-    sums over views) versus the plain kernel one view per launch."""
+    sums over views) versus the plain kernel one view per launch, on every
+    plain_every-th view (histories/s of that subsample)."""
     import midsx_amd as M
     from midsx_amd import scenes
     owner = ctx.job_create()
@@ -201,21 +314,24 @@ def ct_scan(ctx, n_views, scale, seed=4):
         ctx.job_select(j)
         ctx.reset_tallies()
     t = time.perf_counter()
+    progress("CT: one launch of %d views" % n_views)
     st = ctx.run_jobs(views, n, [0] * n_views, seeds)
     q_s = time.perf_counter() - t
+    progress("CT: %.3f s; plain on every %d-th view" % (q_s, plain_every))
+    ctx.set_graphs(False)
     for j in [owner] + views:
         ctx.job_select(j)
         ctx.reset_tallies()
+    sub = list(range(0, n_views, plain_every))
     t = time.perf_counter()
-    pk = 0.0
-    psteps = 0
-    for v, nv, sv in zip(views, n, seeds):
-        ctx.job_select(v)
-        ps = ctx.run(nv, seed=sv, first=0, kernel=M.KERNEL_PLAIN)
+    pk, psteps, pH = 0.0, 0, 0
+    for v in sub:
+        ctx.job_select(views[v])
+        ps = ctx.run(n[v], seed=seeds[v], first=0, kernel=M.KERNEL_PLAIN)
         pk += ps["kernel_ms"]
         psteps += ps["steps"]
+        pH += n[v]
     p_s = time.perf_counter() - t
-    ctx.set_graphs(False)
     for j in views + [owner]:
         ctx.job_release(j)
     return {
@@ -223,15 +339,16 @@ def ct_scan(ctx, n_views, scale, seed=4):
                     "histories per view U[2e6,2e7] x %g (total %d)" % (n_views, scale, H),
         "value": H / q_s,
         "unit": "histories/s",
-        "plain_value": H / p_s,
-        "speedup_vs_plain": p_s / q_s,
+        "plain_value": pH / p_s,
+        "plain_sample": "every %d-th view (%d views, %d histories)" % (plain_every, len(sub), pH),
+        "speedup_vs_plain": (H / q_s) / (pH / p_s),
         "queue_s": q_s,
         "queue_kernel_ms": st["kernel_ms"],
         "plain_s": p_s,
         "plain_kernel_ms": pk,
-        "launches": {"queue": 1, "plain": n_views},
+        "launches": {"queue": 1, "plain": len(sub)},
         "steps_per_history": st["steps"] / H,
-        "plain_steps_per_history": psteps / H,
+        "plain_steps_per_history": psteps / pH,
         "queue_params": tuned,
     }
 
@@ -252,6 +369,7 @@ def precision_study(ctx, n_hist, step=10 ** 8):
     res = {}
     for name, mode in (("fp64", M.DOSE_F64), ("fp32", M.DOSE_F32), ("fp32_compensated", M.DOSE_F32C),
                        ("fp32_compensated_lds", M.DOSE_F32C_LDS)):
+        progress("precision study: %s" % name)
         ctx.set_dose_precision(mode)
         ctx.reset_tallies()
         ms = 0.0
@@ -282,111 +400,72 @@ def precision_study(ctx, n_hist, step=10 ** 8):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=10, help="radiography steps of --histories-per-step")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--histories-per-step", type=int, default=10 ** 7)
+    ap.add_argument("--served-jobs", type=int, default=32)
+    ap.add_argument("--served-seed", type=int, default=2026)
+    ap.add_argument("--served-load", type=float, default=0.7)
+    ap.add_argument("--served-cap", type=int, default=2 * 10 ** 9, help="histories cap of a served job")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-radiography", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--queue-params", default=None,
-                    help="blocks_per_cu,refill_min,interact_min for the queue kernel (skips the autotuner)")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--copy-probe-mb", type=int, default=0)
-    ap.add_argument("--no-served", action="store_true")
-    ap.add_argument("--served-jobs", type=int, default=32)
-    ap.add_argument("--served-seed", type=int, default=2026)
-    ap.add_argument("--served-load", type=float, default=0.7)
+                    help="blocks_per_cu,refill_min,interact_min for the radiography (skips its autotuner)")
+    ap.add_argument("--cpu-seconds", type=float, default=16.0)
     ap.add_argument("--no-ct", action="store_true")
+    ap.add_argument("--ct-views", type=int, default=360)
+    ap.add_argument("--ct-scale", type=float, default=1.0,
+                    help="fraction of the CT workload's per-view histories (1.0: the full ~4e9)")
+    ap.add_argument("--ct-plain-every", type=int, default=10)
+    ap.add_argument("--copy-probe-mb", type=int, default=0)
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--precision-histories", type=int, default=10 ** 9)
-    ap.add_argument("--ct-views", type=int, default=360)
-    ap.add_argument("--ct-scale", type=float, default=0.05,
-                    help="fraction of the CT workload's per-view histories (1.0 = the full scan, ~4e9 histories)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: sharding histories over several GPUs is not built in this "
               "tree; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    affinity = len(os.sched_getaffinity(0))  # before OpenMP binds this thread
+    # the CPU baseline's OpenMP team: threads on cores, kept close
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
 
     import midsx_amd as M
     from midsx_amd import scenes
 
-    kw = dict(shape=(256, 256, 160), voxel_cm=0.125, seed=2, image=(256, 256))
     ctx = M.Context(0)
     ctx.build_tables(sorted(set(scenes.C2_MATERIALS) | set(scenes.DOSE_MATERIALS)))
-    scenes.setup_radiography(ctx, **kw)
-    n = args.histories_per_step
-    tuned = None
-    if args.queue_params:
-        ctx.set_queue_params(*[int(v) for v in args.queue_params.split(",")])
-    elif not args.no_autotune:
-        tuned = ctx.autotune(probe_histories=4000000)  # launches of ~3 ms: stable timings
-    stats, wall, first = timed_steps(ctx, M.KERNEL_QUEUE, n, args.steps, args.warmup, 0)
-    plain = None
-    if not args.no_plain:
-        pstats, pwall, _ = timed_steps(ctx, M.KERNEL_PLAIN, n, args.steps, args.warmup, first)
-        plain = {"value": n * args.steps / pwall, "ms_per_step": pwall / args.steps * 1e3,
-                 "kernel_ms_avg": float(np.mean([s["kernel_ms"] for s in pstats]))}
-    total = n * args.steps
-    kern_ms = np.array([s["kernel_ms"] for s in stats])
-    bytes_per_launch = np.mean([algorithmic_bytes(s) for s in stats])
-    achieved = bytes_per_launch / (kern_ms.mean() / 1e3) / 1e9
-    agg = {k: sum(s[k] for s in stats) for k in M.STAT_NAMES}
-    value = total / wall
-
     out = {
-        "metric": "photon histories/sec vs plain GPU at matched tally \u03c3; p50/p99 request latency",
-        "value": value,
-        "unit": "histories/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "fp32",
-        "data": "synthetic (seeded ellipsoid phantom; EPDL2017 tables for air, water, cortical bone, adipose)",
-        "config": {
-            "workload": "radiography: 256x256x160 uint8 phantom at 1.25 mm, 15-80 keV Kramers cone beam, "
-                        "256x256 image + FP64 voxel dose, 1e8 histories",
-            "histories_per_step": n,
-            "kernel": "queue (event-regrouped, background free flights); plain = one thread per history, "
-                      "majorant steps everywhere, every step replayed for the tallies; equal histories, "
-                      "hence equal tally sigma",
-            "queue_params": dict(zip(("blocks_per_cu", "refill_min", "interact_min"), ctx.queue_params())),
-            "steps_per_history": agg["steps"] / total,
-        },
-        "plain": plain,
-        "speedup_vs_plain": (value / plain["value"]) if plain else None,
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": (measured_traffic_per_history() or 0.0) * n or None,
-            "traffic_source": "rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 per launch (factors from the "
-                              "k_copy16 streaming copy), profiles/k_transport_queue_radiography_256x256x160.md",
-            "kernel_ms_avg": float(kern_ms.mean()),
-            "algorithmic_bytes_per_launch": float(bytes_per_launch),
-        },
-        "events_per_history": {k: agg[k] / total for k in ("steps", "real", "rayleigh", "compton", "photo",
-                                                            "kahn_trials", "deposits", "surface_hits")},
+        "metric": "photon histories/sec vs plain GPU at matched tally σ; p50/p99 request latency",
+        "value": None, "unit": "histories/s", "n_gpus": 1, "steps": None, "warmup": None, "ms_per_step": None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded ellipsoid phantoms; EPDL2017 tables for air, water, lung, adipose, cortical bone)",
     }
-    if tuned:
-        out["config"]["autotune_probe_ms"] = tuned["probe_ms"]
+    jobs = ph = None
     if not args.no_served:
-        out["served"] = served_stream(ctx, args.served_jobs, args.served_seed, args.served_load)
+        jobs, ph, sv = served_stream(ctx, args.served_jobs, args.served_seed, args.served_load, args.served_cap,
+                                     plain=not args.no_plain)
+        out.update(value=sv["value"], steps=sv["rounds"], warmup=1, ms_per_step=sv["served_s"] / sv["rounds"] * 1e3,
+                   speedup_vs_plain=sv["speedup_vs_plain"], latency=sv["latency"], roofline=sv.pop("roofline"))
+        out["config"] = {"workload": "served dose-job stream: " + sv["workload"],
+                         "kernel": "served = continuous batching, k_transport_queue_round per round (hipGraph); "
+                                   "plain = k_transport_plain, one launch per job, FIFO; same histories per job, "
+                                   "hence matched sigma",
+                         "step": "one round of the continuous-batching loop"}
+        out["served"] = sv
+    if not args.no_radiography:
+        out["radiography"] = radiography(ctx, args)
     if not args.no_ct:
-        out["ct"] = ct_scan(ctx, args.ct_views, args.ct_scale)
+        out["ct"] = ct_scan(ctx, args.ct_views, args.ct_scale, args.ct_plain_every)
     if not args.no_precision:
         out["precision"] = precision_study(ctx, args.precision_histories)
     if args.copy_probe_mb:
         out["copy_probe"] = copy_probe(ctx, args.copy_probe_mb)
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(kw, seconds=args.cpu_seconds, threads=args.cpu_threads)
+    if not args.no_cpu_baseline and jobs is not None:
+        out["cpu_baseline"] = cpu_baseline(jobs, ph, args.cpu_seconds, affinity)
     print(json.dumps(out), flush=True)
 
 

@@ -383,6 +383,14 @@ int mdx_select_interaction(mdx_ctx* ctx, int n, const float* s_coh, const float*
  * else 0. */
 int mdx_delta_decision(mdx_ctx* ctx, int n, const float* s_tot, const float* s_max, const float* u,
                        const int* limited, int* real);
+/* Region-of-interest dose statistics: for box b of job jobs[b] (its domain's
+ * grid 0, FP64 voxel dose), voxel index ranges boxes[6 b ..] = (i0, i1, j0,
+ * j1, k0, k1), half-open, the deposits, their sum and their sum of squares
+ * over the box, out[3 b ..] = (n, sum, sum^2): the VectorValue of the box's
+ * voxels together (quantity.cpp:27-94 then give its sum and sigma).  One
+ * launch for all boxes; the serving loop's stopping test.  No reference
+ * counterpart (the reference reads voxels one by one, voxel_grid.cpp:86-104). */
+int mdx_roi_stats(mdx_ctx* ctx, int n, const int* jobs, const int* boxes, double* out);
 /* PhotonSource::generatePhoton (source.cpp:150-155) on the device for the
  * selected job's source: the directions (3 floats per photon) and energies
  * of histories 0 .. n-1 under seed, as the transport kernels draw them. */

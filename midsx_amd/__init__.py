@@ -46,7 +46,7 @@ EXPORTS = [
     "mdx_set_flights", "mdx_set_rita_builder", "mdx_set_dose_precision", "mdx_get_voxel_dose_f32", "mdx_stage_voxel_dose_f32",
     "mdx_read_staged_dose_f32_async", "mdx_read_done", "mdx_copy_probe", "mdx_air_muen", "mdx_set_air_muen_table",
     "mdx_set_local_majorants", "mdx_get_macro_cells", "mdx_delta_decision", "mdx_philox",
-    "mdx_sample_source", "mdx_run_jobs_noalloc",
+    "mdx_sample_source", "mdx_run_jobs_noalloc", "mdx_roi_stats",
 ]
 
 
@@ -518,6 +518,17 @@ class Context:
         self._call(self.lib.mdx_get_voxel_dose, self._h, ctypes.c_int(g), _ptr(ss, ctypes.c_double),
                    None if cnt is None else _ptr(cnt, ctypes.c_uint64))
         return ss[0::2], ss[1::2], cnt
+
+    def roi_stats(self, jobs, boxes):
+        """(n, sum, sum^2) of the deposits in each box (i0, i1, j0, j1, k0, k1,
+        half-open voxel index ranges) of each job's grid 0; one row per box."""
+        j = np.ascontiguousarray(jobs, dtype=np.int32)
+        b = np.ascontiguousarray(boxes, dtype=np.int32).reshape(-1, 6)
+        out = np.zeros((len(j), 3))
+        if len(j):
+            self._call(self.lib.mdx_roi_stats, self._h, ctypes.c_int(len(j)), _ptr(j, ctypes.c_int),
+                       _ptr(b, ctypes.c_int), _ptr(out, ctypes.c_double))
+        return out
 
     def voxel_dose_f32(self, g, n_histories, out=None):
         """(mean deposit per history, relative standard error) per voxel as

@@ -296,6 +296,7 @@ struct mdx_ctx {
   DBuf d_work;  // queue kernel: histories claimed, one counter per job of a launch
   DBuf d_jobs, d_prefix;
   DBuf d_dose32;  // staging of mdx_get_voxel_dose_f32 (on the I/O stream)
+  DBuf d_roi_req, d_roi_out;  // mdx_roi_stats requests and sums
 
   // Round buffer of the graph path: [n_jobs, total, prefix[kRoundMax + 1], jobs[kRoundMax]]
   bool use_graphs = false;
@@ -2262,6 +2263,39 @@ int mdx_sample_source(mdx_ctx* c, int n, uint64_t seed, float* dir, float* energ
     hip_check(hipMemcpyAsync(dir, d.p(), 3 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream), "read");
     hip_check(hipMemcpyAsync(energy, e.p(), n * sizeof(float), hipMemcpyDeviceToHost, c->stream), "read");
     hip_check(hipStreamSynchronize(c->stream), "source");
+  });
+}
+
+int mdx_roi_stats(mdx_ctx* c, int n, const int* jobs, const int* boxes, double* out) {
+  return guard([&] {
+    need(c && (n == 0 || (jobs && boxes && out)) && n >= 0, "bad argument");
+    need(!c->inflight, "a round is in flight (mdx_wait first)");
+    if (n == 0) return;
+    set_device(c);
+    std::vector<mdx_roi_req> req(n);
+    for (int b = 0; b < n; ++b) {
+      Job& O = job_at(c, domain_owner(c, jobs[b]));
+      need(O.voxel_dose && !O.grids.empty(), "the job's voxel dose is not enabled");
+      GridHost& G = *O.grids[0];
+      need(G.dose_mode == MDX_DOSE_F64 && G.d_dose.p, "ROI statistics need FP64 voxel dose");
+      mdx_roi_req& q = req[b];
+      q.rec = G.d_dose.as<double>();
+      q.nx = G.dims[0];
+      q.ny = G.dims[1];
+      for (int k = 0; k < 3; ++k) {
+        q.lo[k] = boxes[6 * b + 2 * k];
+        q.hi[k] = boxes[6 * b + 2 * k + 1];
+        if (!(0 <= q.lo[k] && q.lo[k] < q.hi[k] && q.hi[k] <= G.dims[k]))
+          throw std::out_of_range("ROI box outside the grid or empty");
+      }
+      flush_clears_over(c, G.d_dose.p, G.d_dose.bytes);
+    }
+    c->d_roi_req.upload(req.data(), req.size(), c->stream);
+    c->d_roi_out.ensure((size_t)3 * n * sizeof(double));
+    hip_check(mdx_launch::roi_stats(c->d_roi_req.as<mdx_roi_req>(), n, c->d_roi_out.as<double>(), c->stream), "roi");
+    hip_check(hipMemcpyAsync(out, c->d_roi_out.p, (size_t)3 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream),
+              "read roi");
+    hip_check(hipStreamSynchronize(c->stream), "roi");
   });
 }
 

@@ -68,6 +68,9 @@ hipError_t rita_build(const double* lx, const double* ly, int n_ff, double alpha
 int rita_cap();
 // Streaming copy of n16 16-byte words (HBM reference for traffic counters).
 hipError_t copy16(const void* src, void* dst, size_t n16, int blocks, hipStream_t st);
+// Per requested box (n of them, one block each): (deposits, sum, sum^2) of the
+// FP64 voxel records into out[3 i ..].
+hipError_t roi_stats(const mdx_roi_req* reqs, int n, double* out, hipStream_t st);
 // (mean per history, relative standard error) per voxel from the FP64 voxel
 // records (sum, sum^2, count, 0).
 hipError_t dose_f32(const double* ss, size_t nv, double n, float* out, hipStream_t st);

@@ -213,6 +213,14 @@ struct mdx_job_dev {
 // which holds at most this many jobs.
 #define MDX_NOALLOC_MAX_JOBS 256
 
+// One box of a job's voxel grid whose FP64 voxel records mdx_roi_stats sums:
+// voxel index ranges [lo, hi) per axis of a grid with nx, ny voxels in x, y.
+struct mdx_roi_req {
+  const double* rec;  // the grid's MDX_DOSE_REC-double records
+  int32_t nx, ny;
+  int32_t lo[3], hi[3];
+};
+
 // Event counters, one int64 each; indexes into the counter array.
 #define MDX_C_HIST 0
 #define MDX_C_STEP 1

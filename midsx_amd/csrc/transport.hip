@@ -1433,6 +1433,33 @@ __global__ void k_dose_f32(const double* __restrict__ rec, size_t nv, double n, 
   out[i] = make_float2((float)m, m > 0.0 ? (float)(sqrt(var) / m) : 0.0f);
 }
 
+// Sums (deposits, sum, sum of squares) of the FP64 voxel records over each
+// requested box, one block per box: out[3 b .. 3 b + 2] = (n, sum, sum^2).
+__global__ void __launch_bounds__(256) k_roi_stats(const mdx_roi_req* __restrict__ reqs, double* __restrict__ out) {
+  const mdx_roi_req q = reqs[blockIdx.x];
+  const int sx = q.hi[0] - q.lo[0], sy = q.hi[1] - q.lo[1], sz = q.hi[2] - q.lo[2];
+  const int nv = sx * sy * sz;
+  double n = 0.0, s = 0.0, s2 = 0.0;
+  for (int v = threadIdx.x; v < nv; v += blockDim.x) {
+    const int i = q.lo[0] + v % sx, j = q.lo[1] + (v / sx) % sy, k = q.lo[2] + v / (sx * sy);
+    const double* r = q.rec + MDX_DOSE_REC * ((size_t)i + (size_t)q.nx * ((size_t)j + (size_t)q.ny * (size_t)k));
+    s += r[0];
+    s2 += r[1];
+    n += (double)*reinterpret_cast<const unsigned long long*>(r + 2);
+  }
+  __shared__ double red[3][256];
+  red[0][threadIdx.x] = n;
+  red[1][threadIdx.x] = s;
+  red[2][threadIdx.x] = s2;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int f = 0; f < 3; ++f) red[f][threadIdx.x] += red[f][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) out[3 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
 // Plain streaming copy (16 B per lane per iteration): the HBM reference for
 // scaling traffic counters.
 __global__ void k_copy16(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
@@ -1614,6 +1641,12 @@ hipError_t rita_build(const double* lx, const double* ly, int n_ff, double alpha
 }
 
 int rita_cap() { return kRitaCap; }
+
+hipError_t roi_stats(const mdx_roi_req* reqs, int n, double* out, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_roi_stats, dim3((unsigned)n), dim3(256), 0, st, reqs, out);
+  return hipGetLastError();
+}
 
 hipError_t copy16(const void* src, void* dst, size_t n16, int blocks, hipStream_t st) {
   if (n16 == 0) return hipSuccess;

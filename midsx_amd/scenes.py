@@ -169,6 +169,48 @@ def setup_dose_job(sim, job, ids=None):
     sim.enable_voxel_dose(True)
 
 
+ROI_EDGE_CM = 1.0
+
+
+def dose_job_roi(job):
+    """The job's region of interest: four dose voxels of 1 cm^3 (2 x 2
+    laterally), centred on the beam axis (the phantom's central column),
+    1-2 cm below the entrance face (the phantom's top: the beam travels
+    along -z); each ROI dose voxel holds the phantom voxels whose centres
+    lie in its cube.  Returns the four boxes as half-open voxel index ranges
+    (i0, i1, j0, j1, k0, k1)."""
+    nx, ny, nz = job["shape"]
+    v = job["voxel_cm"]
+
+    def idx(lo, hi, n):  # voxels with centres in [lo, hi) cm
+        a = int(np.ceil(lo / v - 0.5))
+        b = int(np.ceil(hi / v - 0.5))
+        return max(0, min(a, n)), max(0, min(b, n))
+
+    cx, cy, top = nx * v / 2, ny * v / 2, nz * v
+    k0, k1 = idx(top - 2 * ROI_EDGE_CM, top - ROI_EDGE_CM, nz)
+    boxes = []
+    for a in (0, 1):
+        i0, i1 = idx(cx - ROI_EDGE_CM + a * ROI_EDGE_CM, cx + a * ROI_EDGE_CM, nx)
+        for b in (0, 1):
+            j0, j1 = idx(cy - ROI_EDGE_CM + b * ROI_EDGE_CM, cy + b * ROI_EDGE_CM, ny)
+            boxes.append((i0, i1, j0, j1, k0, k1))
+    return boxes
+
+
+def roi_rse(stats):
+    """Relative standard error of each ROI dose voxel's summed deposits from
+    its (n, sum, sum^2): sigma_sum / sum with sigma_sum = sqrt(n s^2) and s^2
+    the sample variance of the deposits (quantity.cpp:37-45, 80-94); inf for
+    fewer than two deposits."""
+    st = np.atleast_2d(np.asarray(stats, dtype=np.float64))
+    n, s, s2 = st[:, 0], st[:, 1], st[:, 2]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        var = np.maximum(s2 - s * s / n, 0.0) / (n - 1)
+        rse = np.sqrt(n * var) / s
+    return np.where((n >= 2) & (s > 0), rse, np.inf)
+
+
 def dose_job_bytes(job, dose="f32"):
     """Device bytes a dose job holds at most: material IDs (1 B per voxel),
     the FP64 voxel records (sum, sum^2, count, 32 B), the macro-cell table

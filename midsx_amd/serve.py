@@ -14,6 +14,14 @@ A job's histories are keyed by its seed and history index, so its tallies do
 not depend on the round structure nor on the other jobs in the mix (the FP64
 sums differ only in their order of summation).
 
+A job may ask for a precision instead of a history count: with
+"rse_target" set, its "histories" is a cap, and it runs until every dose
+voxel of its region of interest (`scenes.dose_job_roi`) has a relative
+standard error at or below the target (the reference's sigma of a sum of
+deposits, quantity.cpp:37-45), checked on the device after every round
+(`Context.roi_stats`); its next slice is sized from the current error
+(histories done x (error / target)^2).
+
 Baseline: `run_plain` serves the same stream first-in first-out with the
 plain kernel, one launch per job (one simulation per launch).
 """
@@ -30,7 +38,7 @@ class JobResult:
     """What a finished job hands back: its voxel dose (sum, sum^2 per voxel,
     optionally reduced by the caller's `on_result`), its image and timing."""
 
-    __slots__ = ("index", "arrival", "admitted", "finished", "histories", "summary")
+    __slots__ = ("index", "arrival", "admitted", "finished", "histories", "summary", "roi_rse")
 
     def __init__(self, index, arrival):
         self.index = index
@@ -39,6 +47,7 @@ class JobResult:
         self.finished = None
         self.histories = 0
         self.summary = None
+        self.roi_rse = None  # largest relative standard error over the ROI dose voxels
 
     @property
     def latency(self):
@@ -60,10 +69,11 @@ def roi_summary(job, mean, rel, image, stride=4):
                 image_energy=float(image[..., 0].sum()))
 
 
-def read_dose(ctx, job, bufs, dose):
+def read_dose(ctx, job, bufs, dose, n_histories=None):
     """The selected job's voxel dose into a reusable (pinned) buffer kept in
     bufs: dose "f32" gives (mean per history, relative standard error) as
-    float32, "f64" the (sum, sum^2) of the deposits as float64."""
+    float32 for the n_histories run (default the job's histories), "f64" the
+    (sum, sum^2) of the deposits as float64."""
     nv = int(np.prod(job["shape"]))
     key = ("read", nv, dose)
     if key not in bufs:
@@ -73,15 +83,15 @@ def read_dose(ctx, job, bufs, dose):
         except Exception:
             bufs[key] = np.empty(2 * nv, dt)
     if dose == "f32":
-        return ctx.voxel_dose_f32(0, job["histories"], out=bufs[key])
+        return ctx.voxel_dose_f32(0, n_histories or job["histories"], out=bufs[key])
     s, s2, _ = ctx.voxel_dose(0, out=bufs[key], counts=False)
     return s, s2
 
 
 class _Active:
-    __slots__ = ("job", "res", "slot", "done", "bytes", "image", "buf")
+    __slots__ = ("job", "res", "slot", "done", "bytes", "image", "buf", "need", "rse", "roi")
 
-    def __init__(self, job, res, slot, nbytes):
+    def __init__(self, job, res, slot, nbytes, first_slice):
         self.job = job
         self.res = res
         self.slot = slot
@@ -89,6 +99,16 @@ class _Active:
         self.bytes = nbytes
         self.image = None
         self.buf = None
+        self.rse = np.inf
+        # precision jobs: the histories planned so far (a first slice, then
+        # estimates from the ROI error), within the job's cap
+        self.need = min(job["histories"], first_slice) if job.get("rse_target") else job["histories"]
+        self.roi = scenes.dose_job_roi(job) if job.get("rse_target") else None
+
+    def finished(self):
+        if self.done >= self.job["histories"]:
+            return True
+        return self.roi is not None and self.rse <= self.job["rse_target"]
 
 
 class Server:
@@ -102,6 +122,7 @@ class Server:
     `scenes.dose_phantom`), so that a client can hand over its own data.
     dose: format of the dose read back ("f32" or "f64", see `read_dose`);
     on_result(job, a, b, image) receives it (the buffers are reused).
+    first_slice: the histories of a precision job's first round at most.
     admit_slice: seconds of job set-up per round at most while others run
     (the next round starts only when the loop gets back to it); a set-up is
     not started when its estimated time (voxels / admit_voxels_per_s: the
@@ -110,7 +131,7 @@ class Server:
 
     def __init__(self, ctx, budget_bytes=None, round_histories=30_000_000, min_slice=100_000, max_active=512,
                  graphs=True, phantoms=None, on_result=None, dose="f32", admit_slice=0.01,
-                 admit_voxels_per_s=1.0e10):
+                 admit_voxels_per_s=1.0e10, first_slice=1_000_000):
         self.ctx = ctx
         if budget_bytes is None:
             free, _ = ctx.mem_info()
@@ -125,12 +146,14 @@ class Server:
         self.dose = dose
         self.admit_slice = float(admit_slice)
         self.admit_voxels_per_s = float(admit_voxels_per_s)
+        self.first_slice = int(first_slice)
         self._buf = ctx.pinned_pool  # f64 read buffers by size, kept by the context
         self._free = ctx.pinned_pool  # free pinned float32 buffers by size, kept by the context
         self.rounds = 0
         self.kernel_ms = 0.0
         self.counters = {}
-        self.host_s = dict(admit=0.0, read=0.0, wait=0.0, release=0.0, launch=0.0)
+        self.host_s = dict(admit=0.0, read=0.0, wait=0.0, release=0.0, launch=0.0, roi=0.0)
+        self.round_log = []  # per round: the launch's event counters and kernel time
 
     # ------------------------------------------------------------------ helpers
     def _pinned(self, n):
@@ -150,22 +173,46 @@ class Server:
         # what the job holds now, plus the FP32 dose it stages for its read
         # (held from its last round until the copy is done)
         staged = 8 * int(np.prod(job["shape"])) if self.dose == "f32" else 0
-        return _Active(job, res, slot, ctx.job_device_bytes(slot) + staged)
+        return _Active(job, res, slot, ctx.job_device_bytes(slot) + staged, self.first_slice)
 
     def _read(self, a):
         """Copies a finished job's results back (overlaps a round in flight)."""
         ctx = self.ctx
         ctx.job_select(a.slot)
-        d1, d2 = read_dose(ctx, a.job, self._buf, self.dose)
+        d1, d2 = read_dose(ctx, a.job, self._buf, self.dose, a.done)
         image = ctx.tally(0)["image"]
         a.res.summary = self.on_result(a.job, d1, d2, image) if self.on_result else None
         a.res.histories = a.done
 
+    def _check_precision(self, active):
+        """The ROI error of every precision job that has run, after a round:
+        one device reduction over all their ROI boxes.  A job above its
+        target plans done x (error / target)^2 histories (10 % more, at
+        least one more slice, at most its cap)."""
+        prec = [a for a in active if a.roi is not None and a.done > 0 and not a.finished()]
+        if not prec:
+            return
+        jobs, boxes = [], []
+        for a in prec:
+            jobs += [a.slot] * len(a.roi)
+            boxes += a.roi
+        st = self.ctx.roi_stats(jobs, boxes)
+        k = 0
+        for a in prec:
+            rse = scenes.roi_rse(st[k:k + len(a.roi)])
+            k += len(a.roi)
+            a.rse = float(rse.max())
+            a.res.roi_rse = a.rse
+            t = a.job["rse_target"]
+            if a.rse > t:
+                est = 4 * a.done if not np.isfinite(a.rse) else int(a.done * (a.rse / t) ** 2 * 1.1)
+                a.need = int(min(a.job["histories"], max(est, a.done + self.min_slice)))
+
     def _plan(self, active):
         """Histories of each admitted job in the next round: equal shares of
         round_histories, the share of a job that has fewer left passed on to
-        the others."""
-        left = np.array([a.job["histories"] - a.done for a in active], dtype=np.int64)
+        the others (a precision job's planned histories count as its total)."""
+        left = np.array([a.need - a.done for a in active], dtype=np.int64)
         take = np.zeros_like(left)
         budget = max(self.round_histories, self.min_slice * len(active))
         open_ = left > 0
@@ -244,7 +291,7 @@ class Server:
                 waiting.popleft()
                 a = self._admit(jobs[i], results[i], clock())
                 resident += a.bytes
-                queued += a.job["histories"]
+                queued += a.need
                 admitted.append(a)
             t3 = clock()
             self.host_s["admit"] += t3 - t2
@@ -253,9 +300,13 @@ class Server:
                 self.host_s["wait"] += clock() - t3
                 self.rounds += 1
                 self.kernel_ms += st["kernel_ms"]
+                self.round_log.append(st)
                 for k, v in st.items():
                     if k != "kernel_ms":
                         self.counters[k] = self.counters.get(k, 0) + v
+                t4 = clock()
+                self._check_precision(active)
+                self.host_s["roi"] += clock() - t4
             t = clock()
             # finished copies: results out, device memory freed
             block = not launched and not admitted
@@ -281,12 +332,12 @@ class Server:
                     resident -= a.bytes
             self.host_s["release"] += clock() - t
             idle = not launched and not to_read and not admitted and not copying
-            to_read = [a for a in active if a.done >= a.job["histories"]]
-            active = [a for a in active if a.done < a.job["histories"]] + admitted
+            to_read = [a for a in active if a.finished()]
+            active = [a for a in active if not a.finished()] + admitted
             if self.dose == "f32":
                 for a in to_read:  # conversion queued ahead of the next round; the copies run beside it
                     ctx.job_select(a.slot)
-                    ctx.stage_voxel_dose_f32(0, a.job["histories"])
+                    ctx.stage_voxel_dose_f32(0, a.done)
             if idle and order:
                 time.sleep(max(0.0, min(arrivals[order[0]] - clock(), 0.05)))
         self.makespan = clock()
@@ -316,6 +367,9 @@ def run_plain(ctx, jobs, arrivals=None, phantoms=None, on_result=None, dose="f32
         scenes.setup_dose_job(ctx, j, ids=phantoms(j))
         tb = time.perf_counter()
         st = ctx.run(j["histories"], seed=j["seed"], first=0, kernel=KERNEL_PLAIN)
+        if j.get("rse_target"):  # the same ROI error the served loop stops on
+            boxes = scenes.dose_job_roi(j)
+            results[i].roi_rse = float(scenes.roi_rse(ctx.roi_stats([slot] * len(boxes), boxes)).max())
         tc = time.perf_counter()
         kernel_ms += st["kernel_ms"]
         for k, v in st.items():

@@ -21,6 +21,8 @@ class FakeCtx:
         self.bytes = {}
         self.admit_order = []
         self.pinned_pool = {}
+        self.hist = {}  # histories run per slot
+        self.rate = {}  # ROI deposits per history, per slot
 
     # calls the server makes
     def set_graphs(self, on):
@@ -69,6 +71,18 @@ class FakeCtx:
     def run_jobs_async(self, slots, n, first, seeds):
         assert self.inflight is None
         self.inflight = list(zip(slots, n, first, seeds))
+        for slot, k, _, _ in self.inflight:
+            self.hist[slot] = self.hist.get(slot, 0) + k
+
+    def roi_stats(self, jobs, boxes):
+        """Deposits in each box at rate[slot] per history, each of unit mean
+        and variance: the ROI error is 1 / sqrt(rate x histories)."""
+        assert self.inflight is None and len(jobs) == len(boxes)
+        out = []
+        for slot in jobs:
+            n = self.rate.get(slot, 0.01) * self.hist.get(slot, 0)
+            out.append((n, n, 2.0 * n))
+        return np.array(out)
 
     def wait(self):
         self.rounds.append(self.inflight)
@@ -179,8 +193,36 @@ def test_round_plan_shares_equally():
         def __init__(self, h, done):
             self.job = {"histories": h}
             self.done = done
+            self.need = h
 
     take = srv._plan([A(10_000, 0), A(50, 0), A(10_000, 9_990), A(10_000, 0)])
     assert take[1] == 50 and take[2] == 10
     assert take[0] == take[3] and take.sum() >= 1000
     assert take.sum() <= 1000 + 4 * 10
+
+
+def test_precision_jobs_stop_at_their_target():
+    """Jobs with rse_target run until the ROI error (here 1 / sqrt(rate x
+    histories)) reaches the target: each stops with histories within a
+    round's growth of (1 / (target^2 rate)), none past its cap, and a job
+    whose cap comes first stops at the cap."""
+    ctx = FakeCtx()
+    jobs = [dict(j, histories=10 ** 9, rse_target=0.01, shape=(16, 16, 16), voxel_cm=0.25) for j in small_jobs(8, 5)]
+    jobs[-1]["histories"] = 300_000  # its cap comes first
+    rates = [0.01, 0.02, 0.05, 0.1, 0.01, 0.03, 0.2, 0.01]
+    orig = ctx.job_create
+
+    def create():
+        s = orig()
+        ctx.rate[s] = rates[s - 1]
+        return s
+    ctx.job_create = create
+    srv = Server(ctx, budget_bytes=10 ** 10, round_histories=2_000_000, min_slice=10_000, first_slice=100_000)
+    res = srv.run(jobs)
+    for r, j, rate in zip(res, jobs, rates):
+        need = 1.0 / (j["rse_target"] ** 2 * rate)
+        if j["histories"] < need:
+            assert r.histories == j["histories"]
+        else:
+            assert need <= r.histories <= 1.35 * need + 100_000, (r.histories, need)
+            assert r.roi_rse <= j["rse_target"]
