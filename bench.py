@@ -222,6 +222,8 @@ def served_stream(ctx, n_jobs, seed, load, cap, plain=True):
         "rounds": srv.rounds,
         "host_s": {k: round(v, 4) for k, v in srv.host_s.items()},
         "steps_per_history": srv.counters["steps"] / H,
+        "events_per_history": {k: srv.counters[k] / H for k in ("steps", "real", "rayleigh", "compton", "photo",
+                                                                 "kahn_trials", "deposits", "surface_hits")},
         "plain_steps_per_history": pinfo["counters"]["steps"] / H,
         "roi_rse": {"served_max": float(np.max(served_rse)), "served_median": float(np.median(served_rse)),
                     "plain_max": float(np.max(plain_rse)), "plain_median": float(np.median(plain_rse))},
