@@ -142,6 +142,12 @@ def roofline(stats_list, traffic=None, source=None):
     return out
 
 
+def event_kernel_name(ctx):
+    import midsx_amd as M
+    kind, bank = ctx.event_kernel()
+    return "bank%d" % bank if kind == M.EVENT_BANK else "queue"
+
+
 def served_stream(ctx, n_jobs, seed, load, cap, plain=True):
     """The served job stream at 1 % ROI error: the continuous-batching loop
     against plain FIFO on the same history counts."""
@@ -167,7 +173,7 @@ def served_stream(ctx, n_jobs, seed, load, cap, plain=True):
     probe = next((j for j in jobs if j["shape"][0] == 256), jobs[0])
     pj = ctx.job_create()
     scenes.setup_dose_job(ctx, probe, ids=ph(probe))
-    tuned = ctx.autotune(probe_histories=4000000)
+    tuned = ctx.autotune(probe_histories=4000000, rounds=True)  # the rounds' multi-job kernels
     ctx.job_release(pj)
     ctx.job_select(keep_job)
     # an untimed pass first: the device memory pool and the pinned result
@@ -260,6 +266,7 @@ def radiography(ctx, args):
                     "256x256 image + FP64 voxel dose, %d steps of %d histories" % (args.steps, n),
         "value": total / wall, "unit": "histories/s", "ms_per_step": wall / args.steps * 1e3,
         "queue_params": dict(zip(("blocks_per_cu", "refill_min", "interact_min"), ctx.queue_params())),
+        "event_kernel": event_kernel_name(ctx),
     }
     agg = {k: sum(s[k] for s in stats) for k in M.STAT_NAMES}
     out["events_per_history"] = {k: agg[k] / total for k in ("steps", "real", "rayleigh", "compton", "photo",
@@ -309,7 +316,9 @@ def ct_scan(ctx, n_views, scale, plain_every, seed=4):
     seeds = [10_000 + v for v in range(n_views)]
     H = sum(n)
     ctx.job_select(views[0])
-    tuned = ctx.autotune(probe_histories=4000000)  # one view: launches of ~15 ms time stably
+    # one view, probed as the multi-job launch the views run in: launches of
+    # ~15 ms time stably
+    tuned = ctx.autotune(probe_histories=4000000, rounds=True)
     ctx.set_graphs(True)
     ctx.run_jobs(views[:8], [100000] * 8, [1 << 40] * 8, seeds[:8])  # warm (capture)
     for j in [owner] + views:
@@ -365,8 +374,10 @@ def precision_study(ctx, n_hist, step=10 ** 8):
     from midsx_amd import scenes
     job = ctx.job_create()
     scenes.setup_slab(ctx)
-    keep = ctx.queue_params()
-    tuned = ctx.autotune(probe_histories=4000000)  # the slab's own launch parameters
+    keep, keep_ev = ctx.queue_params(), ctx.event_kernel()
+    # the slab's own launch parameters on the lane-resident kernel, which
+    # every mode (the per-block LDS one included) runs: identical RNG streams
+    tuned = ctx.autotune(probe_histories=4000000, events=False)
     out = {}
     res = {}
     for name, mode in (("fp64", M.DOSE_F64), ("fp32", M.DOSE_F32), ("fp32_compensated", M.DOSE_F32C),
@@ -393,6 +404,7 @@ def precision_study(ctx, n_hist, step=10 ** 8):
                           "rel_err_sum2_max": float(e2.max()), "same_deposits": bool(np.array_equal(cnt, c64))})
     ctx.job_release(job)
     ctx.set_queue_params(*keep)
+    ctx.set_event_kernel(*keep_ev)
     return {"workload": "slab 40x40x10 water voxels, 60 keV beam, %d histories per mode, identical RNG streams; "
                         "errors over the %d voxels with >= 1000 deposits" % (n_hist, int(roi.sum())),
             "queue_params": {k: tuned[k] for k in ("blocks_per_cu", "refill_min", "interact_min")},

@@ -198,13 +198,35 @@ int mdx_run(mdx_ctx* ctx, uint64_t n_histories, uint64_t seed, uint64_t first_hi
  * (the reference's only parallel knob is the OpenMP thread count). */
 int mdx_set_queue_params(mdx_ctx* ctx, int blocks_per_cu, int refill_min, int interact_min);
 int mdx_get_queue_params(mdx_ctx* ctx, int* blocks_per_cu, int* refill_min, int* interact_min);
-/* Autotuner: times the queue kernel on the selected job with probe_histories
- * per launch over a coordinate search of the three parameters, keeps the
- * fastest, and resets that job's tallies; for a job that uses another job's
- * domain (mdx_job_alias_domain), also that domain's voxel and background
- * dose, where the probes deposited.  Outputs may be NULL. */
-int mdx_autotune(mdx_ctx* ctx, uint64_t probe_histories, int* blocks_per_cu, int* refill_min, int* interact_min,
-                 double* best_ms);
+/* The event form MDX_KERNEL_QUEUE runs (mdx_run, mdx_run_jobs and their
+ * variants): MDX_EVENT_QUEUE, every lane carrying one photon through
+ * wave-level refill / step / interaction phases, or MDX_EVENT_BANK, every
+ * wave keeping a bank of bank_size photons (96 or 128) in LDS with one list
+ * of bank slots per next event (Woodcock step, background flight, real
+ * interaction, free) and running each event on up to 64 photons of one
+ * list at a time.  Same physics and tallies; the per-block LDS dose mode
+ * (MDX_DOSE_F32C_LDS) always runs on MDX_EVENT_QUEUE.  No reference
+ * counterpart (the reference runs one history per OpenMP thread,
+ * run_simulation.cpp:15-35). */
+#define MDX_EVENT_QUEUE 0
+#define MDX_EVENT_BANK 1
+int mdx_set_event_kernel(mdx_ctx* ctx, int kind, int bank_size);
+int mdx_get_event_kernel(mdx_ctx* ctx, int* kind, int* bank_size);
+/* Autotuner: times MDX_KERNEL_QUEUE launches of probe_histories on the
+ * selected job over a coordinate search of the lane-resident event kernel's
+ * three parameters and, with MDX_TUNE_EVENTS, of the bank kernel's (bank
+ * size, blocks per CU, refill and interaction batches), keeps the fastest
+ * event kernel and parameters (mdx_get_event_kernel, mdx_get_queue_params),
+ * and resets that job's tallies; for a job that uses another job's domain
+ * (mdx_job_alias_domain), also that domain's voxel and background dose,
+ * where the probes deposited.  With MDX_TUNE_ROUNDS each probe runs the job
+ * as two slices of one launch, i.e. the multi-job kernels that
+ * mdx_run_jobs and graph rounds launch for several jobs (their register and
+ * LDS budgets differ from the single-job kernels').  Outputs may be NULL. */
+#define MDX_TUNE_ROUNDS 1
+#define MDX_TUNE_EVENTS 2
+int mdx_autotune(mdx_ctx* ctx, uint64_t probe_histories, int flags, int* blocks_per_cu, int* refill_min,
+                 int* interact_min, double* best_ms);
 
 /* Jobs.  A context holds several scenes ("jobs"), each with its own domain,
  * grids, source and tallies, all sharing the context's tables; the per-scene

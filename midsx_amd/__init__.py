@@ -24,6 +24,8 @@ SPEC_MONO, SPEC_POLY = 0, 1
 TALLY_AACUBOID, TALLY_DISC, TALLY_RECT, TALLY_IMAGE = 0, 1, 2, 3
 XS_TOTAL, XS_COHERENT, XS_INCOHERENT, XS_PHOTOELECTRIC, SF_INCOHERENT, FF_COHERENT = 0, 1, 2, 3, 4, 5
 KERNEL_PLAIN, KERNEL_QUEUE = 0, 1
+EVENT_QUEUE, EVENT_BANK = 0, 1
+TUNE_ROUNDS, TUNE_EVENTS = 1, 2
 DOSE_F64, DOSE_F32, DOSE_F32C, DOSE_F32C_LDS = 0, 1, 2, 3
 NQ, NBUCKET, NFIELD = 4, 4, 3
 STAT_NAMES = ["histories", "steps", "real", "rayleigh", "compton", "photo", "kahn_trials", "kahn_s_tests",
@@ -46,7 +48,7 @@ EXPORTS = [
     "mdx_set_flights", "mdx_set_rita_builder", "mdx_set_dose_precision", "mdx_get_voxel_dose_f32", "mdx_stage_voxel_dose_f32",
     "mdx_read_staged_dose_f32_async", "mdx_read_done", "mdx_copy_probe", "mdx_air_muen", "mdx_set_air_muen_table",
     "mdx_set_local_majorants", "mdx_get_macro_cells", "mdx_delta_decision", "mdx_philox",
-    "mdx_sample_source", "mdx_run_jobs_noalloc", "mdx_roi_stats",
+    "mdx_sample_source", "mdx_run_jobs_noalloc", "mdx_roi_stats", "mdx_set_event_kernel", "mdx_get_event_kernel",
 ]
 
 
@@ -357,13 +359,29 @@ class Context:
         self._call(self.lib.mdx_set_queue_params, self._h, ctypes.c_int(blocks_per_cu), ctypes.c_int(refill_min),
                    ctypes.c_int(interact_min))
 
-    def autotune(self, probe_histories=1000000):
-        """Tunes the queue kernel on the selected job (resets its tallies)."""
+    def autotune(self, probe_histories=1000000, rounds=False, events=True):
+        """Tunes the event kernel and its launch parameters on the selected job
+        (resets its tallies): rounds=True probes the multi-job kernels that
+        run_jobs and graph rounds launch for several jobs; events=False keeps
+        to the lane-resident queue kernel."""
         a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         ms = ctypes.c_double()
-        self._call(self.lib.mdx_autotune, self._h, ctypes.c_uint64(probe_histories), ctypes.byref(a),
-                   ctypes.byref(b), ctypes.byref(c), ctypes.byref(ms))
-        return dict(blocks_per_cu=a.value, refill_min=b.value, interact_min=c.value, probe_ms=ms.value)
+        flags = (TUNE_ROUNDS if rounds else 0) | (TUNE_EVENTS if events else 0)
+        self._call(self.lib.mdx_autotune, self._h, ctypes.c_uint64(probe_histories), ctypes.c_int(flags),
+                   ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(ms))
+        kind, bank = self.event_kernel()
+        return dict(blocks_per_cu=a.value, refill_min=b.value, interact_min=c.value, probe_ms=ms.value,
+                    event_kernel="bank" if kind == EVENT_BANK else "queue", bank_size=bank)
+
+    def set_event_kernel(self, kind=EVENT_QUEUE, bank_size=128):
+        """The event form of KERNEL_QUEUE runs: EVENT_QUEUE (lane-resident
+        photons) or EVENT_BANK (per-wave photon banks of bank_size in LDS)."""
+        self._call(self.lib.mdx_set_event_kernel, self._h, ctypes.c_int(kind), ctypes.c_int(bank_size))
+
+    def event_kernel(self):
+        k, b = ctypes.c_int(), ctypes.c_int()
+        self._call(self.lib.mdx_get_event_kernel, self._h, ctypes.byref(k), ctypes.byref(b))
+        return k.value, b.value
 
     def queue_params(self):
         a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

@@ -293,6 +293,8 @@ struct mdx_ctx {
   int queue_blocks_per_cu = 4;
   int refill_min = 32;
   int interact_min = 32;
+  int event_kernel = MDX_EVENT_QUEUE;  // the event form mdx_run_jobs launches (mdx_set_event_kernel)
+  int bank_size = 128;                 // photons per wave bank of MDX_EVENT_BANK
   DBuf d_work;  // queue kernel: histories claimed, one counter per job of a launch
   DBuf d_jobs, d_prefix;
   DBuf d_dose32;  // staging of mdx_get_voxel_dose_f32 (on the I/O stream)
@@ -308,7 +310,7 @@ struct mdx_ctx {
   void* h_round = nullptr;  // pinned
   DBuf d_round;
   hipGraphExec_t graph = nullptr;
-  int graph_params[3] = {0, 0, 0};
+  int graph_params[5] = {0, 0, 0, 0, 0};
 
   void ensure_registry() {
     if (registry_loaded) return;
@@ -948,23 +950,45 @@ void build_graph(mdx_ctx* c) {
   hip_check(mdx_launch::zero_list(reinterpret_cast<const uint64_t*>(d + kRoundClearOff), c->stream), "capture clear");
   hip_check(mdx_launch::zero_u64(c->d_counters.as<unsigned long long>(), MDX_NCOUNTER, c->stream), "capture zero");
   hip_check(mdx_launch::zero_u64(c->d_work.as<unsigned long long>(), kRoundMax, c->stream), "capture zero");
-  hip_check(mdx_launch::transport_queue_round(c->d_tables.as<mdx_tables_dev>(), d, c->d_work.as<unsigned long long>(),
-                                              c->d_counters.as<unsigned long long>(),
-                                              round_blocks(c), c->refill_min, c->interact_min,
-                                              c->stream),
-            "capture kernel");
+  if (c->event_kernel == MDX_EVENT_BANK)
+    hip_check(mdx_launch::transport_bank_round(c->d_tables.as<mdx_tables_dev>(), d, c->d_work.as<unsigned long long>(),
+                                               c->d_counters.as<unsigned long long>(), round_blocks(c), c->refill_min,
+                                               c->interact_min, c->bank_size, c->stream),
+              "capture kernel");
+  else
+    hip_check(mdx_launch::transport_queue_round(c->d_tables.as<mdx_tables_dev>(), d,
+                                                c->d_work.as<unsigned long long>(),
+                                                c->d_counters.as<unsigned long long>(), round_blocks(c),
+                                                c->refill_min, c->interact_min, c->stream),
+              "capture kernel");
   hip_check(hipStreamEndCapture(c->stream, &g), "end capture");
   hip_check(hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0), "hipGraphInstantiate");
   (void)hipGraphDestroy(g);
   c->graph_params[0] = c->queue_blocks_per_cu;
   c->graph_params[1] = c->refill_min;
   c->graph_params[2] = c->interact_min;
+  c->graph_params[3] = c->event_kernel;
+  c->graph_params[4] = c->bank_size;
 }
 
 void finish_round(mdx_ctx* c, mdx_stats* stats) {
   hip_check(hipStreamSynchronize(c->stream), "transport");
   c->inflight = false;
   read_stats(c, stats);
+}
+
+// The selected event kernel over job records already on the device (the
+// lane-resident queue kernel for the per-block LDS dose mode, which the bank
+// kernel does not keep).
+hipError_t launch_events(mdx_ctx* c, const mdx_job_dev* jobs, const uint64_t* prefix, int n_jobs, uint64_t total,
+                         unsigned long long* work, unsigned long long* cnt, uint32_t lds_voxels) {
+  if (c->event_kernel == MDX_EVENT_BANK && lds_voxels == 0)
+    return mdx_launch::transport_bank(c->d_tables.as<mdx_tables_dev>(), jobs, prefix, n_jobs, total, work, cnt,
+                                      c->num_cu * c->queue_blocks_per_cu, c->refill_min, c->interact_min,
+                                      c->bank_size, c->stream);
+  return mdx_launch::transport_queue(c->d_tables.as<mdx_tables_dev>(), jobs, prefix, n_jobs, total, work, cnt,
+                                     c->num_cu * c->queue_blocks_per_cu, c->refill_min, c->interact_min, lds_voxels,
+                                     c->stream);
 }
 
 // One launch of the queue kernel over the listed jobs' history slices;
@@ -978,7 +1002,8 @@ void run_jobs(mdx_ctx* c, int n_jobs, const int* ids, const uint64_t* first, con
   if (c->use_graphs) {
     need(n_jobs <= kRoundMax, "too many jobs in one graph round (max 1024)");
     if (!c->graph || c->graph_params[0] != c->queue_blocks_per_cu || c->graph_params[1] != c->refill_min ||
-        c->graph_params[2] != c->interact_min)
+        c->graph_params[2] != c->interact_min || c->graph_params[3] != c->event_kernel ||
+        c->graph_params[4] != c->bank_size)
       build_graph(c);
     char* h = static_cast<char*>(c->h_round);
     uint64_t* hdr = reinterpret_cast<uint64_t*>(h);
@@ -1017,10 +1042,8 @@ void run_jobs(mdx_ctx* c, int n_jobs, const int* ids, const uint64_t* first, con
   hip_check(mdx_launch::zero_u64(c->d_work.as<unsigned long long>(), std::max(n_jobs, 1), c->stream), "zero work");
   hip_check(hipEventRecord(c->ev0, c->stream), "hipEventRecord");
   if (total > 0)
-    hip_check(mdx_launch::transport_queue(c->d_tables.as<mdx_tables_dev>(), c->d_jobs.as<mdx_job_dev>(),
-                                          c->d_prefix.as<uint64_t>(), n_jobs, total,
-                                          c->d_work.as<unsigned long long>(), cnt, c->num_cu * c->queue_blocks_per_cu,
-                                          c->refill_min, c->interact_min, lds_voxels, c->stream),
+    hip_check(launch_events(c, c->d_jobs.as<mdx_job_dev>(), c->d_prefix.as<uint64_t>(), n_jobs, total,
+                            c->d_work.as<unsigned long long>(), cnt, lds_voxels),
               "queue launch");
   hip_check(hipEventRecord(c->ev1, c->stream), "hipEventRecord");
   c->inflight = true;
@@ -1509,6 +1532,24 @@ int mdx_set_queue_params(mdx_ctx* c, int blocks_per_cu, int refill_min, int inte
   });
 }
 
+int mdx_set_event_kernel(mdx_ctx* c, int kind, int bank_size) {
+  return guard([&] {
+    need(c, "NULL context");
+    need(kind == MDX_EVENT_QUEUE || kind == MDX_EVENT_BANK, "unknown event kernel");
+    need(bank_size == 96 || bank_size == 128, "bank_size must be 96 or 128");
+    c->event_kernel = kind;
+    c->bank_size = bank_size;
+  });
+}
+
+int mdx_get_event_kernel(mdx_ctx* c, int* kind, int* bank_size) {
+  return guard([&] {
+    need(c && kind && bank_size, "NULL argument");
+    *kind = c->event_kernel;
+    *bank_size = c->bank_size;
+  });
+}
+
 int mdx_get_queue_params(mdx_ctx* c, int* blocks_per_cu, int* refill_min, int* interact_min) {
   return guard([&] {
     need(c && blocks_per_cu && refill_min && interact_min, "NULL argument");
@@ -1518,53 +1559,109 @@ int mdx_get_queue_params(mdx_ctx* c, int* blocks_per_cu, int* refill_min, int* i
   });
 }
 
-int mdx_autotune(mdx_ctx* c, uint64_t probe_histories, int* blocks_per_cu, int* refill_min, int* interact_min,
-                 double* best_ms) {
+int mdx_autotune(mdx_ctx* c, uint64_t probe_histories, int flags, int* blocks_per_cu, int* refill_min,
+                 int* interact_min, double* best_ms) {
   return guard([&] {
     need(c, "NULL context");
-    need(probe_histories > 0, "probe_histories must be positive");
+    need(probe_histories > 1, "probe_histories must be at least 2");
     need(!c->inflight, "a round is in flight (mdx_wait first)");
+    need((flags & ~(MDX_TUNE_ROUNDS | MDX_TUNE_EVENTS)) == 0, "unknown autotune flags");
     const int job = c->cur;
+    const bool rounds = (flags & MDX_TUNE_ROUNDS) != 0;
     const uint64_t seed = 0x7475u;
     uint64_t first = 1ull << 40;  // a stream region no production run uses
-    auto probe = [&](int b, int r, int i) {
+    // the probe launch: the job alone (single-job kernels) or as two slices
+    // of itself in one launch (the multi-job kernels that rounds run, with
+    // their own register and LDS budgets)
+    auto launch = [&](mdx_stats* st) {
+      if (rounds) {
+        const int ids[2] = {job, job};
+        const uint64_t h0 = probe_histories / 2;
+        const uint64_t f[2] = {first, first + h0}, n[2] = {h0, probe_histories - h0}, s[2] = {seed, seed};
+        run_jobs(c, 2, ids, f, n, s, st);
+      } else {
+        run_jobs(c, 1, &job, &first, &probe_histories, &seed, st);
+      }
+      first += probe_histories;
+    };
+    auto probe = [&](int kind, int bank, int b, int r, int i) {
+      c->event_kernel = kind;
+      c->bank_size = bank;
       c->queue_blocks_per_cu = b;
       c->refill_min = r;
       c->interact_min = i;
       double best = 1e300;
       for (int rep = 0; rep < 2; ++rep) {
         mdx_stats st;
-        run_jobs(c, 1, &job, &first, &probe_histories, &seed, &st);
-        first += probe_histories;
+        launch(&st);
         best = std::min(best, st.kernel_ms);
       }
       return best;
     };
+    int kind = MDX_EVENT_QUEUE, bank = c->bank_size;
     int b = c->queue_blocks_per_cu, r = c->refill_min, i = c->interact_min;
-    double cur = probe(b, r, i);
-    // coordinate search: interaction batch, refill batch (up to 62 of a
-    // wave's 64 lanes: a source far from the grid wants its photons' free
-    // flights in large batches), both again, then
-    // occupancy (the kernel's 128 registers allow four waves per SIMD, i.e.
-    // four blocks of 256 lanes per CU; three is the only smaller choice worth
-    // trying, more blocks than fit only queue behind the resident ones)
+    if (c->event_kernel != MDX_EVENT_QUEUE) b = 4, r = 32, i = 32;
+    double cur = probe(kind, bank, b, r, i);
+    // queue kernel, coordinate search: interaction batch, refill batch (up to
+    // 62 of a wave's 64 lanes: a source far from the grid wants its photons'
+    // free flights in large batches), both again, then occupancy (the
+    // kernel's 128 registers allow four waves per SIMD, i.e. four blocks of
+    // 256 lanes per CU; three is the only smaller choice worth trying, more
+    // blocks than fit only queue behind the resident ones)
     for (int pass = 0; pass < 2; ++pass) {
       for (int v : {8, 16, 24, 32, 40, 48}) {
         if (v == i) continue;
-        double t = probe(b, r, v);
+        double t = probe(kind, bank, b, r, v);
         if (t < cur) { cur = t; i = v; }
       }
       for (int v : {8, 16, 24, 32, 40, 48, 56, 60, 62}) {
         if (v == r) continue;
-        double t = probe(b, v, i);
+        double t = probe(kind, bank, b, v, i);
         if (t < cur) { cur = t; r = v; }
       }
     }
     for (int v : {3, 4}) {
       if (v == b) continue;
-      double t = probe(v, r, i);
+      double t = probe(kind, bank, v, r, i);
       if (t < cur) { cur = t; b = v; }
     }
+    // the bank kernel (not for the per-block LDS dose mode, which always runs
+    // on the queue kernel): both bank sizes at two or three blocks per CU
+    // (their LDS fits three blocks of the single-job kernel with 96-photon
+    // banks, two otherwise), then its refill and interaction batches
+    const Job& O = job_at(c, domain_owner(c, job));
+    const bool lds_dose = O.voxel_dose && O.dose_mode == MDX_DOSE_F32C_LDS && !rounds;
+    if ((flags & MDX_TUNE_EVENTS) && !lds_dose) {
+      int kb = 96, bb = 2, rb = 64, ib = 32;
+      double tb = 1e300;
+      for (int bs : {96, 128})
+        for (int v : {2, 3}) {
+          double t = probe(MDX_EVENT_BANK, bs, v, rb, ib);
+          if (t < tb) { tb = t; kb = bs; bb = v; }
+        }
+      for (int pass = 0; pass < 2; ++pass) {
+        for (int v : {16, 24, 32, 48, 64}) {
+          if (v == ib) continue;
+          double t = probe(MDX_EVENT_BANK, kb, bb, rb, v);
+          if (t < tb) { tb = t; ib = v; }
+        }
+        for (int v : {24, 32, 48, 64}) {
+          if (v == rb) continue;
+          double t = probe(MDX_EVENT_BANK, kb, bb, v, ib);
+          if (t < tb) { tb = t; rb = v; }
+        }
+      }
+      if (tb < cur) {
+        cur = tb;
+        kind = MDX_EVENT_BANK;
+        bank = kb;
+        b = bb;
+        r = rb;
+        i = ib;
+      }
+    }
+    c->event_kernel = kind;
+    c->bank_size = bank;
     c->queue_blocks_per_cu = b;
     c->refill_min = r;
     c->interact_min = i;
@@ -1809,11 +1906,7 @@ int mdx_run_jobs_noalloc(mdx_ctx* c, int n_jobs, const int* ids, const uint64_t*
     hip_check(mdx_launch::zero_u64(work, n_jobs, c->stream), "zero work");
     const uint64_t total = prefix[n_jobs];
     hip_check(hipEventRecord(c->ev0, c->stream), "hipEventRecord");
-    if (total > 0)
-      hip_check(mdx_launch::transport_queue(c->d_tables.as<mdx_tables_dev>(), djobs, dprefix, n_jobs, total, work, cnt,
-                                            c->num_cu * c->queue_blocks_per_cu, c->refill_min, c->interact_min,
-                                            lds_voxels, c->stream),
-                "queue launch");
+    if (total > 0) hip_check(launch_events(c, djobs, dprefix, n_jobs, total, work, cnt, lds_voxels), "queue launch");
     hip_check(hipEventRecord(c->ev1, c->stream), "hipEventRecord");
     unsigned long long h[MDX_NCOUNTER];
     hip_check(hipMemcpyAsync(h, cnt, sizeof h, hipMemcpyDeviceToHost, c->stream), "counters");

@@ -35,6 +35,15 @@ size_t transport_queue_static_lds(bool on_device);
 hipError_t transport_queue_round(const mdx_tables_dev* T, const char* round, unsigned long long* work,
                                  unsigned long long* counters, int grid_blocks, int refill_min, int interact_min,
                                  hipStream_t st);
+// The banked event kernel (photon banks of `bank` slots per wave, 96 or
+// 128, in LDS; full-lane batches of one event type): the same contract as
+// transport_queue / transport_queue_round.
+hipError_t transport_bank(const mdx_tables_dev* T, const mdx_job_dev* jobs, const uint64_t* prefix, int n_jobs,
+                          uint64_t total, unsigned long long* work, unsigned long long* counters, int grid_blocks,
+                          int refill_min, int interact_min, int bank, hipStream_t st);
+hipError_t transport_bank_round(const mdx_tables_dev* T, const char* round, unsigned long long* work,
+                                unsigned long long* counters, int grid_blocks, int refill_min, int interact_min,
+                                int bank, hipStream_t st);
 hipError_t eval_curve(const mdx_tables_dev* T, int slot, int which, int n, const float* x, float* out,
                       hipStream_t st);
 hipError_t rita(const mdx_tables_dev* T, int slot, int n, const float* E, const float* y, float* mu,

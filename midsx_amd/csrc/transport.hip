@@ -1151,6 +1151,500 @@ k_transport_queue_round(const mdx_tables_dev* __restrict__ T, const char* __rest
   transport_queue_body<true>(T, jobs, prefix, n_jobs, total, work, counters, refill_min, interact_min, 0u);
 }
 
+// ------------------------------------------------------------------ bank kernel
+// The event form with photon banks.  Every wave keeps a bank of B photons in
+// LDS (structure of arrays, ~100 B each) and a list of bank slots per next
+// event: a Woodcock step inside a grid, a free flight through the
+// background, a parked real interaction, or a free slot.  Each iteration the
+// wave takes up to 64 photons from one list (the fullest step or flight
+// list, the interaction list once it holds interact_min photons, new source
+// photons once refill_min slots are free), loads their state into its lanes,
+// runs that one event for all of them, stores the state back and appends
+// each slot to the list of its photon's next event: ballot per event type,
+// the lane's rank among the lanes of its type (popcount of the lower lanes)
+// and one count per type, all wave-uniform.  So every event runs on lanes
+// that all do the same kind of work, whatever mix of events the bank holds,
+// instead of the lanes a wave's resident photons happen to leave for it.
+// The physics of each event is that of transport_queue_body; the
+// differences are bookkeeping: the interaction stream's buffered Philox
+// words are dropped at the end of every event (its block counter lives in
+// the bank; no counter is reused), the start of an interaction's step is
+// recomputed from its end and length for the tally replay, and a photon
+// that interacts in the background resumes with a free flight.
+// A wave holds the photons of one job at a time (MULTI: the next job with
+// histories left once its own has run out and its bank is empty).
+enum { kEvStep = 0, kEvFly = 1, kEvInt = 2, kEvFree = 3, kEvTypes = 4 };
+constexpr int kBankCache = 6;  // cached total cross sections: slots 0 .. 4 and the global majorant (entry 5)
+
+template <int B>
+struct WaveBank {
+  mdx_v4 pe[B];   // position, energy
+  mdx_v4 dm[B];   // direction, majorant of the next step
+  // safe distance, sigma_tot of the cached slot, distance to the next tally
+  // event, length of the step that ended in the pending interaction
+  mdx_v4 aux[B];
+  uint32_t hlo[B], hhi[B], c0[B], nstep[B];  // history index, Philox block counter, step counter
+  uint32_t flags[B];  // see fl_get / fl_set
+  uint32_t qlin[B];   // voxel of the pending interaction
+  float sc[kBankCache][B];
+  uint8_t list[kEvTypes][B];
+};
+
+// flags: bits 0-1 coherent scatters (saturating at 3), 2-3 incoherent, 4-11
+// majorant class (255: global), 12-19 slot of the cached sigma_tot (255:
+// none), 20-23 grid of the pending interaction + 1 (0: background), 24-29
+// valid cache entries.
+__device__ __forceinline__ uint32_t fl_get(uint32_t f, int sh, uint32_t m) { return (f >> sh) & m; }
+__device__ __forceinline__ uint32_t fl_set(uint32_t f, int sh, uint32_t m, uint32_t v) {
+  return (f & ~(m << sh)) | ((v & m) << sh);
+}
+
+// The step stream's key of a history (StepRng::init).
+__device__ __forceinline__ uint32_t step_key(uint64_t seed, uint32_t job, uint32_t hhi) {
+  return mix32(mix32((uint32_t)seed, (uint32_t)(seed >> 32)), mix32(job, hhi));
+}
+
+__device__ __forceinline__ void step_uniforms(uint32_t key, uint32_t hlo, uint32_t step, float& u1, float& u2) {
+  uint32_t x0, x1;
+  philox2x32_10(hlo, step, key, x0, x1);
+  u1 = (float)((x0 >> 8) + 1u) * (1.0f / 16777216.0f);
+  u2 = (float)((x1 >> 8) + 1u) * (1.0f / 16777216.0f);
+}
+
+// One photon in a lane while an event runs on it.
+struct Ph {
+  float px, py, pz, E, dx, dy, dz, smax, dsafe, stot, tfl, qs;
+  uint32_t hlo, hhi, c0, nstep, flags, qlin;
+};
+
+template <int B>
+__device__ __forceinline__ Ph bank_load(const WaveBank<B>* W, int s) {
+  Ph p;
+  const mdx_v4 a = W->pe[s], b = W->dm[s], c = W->aux[s];
+  p.px = a.x; p.py = a.y; p.pz = a.z; p.E = a.w;
+  p.dx = b.x; p.dy = b.y; p.dz = b.z; p.smax = b.w;
+  p.dsafe = c.x; p.stot = c.y; p.tfl = c.z; p.qs = c.w;
+  p.hlo = W->hlo[s]; p.hhi = W->hhi[s]; p.c0 = W->c0[s]; p.nstep = W->nstep[s];
+  p.flags = W->flags[s]; p.qlin = W->qlin[s];
+  return p;
+}
+
+template <int B>
+__device__ __forceinline__ void bank_store(WaveBank<B>* W, int s, const Ph& p) {
+  W->pe[s] = mdx_v4{p.px, p.py, p.pz, p.E};
+  W->dm[s] = mdx_v4{p.dx, p.dy, p.dz, p.smax};
+  W->aux[s] = mdx_v4{p.dsafe, p.stot, p.tfl, p.qs};
+  W->hlo[s] = p.hlo; W->hhi[s] = p.hhi; W->c0[s] = p.c0; W->nstep[s] = p.nstep;
+  W->flags[s] = p.flags; W->qlin[s] = p.qlin;
+}
+
+// A wave's own LDS writes made visible to its other lanes before they read.
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Total cross section of material slot (or the global majorant, slot 255)
+// at the photon's energy, from its bank cache when valid.
+template <int B>
+__device__ __forceinline__ float bank_sigma(const mdx_tables_dev* T, const mdx_material_dev* lmat, const mdx_curve& lmaj,
+                                            WaveBank<B>* W, int s, Ph& p, int slot) {
+  const int ce = slot == 255 ? kBankCache - 1 : (slot < kBankCache - 1 ? slot : -1);  // -1: not cached
+  const uint32_t valid = fl_get(p.flags, 24, 63u);
+  if (ce >= 0 && (valid >> ce & 1u)) return W->sc[ce][s];
+  const float v = curve_eval(T, slot == 255 ? lmaj : lmat[slot].total, p.E);
+  if (ce >= 0) {
+    W->sc[ce][s] = v;
+    p.flags = fl_set(p.flags, 24, 63u, valid | (1u << ce));
+  }
+  return v;
+}
+
+template <bool MULTI, int B>
+__device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __restrict__ T,
+                                                    const mdx_job_dev* __restrict__ jobs,
+                                                    const uint64_t* __restrict__ prefix, int n_jobs, uint64_t total,
+                                                    unsigned long long* __restrict__ work,
+                                                    unsigned long long* __restrict__ counters, int refill_min,
+                                                    int interact_min) {
+  // scalar tallies and spectra go to global atomics with the lanes of a wave
+  // that hit one record summed first (no LDS pre-reduction: the LDS holds
+  // photon banks)
+  __shared__ unsigned long long cnt[MDX_NCOUNTER];
+  __shared__ mdx_material_dev lmat[MDX_MAX_MATERIALS];
+  __shared__ mdx_curve lmaj;
+  __shared__ uint8_t lslot[256];
+  __shared__ JobDesc desc[MULTI ? kWaves : 1];
+  __shared__ WaveBank<B> banks[kWaves];
+  block_copy(lmat, T->mat, T->n_mat * (int)sizeof(mdx_material_dev));
+  block_copy(lslot, T->id_to_slot, 256);
+  block_copy(&lmaj, &T->majorant, (int)sizeof(mdx_curve));
+  if (!MULTI) {
+    block_copy(&desc[0].D, jobs[0].D, (int)sizeof(mdx_domain_dev));
+    block_copy(&desc[0].S, jobs[0].S, (int)sizeof(mdx_source_dev));
+    block_copy(&desc[0].TL, jobs[0].TL, (int)sizeof(mdx_tallies_dev));
+  }
+  if (threadIdx.x < MDX_NCOUNTER) cnt[threadIdx.x] = 0;
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63;
+  const int wsl = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  WaveBank<B>* W = &banks[wsl];
+  uint32_t c_hist = 0, c_step = 0, c_real = 0, c_ray = 0, c_comp = 0, c_photo = 0, c_trial = 0, c_stest = 0,
+           c_dep = 0, c_surf = 0, c_esc = 0;
+  // every slot starts free; list counts are wave-uniform
+  for (int s = lane; s < B; s += 64) W->list[kEvFree][s] = (uint8_t)s;
+  int n_step = 0, n_fly = 0, n_int = 0, n_free = B;
+  bool exhausted = false;  // the current job has no histories left to claim
+  int jc = 0;
+  bool done_all = false;
+  // the current job's record, wave-uniform
+  uint64_t jseed = 0, jfirst = 0, jn = total;
+  uint32_t jstream = 0;
+  bool jflights = false;
+  auto take_job = [&](int j) {
+    jseed = jobs[j].seed;
+    jfirst = jobs[j].first;
+    jn = MULTI ? jobs[j].n : total;
+    jstream = jobs[j].stream;
+    jflights = jobs[j].flights != 0;
+  };
+  if (MULTI) {
+    const uint64_t nw = (uint64_t)gridDim.x * kWaves;
+    const uint64_t gw = (uint64_t)blockIdx.x * kWaves + (uint64_t)wsl;
+    const uint64_t h = (uint64_t)(((double)gw + 0.5) / (double)nw * (double)total);
+    int lo = 0, hi = n_jobs - 1;  // last job with prefix[j] <= h
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (prefix[mid] <= h) lo = mid; else hi = mid - 1;
+    }
+    jc = lo;
+    load_job(&desc[wsl], jobs[jc]);
+  }
+  take_job(jc);
+  wave_sync_lds();
+  refill_min = refill_min > B ? B : refill_min;
+  for (;;) {
+    const JobDesc* jd;
+    {
+      int o = 0;
+      asm volatile("" : "+s"(o));
+      jd = MULTI ? &desc[wsl] + o : &desc[0] + o;
+    }
+    const mdx_domain_dev* D = &jd->D;
+    const mdx_source_dev* S = &jd->S;
+    const mdx_tallies_dev* TL = &jd->TL;
+    const int alive = n_step + n_fly + n_int;
+    if (MULTI && exhausted && alive == 0) {  // the wave's job is done here: the next one with histories left
+      int jnext = -1;
+      for (int k = 0; k < n_jobs; ++k) {
+        const int j = (jc + 1 + k) % n_jobs;
+        if (__hip_atomic_load(work + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < jobs[j].n) {
+          jnext = j;
+          break;
+        }
+      }
+      if (jnext < 0) {
+        done_all = true;
+      } else {
+        load_job(&desc[wsl], jobs[jnext]);
+        jc = jnext;
+        take_job(jc);
+        exhausted = false;
+      }
+      if (done_all) break;
+      continue;
+    }
+    if (!MULTI && exhausted && alive == 0) break;
+    // the event to run: new photons once refill_min slots are free, else the
+    // fullest of the step and flight lists, the interactions once
+    // interact_min are parked or they are the most numerous
+    int ev;
+    if (!exhausted && n_free >= refill_min) ev = kEvFree;
+    else if (n_int > 0 && (n_int >= interact_min || (n_int >= n_step && n_int >= n_fly))) ev = kEvInt;
+    else if (n_step >= n_fly && n_step > 0) ev = kEvStep;
+    else if (n_fly > 0) ev = kEvFly;
+    else if (n_int > 0) ev = kEvInt;
+    else ev = kEvFree;  // nothing alive, slots free, histories left
+    int avail = ev == kEvStep ? n_step : ev == kEvFly ? n_fly : ev == kEvInt ? n_int : n_free;
+    const int k = avail < 64 ? avail : 64;
+    avail -= k;
+    if (ev == kEvStep) n_step = avail;
+    else if (ev == kEvFly) n_fly = avail;
+    else if (ev == kEvInt) n_int = avail;
+    else n_free = avail;
+    const bool act = lane < k;
+    const int slot = act ? (int)W->list[ev][avail + lane] : 0;
+    int next = kEvFree;
+    if (ev == kEvFree) {
+      // ---- new source photons for the k free slots
+      unsigned long long base = 0;
+      if (lane == 0) base = atomicAdd(MULTI ? work + jc : work, (unsigned long long)k);
+      base = __shfl(base, 0);
+      if (base + (uint64_t)k >= jn) exhausted = true;
+      if (act && base + (uint64_t)lane < jn) {
+        const uint64_t h = jfirst + base + (uint64_t)lane;
+        Philox rng;
+        rng.init(jseed, jstream, h);
+        Ph p;
+        generate(S, rng, p.px, p.py, p.pz, p.dx, p.dy, p.dz, p.E);
+        ++c_hist;
+        p.hlo = (uint32_t)h;
+        p.hhi = (uint32_t)(h >> 32);
+        p.c0 = rng.c0;
+        p.nstep = 0;
+        p.flags = fl_set(fl_set(0u, 4, 255u, 255u), 12, 255u, 255u);
+        p.smax = curve_eval(T, lmaj, p.E);
+        W->sc[kBankCache - 1][slot] = p.smax;
+        p.flags = fl_set(p.flags, 24, 63u, 1u << (kBankCache - 1));
+        p.dsafe = INFINITY;
+        p.stot = 0.0f;
+        p.qs = 0.0f;
+        p.qlin = 0;
+        p.tfl = next_tally_event(TL, p.px, p.py, p.pz, p.dx, p.dy, p.dz);
+        uint64_t lin0;
+        const bool fly = jflights && in_domain(D, p.px, p.py, p.pz) && voxel_lookup(D, p.px, p.py, p.pz, lin0) < 0;
+        next = fly ? kEvFly : kEvStep;
+        bank_store(W, slot, p);
+      }
+    } else if (ev == kEvStep || ev == kEvFly) {
+      // ---- one Woodcock step or one background free flight (physics_engine.cpp:38-66)
+      if (act) {
+        Ph p = bank_load(W, slot);
+        ++c_step;
+        float u1, u2;
+        step_uniforms(step_key(jseed, jstream, p.hhi), p.hlo, p.nstep, u1, u2);
+        ++p.nstep;
+        const float x0 = p.px, y0 = p.py, z0 = p.pz;
+        const int nc = (int)fl_get(p.flags, 0, 3u), ni = (int)fl_get(p.flags, 2, 3u);
+        int cslot = (int)fl_get(p.flags, 12, 255u);
+        bool escaped = false, replay = false, alive_ = true;
+        float seg = 0.0f;
+        int g = -1;
+        uint64_t lin = 0;
+        uint32_t mword = 255u;
+        if (ev == kEvStep) {
+          float s = -__log2f(u1) * (0.69314718055994531f * frcp(p.smax));
+          bool limited = false;
+          if (s > p.dsafe) {
+            s = p.dsafe;
+            limited = true;
+          }
+          p.px = x0 + s * p.dx;
+          p.py = y0 + s * p.dy;
+          p.pz = z0 + s * p.dz;
+          seg = s;
+          escaped = !in_domain(D, p.px, p.py, p.pz);
+          if (escaped) {
+            ++c_esc;
+            replay = true;
+            alive_ = false;
+          } else {
+            uint32_t cell = 0;
+            g = voxel_lookup(D, p.px, p.py, p.pz, lin, &cell);
+            const uint16_t* mc = D->grids[0].macro;
+            if (g == 0 && mc) mword = gld(mc + cell);
+            const uint32_t c8 = mword & 255u;
+            int sl;
+            if (g < 0) sl = D->bg_slot;
+            else if (c8 != 255u && (c8 & MDX_MACRO_UNIFORM)) sl = (int)(c8 & 0x7Fu);
+            else sl = (int)lslot[gld(D->grids[g].ids + lin)];
+            if (sl != cslot) {
+              p.stot = bank_sigma(T, lmat, lmaj, W, slot, p, sl);
+              cslot = sl;
+            }
+            if (!limited && real_collision(u2, delta_threshold(p.stot, p.smax))) {
+              next = kEvInt;
+              p.qs = s;
+              p.qlin = (uint32_t)lin;
+              p.flags = fl_set(p.flags, 20, 15u, (uint32_t)(g + 1));
+            } else {
+              p.tfl -= s;
+              replay = p.tfl <= 0.0f;
+              if (p.nstep >= kMaxSteps) alive_ = false;
+              next = (jflights && g < 0) ? kEvFly : kEvStep;
+            }
+            // the majorant for the step from the new position
+            int ncls = (mword & 255u) == 255u ? 255 : (int)(mword & 0x7Fu);
+            const uint32_t nr = mword >> 8;
+            if (nr == 0u || p.E < 1.0e3f) ncls = 255;
+            if (ncls != (int)fl_get(p.flags, 4, 255u)) {
+              p.flags = fl_set(p.flags, 4, 255u, (uint32_t)ncls);
+              p.smax = bank_sigma(T, lmat, lmaj, W, slot, p, ncls);
+            }
+            p.dsafe = ncls == 255 ? INFINITY : (float)nr * D->grids[0].mcell;
+          }
+        } else {
+          // background free flight at the background's own cross section,
+          // cut at the next grid face or the domain's boundary
+          const int sl = D->bg_slot;
+          if (sl != cslot) {
+            p.stot = bank_sigma(T, lmat, lmaj, W, slot, p, sl);
+            cslot = sl;
+          }
+          const float si = -__log2f(u1) * (0.69314718055994531f * frcp(p.stot));
+          float dgrid, dexit;
+          flight_bounds(D, x0, y0, z0, p.dx, p.dy, p.dz, dgrid, dexit);
+          if (si < dgrid && si < dexit) {  // a real interaction in the background
+            p.px = x0 + si * p.dx;
+            p.py = y0 + si * p.dy;
+            p.pz = z0 + si * p.dz;
+            next = kEvInt;
+            p.qs = si;
+            p.qlin = 0;
+            p.flags = fl_set(p.flags, 20, 15u, 0u);
+          } else if (dgrid <= dexit) {  // up to the grid's face; majorant steps from there
+            p.px = x0 + dgrid * p.dx;
+            p.py = y0 + dgrid * p.dy;
+            p.pz = z0 + dgrid * p.dz;
+            p.tfl -= dgrid;
+            replay = p.tfl <= 0.0f;
+            seg = dgrid;
+            next = kEvStep;
+            if (p.nstep >= kMaxSteps) alive_ = false;
+          } else {  // out of the domain, overshooting by a majorant step
+            seg = dexit - __log2f(u2) * (0.69314718055994531f * frcp(p.smax));
+            p.px = x0 + seg * p.dx;
+            p.py = y0 + seg * p.dy;
+            p.pz = z0 + seg * p.dz;
+            ++c_esc;
+            replay = true;
+            alive_ = false;
+          }
+        }
+        p.flags = fl_set(p.flags, 12, 255u, (uint32_t)cslot);
+        if (replay) {
+          Step r{x0, y0, z0, p.dx, p.dy, p.dz, seg, p.E, nc, ni, 0, 0, 0.0f, false};
+          tallies_step<true>(TL, nullptr, r, c_surf, nullptr, nullptr);
+          if (alive_) p.tfl = next_tally_event(TL, p.px, p.py, p.pz, p.dx, p.dy, p.dz);
+        }
+        if (!alive_) next = kEvFree;
+        else bank_store(W, slot, p);
+      }
+    } else {
+      // ---- the parked real interactions (physics_engine.cpp:62-78)
+      if (act) {
+        Ph p = bank_load(W, slot);
+        ++c_real;
+        const int cslot = (int)fl_get(p.flags, 12, 255u);
+        const mdx_material_dev& M = lmat[cslot];
+        int nc = (int)fl_get(p.flags, 0, 3u), ni = (int)fl_get(p.flags, 2, 3u);
+        Step r{p.px - p.qs * p.dx, p.py - p.qs * p.dy, p.pz - p.qs * p.dz, p.dx, p.dy, p.dz, p.qs, p.E,
+               nc, ni, 0, 0, 0.0f, true};
+        Philox rng;
+        rng.init(jseed, jstream, (uint64_t)p.hlo | ((uint64_t)p.hhi << 32));
+        rng.c0 = p.c0;
+        float sc, si;
+        curve_eval2(T, M.coh, p.E, M.incoh, p.E, sc, si);
+        const int type = select_type(sc, si, p.stot, rng.uniform());
+        float dep = 0.0f;
+        bool alive_ = true;
+        if (type == 1) {
+          ++c_ray;
+          nc = nc < 3 ? nc + 1 : 3;
+          const float mu = rita_sample(T, M, p.E, rng.uniform());
+          rotate(p.dx, p.dy, p.dz, mu, 2.0f * kPiF * rng.uniform());
+        } else if (type == 2) {
+          ++c_comp;
+          ni = ni < 3 ? ni + 1 : 3;
+          float mu;
+          const float Ep = compton(T, M, rng, p.E, mu, c_trial, c_stest);
+          rotate(p.dx, p.dy, p.dz, mu, 2.0f * kPiF * rng.uniform());
+          dep = p.E - Ep;
+          p.E = Ep;
+          int mcls = (int)fl_get(p.flags, 4, 255u);
+          if (p.E < 1.0e3f) {
+            mcls = 255;
+            p.dsafe = INFINITY;
+          }
+          p.flags = fl_set(fl_set(p.flags, 4, 255u, (uint32_t)mcls), 24, 63u, 0u);  // a new energy: cache invalid
+          p.smax = bank_sigma(T, lmat, lmaj, W, slot, p, mcls);
+          p.flags = fl_set(p.flags, 12, 255u, 255u);
+        } else {
+          ++c_photo;
+          dep = p.E;
+          alive_ = false;
+        }
+        p.c0 = rng.c0;
+        p.flags = fl_set(fl_set(p.flags, 0, 3u, (uint32_t)nc), 2, 3u, (uint32_t)ni);
+        r.ncf = nc;
+        r.nif = ni;
+        r.dep = dep;
+        const int qg = (int)fl_get(p.flags, 20, 15u) - 1;
+        if (dep != 0.0f) {
+          ++c_dep;
+          if (qg >= 0) {
+            voxel_deposit(D->grids[qg], p.qlin, dep);
+          } else if (D->bg_dose) {
+            gadd(D->bg_dose, (double)dep);
+            gadd(D->bg_dose + 1, (double)dep * (double)dep);
+            gadd(D->bg_dose + 2, 1.0);
+          }
+        }
+        tallies_step<true>(TL, nullptr, r, c_surf, nullptr, nullptr);
+        if (p.nstep >= kMaxSteps) alive_ = false;
+        if (alive_) {
+          p.tfl = next_tally_event(TL, p.px, p.py, p.pz, p.dx, p.dy, p.dz);
+          next = (jflights && qg < 0) ? kEvFly : kEvStep;
+          bank_store(W, slot, p);
+        }
+      }
+    }
+    // ---- each processed slot to the list of its photon's next event
+    wave_sync_lds();
+    {
+      const unsigned long long below = (1ull << lane) - 1ull;
+      const unsigned long long m0 = __ballot(act && next == kEvStep), m1 = __ballot(act && next == kEvFly),
+                               m2 = __ballot(act && next == kEvInt), m3 = __ballot(act && next == kEvFree);
+      const unsigned long long mine = next == kEvStep ? m0 : next == kEvFly ? m1 : next == kEvInt ? m2 : m3;
+      const int basep = next == kEvStep ? n_step : next == kEvFly ? n_fly : next == kEvInt ? n_int : n_free;
+      if (act) W->list[next][basep + __popcll(mine & below)] = (uint8_t)slot;
+      n_step += __popcll(m0);
+      n_fly += __popcll(m1);
+      n_int += __popcll(m2);
+      n_free += __popcll(m3);
+    }
+    wave_sync_lds();
+  }
+  atomicAdd(&cnt[MDX_C_HIST], (unsigned long long)c_hist);
+  atomicAdd(&cnt[MDX_C_STEP], (unsigned long long)c_step);
+  atomicAdd(&cnt[MDX_C_REAL], (unsigned long long)c_real);
+  atomicAdd(&cnt[MDX_C_RAYLEIGH], (unsigned long long)c_ray);
+  atomicAdd(&cnt[MDX_C_COMPTON], (unsigned long long)c_comp);
+  atomicAdd(&cnt[MDX_C_PHOTO], (unsigned long long)c_photo);
+  atomicAdd(&cnt[MDX_C_KAHN_TRIAL], (unsigned long long)c_trial);
+  atomicAdd(&cnt[MDX_C_KAHN_S], (unsigned long long)c_stest);
+  atomicAdd(&cnt[MDX_C_DEPOSIT], (unsigned long long)c_dep);
+  atomicAdd(&cnt[MDX_C_SURFHIT], (unsigned long long)c_surf);
+  atomicAdd(&cnt[MDX_C_ESCAPE], (unsigned long long)c_esc);
+  __syncthreads();
+  if (threadIdx.x < MDX_NCOUNTER && cnt[threadIdx.x]) atomicAdd(counters + threadIdx.x, cnt[threadIdx.x]);
+}
+
+template <bool MULTI, int B>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
+k_transport_bank(const mdx_tables_dev* __restrict__ T, const mdx_job_dev* __restrict__ jobs,
+                 const uint64_t* __restrict__ prefix, int n_jobs, uint64_t total, unsigned long long* __restrict__ work,
+                 unsigned long long* __restrict__ counters, int refill_min, int interact_min) {
+  transport_bank_body<MULTI, B>(T, jobs, prefix, n_jobs, total, work, counters, refill_min, interact_min);
+}
+
+// Graph form (round buffer as for k_transport_queue_round).
+template <int B>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
+k_transport_bank_round(const mdx_tables_dev* __restrict__ T, const char* __restrict__ round,
+                       unsigned long long* __restrict__ work, unsigned long long* __restrict__ counters,
+                       int refill_min, int interact_min) {
+  const uint64_t* hdr = reinterpret_cast<const uint64_t*>(round);
+  const int n_jobs = (int)hdr[0];
+  const uint64_t total = hdr[1];
+  const uint64_t* prefix = hdr + 2;
+  const mdx_job_dev* jobs = reinterpret_cast<const mdx_job_dev*>(round + 2 * sizeof(uint64_t) + 1025 * sizeof(uint64_t));
+  if (n_jobs <= 0 || total == 0) return;
+  transport_bank_body<true, B>(T, jobs, prefix, n_jobs, total, work, counters, refill_min, interact_min);
+}
+
 // ------------------------------------------------------------------ RITA table build (FP64)
 // The adaptive PENELOPE RITA tables of the Rayleigh angular distribution
 // (probability_dist.cpp:62-71,121-175,267-287), one wave per energy of the
@@ -1542,6 +2036,45 @@ hipError_t transport_queue(const mdx_tables_dev* T, const mdx_job_dev* jobs, con
     hipLaunchKernelGGL(k_transport_queue<true>, dim3(blocks), dim3(kBlock), 0, st, T, jobs, prefix, n_jobs, total,
                        work, counters, refill_min, interact_min, 0u);
   }
+  return hipGetLastError();
+}
+
+template <bool MULTI>
+static hipError_t launch_bank(int bank, unsigned blocks, const mdx_tables_dev* T, const mdx_job_dev* jobs,
+                              const uint64_t* prefix, int n_jobs, uint64_t total, unsigned long long* work,
+                              unsigned long long* counters, int refill_min, int interact_min, hipStream_t st) {
+  if (bank == 96)
+    hipLaunchKernelGGL((k_transport_bank<MULTI, 96>), dim3(blocks), dim3(kBlock), 0, st, T, jobs, prefix, n_jobs,
+                       total, work, counters, refill_min, interact_min);
+  else
+    hipLaunchKernelGGL((k_transport_bank<MULTI, 128>), dim3(blocks), dim3(kBlock), 0, st, T, jobs, prefix, n_jobs,
+                       total, work, counters, refill_min, interact_min);
+  return hipGetLastError();
+}
+
+hipError_t transport_bank(const mdx_tables_dev* T, const mdx_job_dev* jobs, const uint64_t* prefix, int n_jobs,
+                          uint64_t total, unsigned long long* work, unsigned long long* counters, int grid_blocks,
+                          int refill_min, int interact_min, int bank, hipStream_t st) {
+  unsigned blocks = (unsigned)grid_blocks;
+  const uint64_t need = (total + kBlock - 1) / kBlock;
+  if (need < blocks) blocks = (unsigned)(need ? need : 1);
+  refill_min = refill_min < 1 ? 1 : (refill_min > 64 ? 64 : refill_min);
+  interact_min = interact_min < 1 ? 1 : (interact_min > 64 ? 64 : interact_min);
+  if (n_jobs == 1)
+    return launch_bank<false>(bank, blocks, T, jobs, prefix, n_jobs, total, work, counters, refill_min, interact_min,
+                              st);
+  return launch_bank<true>(bank, blocks, T, jobs, prefix, n_jobs, total, work, counters, refill_min, interact_min, st);
+}
+
+hipError_t transport_bank_round(const mdx_tables_dev* T, const char* round, unsigned long long* work,
+                                unsigned long long* counters, int grid_blocks, int refill_min, int interact_min,
+                                int bank, hipStream_t st) {
+  if (bank == 96)
+    hipLaunchKernelGGL(k_transport_bank_round<96>, dim3((unsigned)grid_blocks), dim3(kBlock), 0, st, T, round, work,
+                       counters, refill_min, interact_min);
+  else
+    hipLaunchKernelGGL(k_transport_bank_round<128>, dim3((unsigned)grid_blocks), dim3(kBlock), 0, st, T, round, work,
+                       counters, refill_min, interact_min);
   return hipGetLastError();
 }
 

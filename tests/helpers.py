@@ -3,9 +3,16 @@ of the macro-cell table shared by the host and device tests."""
 import math
 
 import numpy as np
+import pytest
 from scipy import stats
 
 import midsx_amd as M
+
+# the event kernels MDX_KERNEL_QUEUE runs: (kind, bank size) for
+# Context.set_event_kernel
+EVENT_FORMS = [pytest.param((M.EVENT_QUEUE, 128), id="queue"),
+               pytest.param((M.EVENT_BANK, 96), id="bank96"),
+               pytest.param((M.EVENT_BANK, 128), id="bank128")]
 
 
 def batch_mean_sigma(samples):

@@ -17,7 +17,8 @@ bin the batch means are compared by a Welch t mapped to a normal score.
   0.01; the same 3 sigma screen applies to each bin.
 
 Scenes: the slab of the plain method (40x40x10 water voxels, 60 keV pencil
-beam), with the plain and the queue kernel; the radiography workload at
+beam), with the plain kernel and every event kernel of the queue launch;
+the other scenes on every event kernel; the radiography workload at
 32x32x24 voxels (cone beam, Kramers 15-80 keV, water / bone / adipose); one
 CT view of a 32x32x16 phantom; a served dose job (16^3 phantom, 80 kVp)
 through the request loop.
@@ -29,7 +30,7 @@ import midsx_amd as M
 import oracle
 from midsx_amd import scenes
 from midsx_amd.serve import Server
-from helpers import binwise, hotelling
+from helpers import EVENT_FORMS, binwise, hotelling
 
 pytestmark = pytest.mark.gpu
 
@@ -90,12 +91,15 @@ def slab_oracle():
     return orc_batches(sim, 40, 250000, 301, slab_read)
 
 
-@pytest.mark.parametrize("kernel", [M.KERNEL_PLAIN, M.KERNEL_QUEUE])
-def test_slab_every_bin(slab_oracle, kernel):
+@pytest.mark.parametrize("kernel, events", [pytest.param(M.KERNEL_PLAIN, None, id="plain")] +
+                         [pytest.param(M.KERNEL_QUEUE, e.values[0], id=e.id) for e in EVENT_FORMS])
+def test_slab_every_bin(slab_oracle, kernel, events):
     ctx = M.Context(0)
+    if events:
+        ctx.set_event_kernel(*events)
     ctx.build_tables(list(scenes.C1_MATERIALS))
     scenes.setup_slab(ctx)
-    g = gpu_batches(ctx, 40, 250000, 71 + kernel, slab_read, kernel)
+    g = gpu_batches(ctx, 40, 250000, 71 + kernel + 10 * (events[0] if events else 0), slab_read, kernel)
     o = slab_oracle
     r = assert_binwise("slab voxel dose", g, o, "dose", "dose_n")
     assert r["standalone"] > 500  # the pencil beam's dose is on and near its axis
@@ -123,14 +127,21 @@ def rad_read(src):
     return out
 
 
-def test_radiography_every_bin():
+@pytest.fixture(scope="module")
+def rad_oracle():
+    sim = oracle.Simulation(scenes.C2_MATERIALS)
+    scenes.setup_radiography(sim, **RAD)
+    return orc_batches(sim, 20, 100000, 401, rad_read)
+
+
+@pytest.mark.parametrize("events", EVENT_FORMS)
+def test_radiography_every_bin(rad_oracle, events):
     ctx = M.Context(0)
+    ctx.set_event_kernel(*events)
     ctx.build_tables(list(scenes.C2_MATERIALS))
     scenes.setup_radiography(ctx, **RAD)
     g = gpu_batches(ctx, 20, 100000, 81, rad_read)
-    sim = oracle.Simulation(scenes.C2_MATERIALS)
-    scenes.setup_radiography(sim, **RAD)
-    o = orc_batches(sim, 20, 100000, 401, rad_read)
+    o = rad_oracle
     r = assert_binwise("radiography voxel dose", g, o, "dose", "dose_n")
     assert r["standalone"] > 3000
     assert_binwise("radiography image energy", g, o, "img_e", "img_n")
@@ -142,23 +153,29 @@ def test_radiography_every_bin():
 CT_SHAPE, CT_VOX = (32, 32, 16), 0.5
 
 
-def test_ct_view_every_bin():
+def ct_view_setup(sim):
     geo = scenes.ct_geometry(CT_SHAPE, CT_VOX)
-    ids = scenes.ct_phantom(CT_SHAPE, CT_VOX, 4)
+    sim.set_domain(geo["dim"], scenes.AIR)
+    sim.add_grid(scenes.ct_phantom(CT_SHAPE, CT_VOX, 4), (CT_VOX,) * 3, geo["origin"])
+    sim.enable_voxel_dose(True)
+    scenes.setup_ct_view(sim, geo, 30.0, pixels=(32, 8))
 
-    def setup(sim):
-        sim.set_domain(geo["dim"], scenes.AIR)
-        sim.add_grid(ids, (CT_VOX,) * 3, geo["origin"])
-        sim.enable_voxel_dose(True)
-        scenes.setup_ct_view(sim, geo, 30.0, pixels=(32, 8))
 
-    ctx = M.Context(0)
-    ctx.build_tables(list(scenes.CT_MATERIALS))
-    setup(ctx)
-    g = gpu_batches(ctx, 20, 60000, 91, rad_read)
+@pytest.fixture(scope="module")
+def ct_oracle():
     sim = oracle.Simulation(scenes.CT_MATERIALS)
-    setup(sim)
-    o = orc_batches(sim, 20, 60000, 501, rad_read)
+    ct_view_setup(sim)
+    return orc_batches(sim, 20, 60000, 501, rad_read)
+
+
+@pytest.mark.parametrize("events", EVENT_FORMS)
+def test_ct_view_every_bin(ct_oracle, events):
+    ctx = M.Context(0)
+    ctx.set_event_kernel(*events)
+    ctx.build_tables(list(scenes.CT_MATERIALS))
+    ct_view_setup(ctx)
+    g = gpu_batches(ctx, 20, 60000, 91, rad_read)
+    o = ct_oracle
     r = assert_binwise("CT voxel dose", g, o, "dose", "dose_n")
     assert r["standalone"] > 200
     assert_binwise("CT image energy", g, o, "img_e", "img_n")
@@ -167,7 +184,8 @@ def test_ct_view_every_bin():
 
 
 # --------------------------------------------------------------------- served dose job (C5, reduced)
-def test_served_dose_job_every_bin():
+@pytest.mark.parametrize("events", EVENT_FORMS)
+def test_served_dose_job_every_bin(events):
     """24 copies of one dose job (different seeds) served through the
     request loop (FP64 dose read) against 24 oracle batches of the same job;
     the pooling uses the events of a separate GPU pilot run of the job."""
@@ -175,6 +193,7 @@ def test_served_dose_job_every_bin():
                kvp=80.0)
     k = 24
     ctx = M.Context(0)
+    ctx.set_event_kernel(*events)
     ctx.build_tables(list(scenes.DOSE_MATERIALS))
     keep = lambda j, s, s2, image: (s.copy(), image.copy())
     srv = Server(ctx, round_histories=400_000, min_slice=20_000, on_result=keep, dose="f64")

@@ -14,7 +14,7 @@ import pytest
 import midsx_amd as M
 import oracle
 from midsx_amd import scenes
-from helpers import batch_mean_sigma, hotelling, welch_normal_scores
+from helpers import EVENT_FORMS, batch_mean_sigma, hotelling, welch_normal_scores
 
 pytestmark = pytest.mark.gpu
 
@@ -30,8 +30,10 @@ def ct():
     ctx.close()
 
 
-def test_views_same_alone_and_in_one_launch(ct):
+@pytest.mark.parametrize("events", EVENT_FORMS)
+def test_views_same_alone_and_in_one_launch(ct, events):
     ctx, geo, views, angles = ct
+    ctx.set_event_kernel(*events)
     n = [20000 + 1000 * i for i in range(len(views))]
     seeds = [100 + i for i in range(len(views))]
     ctx.set_graphs(True)
@@ -62,10 +64,13 @@ def test_views_same_alone_and_in_one_launch(ct):
         dose_sum += ctx.voxel_dose(0)[0]
     assert np.allclose(dose_all, dose_sum, rtol=1e-9, atol=1e-6)
     ctx.set_graphs(False)
+    ctx.set_event_kernel(M.EVENT_QUEUE)
 
 
-def test_view_matches_oracle(ct):
+@pytest.mark.parametrize("events", EVENT_FORMS)
+def test_view_matches_oracle(ct, events):
     ctx, geo, views, angles = ct
+    ctx.set_event_kernel(*events)
     v = views[1]  # 30 degrees
     k, n = 24, 6000
     ids = scenes.ct_phantom(SHAPE, VOX, 4)
@@ -104,3 +109,4 @@ def test_view_matches_oracle(ct):
     assert np.all(np.abs(z) < 4.0), np.max(np.abs(z))
     p, t2, rank = hotelling(g[:, keep], o[:, keep])
     assert p > 0.01, (p, t2, rank)
+    ctx.set_event_kernel(M.EVENT_QUEUE)

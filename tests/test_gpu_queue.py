@@ -1,4 +1,5 @@
-"""The persistent queue kernel (per-lane refill) and multi-job launches.
+"""The persistent event kernels (lane-resident queue, per-wave photon banks)
+and multi-job launches, each test on every event kernel.
 
 * parity of the queue kernel against the oracle on the slab workload (same
   Hotelling T^2 test as the plain kernel);
@@ -12,14 +13,16 @@ import pytest
 import midsx_amd as M
 import oracle
 from midsx_amd import scenes
-from helpers import batch_mean_sigma, check_macro_table, hotelling, welch_normal_scores
+from helpers import EVENT_FORMS, batch_mean_sigma, check_macro_table, hotelling, welch_normal_scores
 from test_gpu_parity import orc_batches, slab_extract
 
 pytestmark = pytest.mark.gpu
 
 
-def test_queue_kernel_slab_matches_oracle():
+@pytest.mark.parametrize("events", EVENT_FORMS)
+def test_queue_kernel_slab_matches_oracle(events):
     ctx = M.Context(0)
+    ctx.set_event_kernel(*events)
     ctx.build_tables(list(scenes.C1_MATERIALS))
     scenes.setup_slab(ctx)
     k, n = 48, 50000
@@ -76,8 +79,10 @@ def snapshot(ctx, job):
     return np.concatenate(out)
 
 
-def test_job_same_alone_and_in_mix():
+@pytest.mark.parametrize("events", EVENT_FORMS)
+def test_job_same_alone_and_in_mix(events):
     ctx = M.Context(0)
+    ctx.set_event_kernel(*events)
     ctx.build_tables(list(set(scenes.C1_MATERIALS) | set(scenes.C2_MATERIALS)))
     jobs = make_jobs(ctx)
     n = [300000, 200000, 100000]
@@ -138,12 +143,14 @@ def play_rounds(ctx, jobs, rounds):
     return [snapshot(ctx, j) for j in jobs], counts
 
 
-def test_graph_rounds_match_eager():
+@pytest.mark.parametrize("events", EVENT_FORMS)
+def test_graph_rounds_match_eager(events):
     """Rounds replayed from one captured hipGraph (round-buffer copy, counter
     zeroing kernels, queue kernel) give the eager launches' results; eager
     work and synchronisations run between replays, and a change of queue
     parameters forces a new capture."""
     ctx = M.Context(0)
+    ctx.set_event_kernel(*events)
     ctx.build_tables(list(set(scenes.C1_MATERIALS) | set(scenes.C2_MATERIALS)))
     jobs = make_jobs(ctx)
     rounds = [(jobs, [100000, 60000, 30000], [0, 0, 0], [5, 6, 7]),
@@ -170,11 +177,13 @@ def test_graph_rounds_match_eager():
     ctx.close()
 
 
-def test_alias_domain_matches_own_copy():
+@pytest.mark.parametrize("events", EVENT_FORMS)
+def test_alias_domain_matches_own_copy(events):
     """A job that aliases another job's phantom tallies what a job with its
     own copy of the phantom tallies; the shared per-voxel dose holds both
     jobs' deposits."""
     ctx = M.Context(0)
+    ctx.set_event_kernel(*events)
     ctx.build_tables(list(scenes.C2_MATERIALS))
     kw = dict(shape=(32, 32, 24), voxel_cm=0.5, seed=7, image=(16, 16))
     scenes.setup_radiography(ctx, **kw)  # job 0 owns the phantom
@@ -206,11 +215,14 @@ def test_alias_domain_matches_own_copy():
     ctx.close()
 
 
-def test_autotune_on_aliased_job_clears_shared_dose():
+@pytest.mark.parametrize("rounds", [False, True])
+def test_autotune_on_aliased_job_clears_shared_dose(rounds):
     """The autotuner's probes on a job that uses another job's phantom
     deposit in that phantom's voxels and background voxel: after it, the
     owner's voxel dose (sums and counts) and background dose read zero, as
-    do the tuned job's own tallies."""
+    do the tuned job's own tallies; in both probe forms (the job alone, or
+    two slices of it in one multi-job launch), and the event kernel it keeps
+    is one of those it tried."""
     ctx = M.Context(0)
     ctx.build_tables(list(scenes.C2_MATERIALS))
     kw = dict(shape=(32, 32, 24), voxel_cm=0.5, seed=7, image=(16, 16))
@@ -222,7 +234,11 @@ def test_autotune_on_aliased_job_clears_shared_dose():
     ctx.job_select(0)
     assert ctx.voxel_dose(0)[0].sum() > 0
     ctx.job_select(view)
-    ctx.autotune(probe_histories=20000)
+    tuned = ctx.autotune(probe_histories=20000, rounds=rounds)
+    assert tuned["event_kernel"] in ("queue", "bank") and tuned["bank_size"] in (96, 128)
+    assert ctx.event_kernel() == (M.EVENT_BANK if tuned["event_kernel"] == "bank" else M.EVENT_QUEUE,
+                                  tuned["bank_size"])
+    assert ctx.queue_params() == (tuned["blocks_per_cu"], tuned["refill_min"], tuned["interact_min"])
     assert not ctx.tally(0)["image"].any()
     s, s2, n = ctx.voxel_dose(0)
     assert not s.any() and not s2.any() and not n.any()
@@ -233,13 +249,15 @@ def test_autotune_on_aliased_job_clears_shared_dose():
     ctx.close()
 
 
-def test_free_flights_match_majorant_steps():
+@pytest.mark.parametrize("events", EVENT_FORMS)
+def test_free_flights_match_majorant_steps(events):
     """Background free flights change only the number of rejected steps: the
     image, dose and interaction counts of a radiography job agree with the
     same job run with majorant steps everywhere (Welch scores and Hotelling
     T^2 over 40 batches each), in far fewer steps.  A job with a volume
     tally never takes them."""
     ctx = M.Context(0)
+    ctx.set_event_kernel(*events)
     ctx.build_tables(list(scenes.C2_MATERIALS))
     scenes.setup_radiography(ctx, shape=(48, 48, 40), voxel_cm=0.5, seed=4, image=(16, 16))
     k, n = 40, 40000
@@ -279,13 +297,15 @@ def test_free_flights_match_majorant_steps():
     ctx.close()
 
 
-def test_setup_during_round_clears_reused_blocks():
+@pytest.mark.parametrize("events", EVENT_FORMS)
+def test_setup_during_round_clears_reused_blocks(events):
     """A job set up while a graph round is in flight has its dose and tally
     buffers cleared by the next round's graph (the set-up calls do not wait
     for the round).  Its buffers here come back from the pool holding a
     released job's sums, so a lost clear would show in its results, which
     must equal those of the same job set up with the device idle."""
     ctx = M.Context(0)
+    ctx.set_event_kernel(*events)
     ctx.build_tables(list(scenes.DOSE_MATERIALS))
     ctx.set_graphs(True)
     small = dict(shape=(32, 32, 32), voxel_cm=0.2, phantom_seed=2, image=(16, 16), energy=50e3)
@@ -322,7 +342,8 @@ def test_setup_during_round_clears_reused_blocks():
     ctx.close()
 
 
-def test_local_majorants_match_global_majorant():
+@pytest.mark.parametrize("events", EVENT_FORMS)
+def test_local_majorants_match_global_majorant(events):
     """Local majorants (macro cells with a bounding material and a safe
     radius) change only the number of rejected steps: a radiography job's
     image, dose and interaction counts agree with the same job run with the
@@ -330,6 +351,7 @@ def test_local_majorants_match_global_majorant():
     family-wise level of ~0.5 %, and Hotelling T^2 over 40 batches), in fewer
     steps; the macro-cell table holds what it claims."""
     ctx = M.Context(0)
+    ctx.set_event_kernel(*events)
     materials = list(scenes.C2_MATERIALS)
     ctx.build_tables(materials)
     shape = (64, 64, 48)

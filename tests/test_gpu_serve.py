@@ -16,7 +16,7 @@ import midsx_amd as M
 import oracle
 from midsx_amd import scenes
 from midsx_amd.serve import Server, poisson_arrivals, run_plain
-from helpers import batch_mean_sigma, hotelling, welch_normal_scores
+from helpers import EVENT_FORMS, batch_mean_sigma, hotelling, welch_normal_scores
 
 pytestmark = pytest.mark.gpu
 
@@ -33,7 +33,9 @@ def ctx():
     c.close()
 
 
-def test_served_same_alone_and_in_mix(ctx):
+@pytest.mark.parametrize("events", EVENT_FORMS)
+def test_served_same_alone_and_in_mix(ctx, events):
+    ctx.set_event_kernel(*events)
     jobs = scenes.draw_dose_jobs(10, 5, shapes=(16, 24, 32), h_lo=5e4, h_hi=4e5)
     budget = 3 * max(scenes.dose_job_bytes(j) for j in jobs)
     srv = Server(ctx, budget_bytes=budget, round_histories=300_000, min_slice=20_000, on_result=keep_arrays,
@@ -47,6 +49,7 @@ def test_served_same_alone_and_in_mix(ctx):
             assert np.count_nonzero(a) > 10
             assert np.allclose(m, a, rtol=1e-12, atol=1e-12 * np.abs(a).max()), (i, np.max(np.abs(m - a)))
     ctx.job_select(0)
+    ctx.set_event_kernel(M.EVENT_QUEUE)
 
 
 def test_plain_fifo_serves_same_jobs(ctx):
@@ -63,7 +66,9 @@ def block_sums(s, shape, b=2):
     return d.reshape(b, nz // b, b, ny // b, b, nx // b).sum(axis=(1, 3, 5)).ravel()
 
 
-def test_served_dose_matches_oracle(ctx):
+@pytest.mark.parametrize("events", EVENT_FORMS)
+def test_served_dose_matches_oracle(ctx, events):
+    ctx.set_event_kernel(*events)
     job = dict(shape=(16, 16, 16), voxel_cm=0.25, phantom_seed=11, histories=20000, seed=1, image=(8, 8),
                kvp=80.0)
     k = 24
@@ -92,6 +97,7 @@ def test_served_dose_matches_oracle(ctx):
     assert np.all(np.abs(z) < 4.0), np.max(np.abs(z))
     p, t2, rank = hotelling(g[:, keep], o[:, keep])
     assert p > 0.01, (p, t2, rank)
+    ctx.set_event_kernel(M.EVENT_QUEUE)
 
 
 def test_dose_f32_read_matches_f64(ctx):

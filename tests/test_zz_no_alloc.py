@@ -10,6 +10,7 @@ import pytest
 
 import midsx_amd as M
 from midsx_amd import scenes
+from helpers import EVENT_FORMS
 
 pytestmark = pytest.mark.gpu
 
@@ -53,8 +54,10 @@ def same(ref, got):
             assert np.allclose(r[k], g[k], rtol=1e-12, atol=1e-9), k
 
 
-def test_noalloc_matches_run_jobs_twice():
+@pytest.mark.parametrize("events", EVENT_FORMS)
+def test_noalloc_matches_run_jobs_twice(events):
     ctx = M.Context(0)
+    ctx.set_event_kernel(*events)
     jobs = make_jobs(ctx)
     calls = [(jobs, [200000, 150000, 100000], [0, 0, 0], [11, 12, 13]),
              (jobs[:2], [50000, 30000], [1000, 7], [21, 22])]  # smaller: fewer jobs, fewer histories
