@@ -1626,16 +1626,17 @@ int mdx_autotune(mdx_ctx* c, uint64_t probe_histories, int flags, int* blocks_pe
       if (t < cur) { cur = t; b = v; }
     }
     // the bank kernel (not for the per-block LDS dose mode, which always runs
-    // on the queue kernel): both bank sizes at two or three blocks per CU
-    // (their LDS fits three blocks of the single-job kernel with 96-photon
-    // banks, two otherwise), then its refill and interaction batches
+    // on the queue kernel): both bank sizes at two to four blocks per CU
+    // (LDS and registers fit four blocks of the single-job kernel with
+    // 96-photon banks, three otherwise), then its refill and interaction
+    // batches
     const Job& O = job_at(c, domain_owner(c, job));
     const bool lds_dose = O.voxel_dose && O.dose_mode == MDX_DOSE_F32C_LDS && !rounds;
     if ((flags & MDX_TUNE_EVENTS) && !lds_dose) {
       int kb = 96, bb = 2, rb = 64, ib = 32;
       double tb = 1e300;
       for (int bs : {96, 128})
-        for (int v : {2, 3}) {
+        for (int v : {2, 3, 4}) {
           double t = probe(MDX_EVENT_BANK, bs, v, rb, ib);
           if (t < tb) { tb = t; kb = bs; bb = v; }
         }

@@ -1153,7 +1153,7 @@ k_transport_queue_round(const mdx_tables_dev* __restrict__ T, const char* __rest
 
 // ------------------------------------------------------------------ bank kernel
 // The event form with photon banks.  Every wave keeps a bank of B photons in
-// LDS (structure of arrays, ~100 B each) and a list of bank slots per next
+// LDS (structure of arrays, 80 B each) and a list of bank slots per next
 // event: a Woodcock step inside a grid, a free flight through the
 // background, a parked real interaction, or a free slot.  Each iteration the
 // wave takes up to 64 photons from one list (the fullest step or flight
@@ -1174,7 +1174,23 @@ k_transport_queue_round(const mdx_tables_dev* __restrict__ T, const char* __rest
 // A wave holds the photons of one job at a time (MULTI: the next job with
 // histories left once its own has run out and its bank is empty).
 enum { kEvStep = 0, kEvFly = 1, kEvInt = 2, kEvFree = 3, kEvTypes = 4 };
-constexpr int kBankCache = 6;  // cached total cross sections: slots 0 .. 4 and the global majorant (entry 5)
+// Cached total cross sections per bank slot: the global majorant only.
+// Caching material cross sections as well costs more LDS (i.e. waves per
+// CU) than the table lookups it saves.
+constexpr int kBankCache = 1;
+
+// A wave's job in the bank kernel: domain and source in LDS (read by every
+// step); the tally descriptors are read from global memory with wave-uniform
+// addresses (scalar loads), which keeps three blocks' banks within a CU's LDS.
+struct BankJob {
+  mdx_domain_dev D;
+  mdx_source_dev S;
+};
+
+__device__ __forceinline__ void load_bank_job(BankJob* dst, const mdx_job_dev& j) {
+  wave_copy(&dst->D, j.D, (int)sizeof(mdx_domain_dev));
+  wave_copy(&dst->S, j.S, (int)sizeof(mdx_source_dev));
+}
 
 template <int B>
 struct WaveBank {
@@ -1192,8 +1208,8 @@ struct WaveBank {
 
 // flags: bits 0-1 coherent scatters (saturating at 3), 2-3 incoherent, 4-11
 // majorant class (255: global), 12-19 slot of the cached sigma_tot (255:
-// none), 20-23 grid of the pending interaction + 1 (0: background), 24-29
-// valid cache entries.
+// none), 20-23 grid of the pending interaction + 1 (0: background), 24 ..
+// 24 + kBankCache - 1 valid cache entries.
 __device__ __forceinline__ uint32_t fl_get(uint32_t f, int sh, uint32_t m) { return (f >> sh) & m; }
 __device__ __forceinline__ uint32_t fl_set(uint32_t f, int sh, uint32_t m, uint32_t v) {
   return (f & ~(m << sh)) | ((v & m) << sh);
@@ -1275,7 +1291,7 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
   __shared__ mdx_material_dev lmat[MDX_MAX_MATERIALS];
   __shared__ mdx_curve lmaj;
   __shared__ uint8_t lslot[256];
-  __shared__ JobDesc desc[MULTI ? kWaves : 1];
+  __shared__ BankJob desc[MULTI ? kWaves : 1];
   __shared__ WaveBank<B> banks[kWaves];
   block_copy(lmat, T->mat, T->n_mat * (int)sizeof(mdx_material_dev));
   block_copy(lslot, T->id_to_slot, 256);
@@ -1283,7 +1299,6 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
   if (!MULTI) {
     block_copy(&desc[0].D, jobs[0].D, (int)sizeof(mdx_domain_dev));
     block_copy(&desc[0].S, jobs[0].S, (int)sizeof(mdx_source_dev));
-    block_copy(&desc[0].TL, jobs[0].TL, (int)sizeof(mdx_tallies_dev));
   }
   if (threadIdx.x < MDX_NCOUNTER) cnt[threadIdx.x] = 0;
   __syncthreads();
@@ -1303,7 +1318,9 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
   uint64_t jseed = 0, jfirst = 0, jn = total;
   uint32_t jstream = 0;
   bool jflights = false;
+  const mdx_tallies_dev* jtl = nullptr;
   auto take_job = [&](int j) {
+    jtl = jobs[j].TL;
     jseed = jobs[j].seed;
     jfirst = jobs[j].first;
     jn = MULTI ? jobs[j].n : total;
@@ -1320,13 +1337,13 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
       if (prefix[mid] <= h) lo = mid; else hi = mid - 1;
     }
     jc = lo;
-    load_job(&desc[wsl], jobs[jc]);
+    load_bank_job(&desc[wsl], jobs[jc]);
   }
   take_job(jc);
   wave_sync_lds();
   refill_min = refill_min > B ? B : refill_min;
   for (;;) {
-    const JobDesc* jd;
+    const BankJob* jd;
     {
       int o = 0;
       asm volatile("" : "+s"(o));
@@ -1334,7 +1351,7 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
     }
     const mdx_domain_dev* D = &jd->D;
     const mdx_source_dev* S = &jd->S;
-    const mdx_tallies_dev* TL = &jd->TL;
+    const mdx_tallies_dev* TL = jtl;
     const int alive = n_step + n_fly + n_int;
     if (MULTI && exhausted && alive == 0) {  // the wave's job is done here: the next one with histories left
       int jnext = -1;
@@ -1348,7 +1365,7 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
       if (jnext < 0) {
         done_all = true;
       } else {
-        load_job(&desc[wsl], jobs[jnext]);
+        load_bank_job(&desc[wsl], jobs[jnext]);
         jc = jnext;
         take_job(jc);
         exhausted = false;
