@@ -15,15 +15,17 @@ job queued at the start; then p50/p99 latency under Poisson arrivals at 70 %
 of each service's own capacity.  Phantoms are built before the clock starts
 (they are the clients' data, in pinned host memory); uploads, tally zeroing,
 transport, the ROI checks and the read-back of every job's dose and image
-are timed.  The roofline is that of the round kernel (k_transport_queue_round)
+are timed.  The round kernel is the event kernel the autotuner picks for the
+stream (k_transport_bank_round, photon banks per wave, or
+k_transport_queue_round, a photon per lane); the roofline is that kernel's
 over the timed rounds: algorithmic bytes from its int64 event counters over
 its HIP-event time.
 
 Beside it: the radiography workload (256 x 256 x 160 phantom at 1.25 mm,
 15-80 keV Kramers cone beam, 256 x 256 image, FP64 voxel dose; 10 steps of
-1e7 histories) on the queue kernel, the plain kernel, and the queue kernel
-with background free flights and local majorants off (what batching alone
-gives); the CT scan (360 views of a 512 x 512 x 256 phantom in one launch;
+1e7 histories) on the tuned event kernel, the plain kernel, and the event
+kernel with background free flights and local majorants off (what batching
+alone gives); the CT scan (360 views of a 512 x 512 x 256 phantom in one launch;
 plain on every tenth view); the precision study (the slab at 1e9 histories
 per accumulation mode); and the FP64 CPU restatement of the reference's
 OpenMP path timed on a bounded sample of the served stream's jobs on the
@@ -62,7 +64,7 @@ def profile_value(name, key):
     """A figure recorded in a committed rocprofv3 summary (None if absent)."""
     try:
         for line in open(os.path.join(ROOT, "profiles", name)):
-            if line.startswith(key):
+            if line.strip().startswith(key + " ="):
                 return float(line.split("=")[1])
     except (OSError, ValueError):
         pass
@@ -148,7 +150,23 @@ def event_kernel_name(ctx):
     return "bank%d" % bank if kind == M.EVENT_BANK else "queue"
 
 
-def served_stream(ctx, n_jobs, seed, load, cap, plain=True):
+def event_kernel_symbol(ctx):
+    """The kernel MDX_KERNEL_QUEUE launches run (without the _round suffix
+    of the graph form)."""
+    import midsx_amd as M
+    return "k_transport_bank" if ctx.event_kernel()[0] == M.EVENT_BANK else "k_transport_queue"
+
+
+def set_event_kernel(ctx, name):
+    """name: queue, bank96 or bank128."""
+    import midsx_amd as M
+    if name == "queue":
+        ctx.set_event_kernel(M.EVENT_QUEUE)
+    else:
+        ctx.set_event_kernel(M.EVENT_BANK, int(name[4:]))
+
+
+def served_stream(ctx, n_jobs, seed, load, cap, plain=True, params=None):
     """The served job stream at 1 % ROI error: the continuous-batching loop
     against plain FIFO on the same history counts."""
     from midsx_amd import scenes
@@ -173,7 +191,13 @@ def served_stream(ctx, n_jobs, seed, load, cap, plain=True):
     probe = next((j for j in jobs if j["shape"][0] == 256), jobs[0])
     pj = ctx.job_create()
     scenes.setup_dose_job(ctx, probe, ids=ph(probe))
-    tuned = ctx.autotune(probe_histories=4000000, rounds=True)  # the rounds' multi-job kernels
+    if params:  # fixed: event kernel, blocks per CU, refill, interact
+        ev, b, r, i = params.split(",")
+        set_event_kernel(ctx, ev)
+        ctx.set_queue_params(int(b), int(r), int(i))
+        tuned = dict(blocks_per_cu=int(b), refill_min=int(r), interact_min=int(i), event_kernel=ev, fixed=True)
+    else:
+        tuned = ctx.autotune(probe_histories=4000000, rounds=True)  # the rounds' multi-job kernels
     ctx.job_release(pj)
     ctx.job_select(keep_job)
     # an untimed pass first: the device memory pool and the pinned result
@@ -208,7 +232,9 @@ def served_stream(ctx, n_jobs, seed, load, cap, plain=True):
         parr = poisson_arrivals(n_jobs, load * n_jobs / plain_s, seed + 2)
         plat, _ = run_plain(ctx, pjobs, parr, phantoms=ph)
         q50, q99 = latency_percentiles(plat)
-    traffic_ph = profile_value("k_transport_queue_round_served_dose_stream.md", "traffic_bytes_per_history")
+    kernel = event_kernel_symbol(ctx) + "_round"
+    prof = "%s_served_dose_stream.md" % kernel
+    traffic_ph = profile_value(prof, "traffic_bytes_per_history")
     rounds = srv.round_log
     traffic = traffic_ph * float(np.mean([r["histories"] for r in rounds])) if traffic_ph else None
     return jobs, ph, {
@@ -236,10 +262,10 @@ def served_stream(ctx, n_jobs, seed, load, cap, plain=True):
         "latency": {"load": load, "served_arrival_rate_jobs_per_s": load * n_jobs / served_s, "p50_s": p50,
                     "p99_s": p99, "plain_arrival_rate_jobs_per_s": load * n_jobs / plain_s if plain_s else None,
                     "plain_fifo_p50_s": q50, "plain_fifo_p99_s": q99},
-        "roofline": roofline(rounds, traffic, "rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 per history of the "
-                                              "served stream (factors from the k_copy16 streaming copy), "
-                                              "profiles/k_transport_queue_round_served_dose_stream.md"
-                             if traffic else None),
+        "roofline": dict(kernel=kernel, **roofline(
+            rounds, traffic, "rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 per history of the served stream "
+                             "(factors from the k_copy16 streaming copy), profiles/" + prof if traffic else None)),
+        "event_kernel": event_kernel_name(ctx),
         "queue_params": tuned,
     }
 
@@ -255,6 +281,7 @@ def radiography(ctx, args):
     n = args.histories_per_step
     tuned = None
     if args.queue_params:
+        set_event_kernel(ctx, args.event_kernel)
         ctx.set_queue_params(*[int(v) for v in args.queue_params.split(",")])
     elif not args.no_autotune:
         tuned = ctx.autotune(probe_histories=4000000)  # launches of ~3 ms: stable timings
@@ -271,11 +298,12 @@ def radiography(ctx, args):
     agg = {k: sum(s[k] for s in stats) for k in M.STAT_NAMES}
     out["events_per_history"] = {k: agg[k] / total for k in ("steps", "real", "rayleigh", "compton", "photo",
                                                               "kahn_trials", "deposits", "surface_hits")}
-    tph = profile_value("k_transport_queue_radiography_256x256x160.md", "traffic_bytes_per_history")
-    out["roofline"] = roofline(stats, tph * n if tph else None,
-                               "rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 per launch (factors from the "
-                               "k_copy16 streaming copy), profiles/k_transport_queue_radiography_256x256x160.md"
-                               if tph else None)
+    kernel = event_kernel_symbol(ctx)
+    prof = "%s_radiography_256x256x160.md" % kernel
+    tph = profile_value(prof, "traffic_bytes_per_history")
+    out["roofline"] = dict(kernel=kernel, **roofline(
+        stats, tph * n if tph else None, "rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 per launch (factors "
+                                         "from the k_copy16 streaming copy), profiles/" + prof if tph else None))
     if tuned:
         out["autotune_probe_ms"] = tuned["probe_ms"]
     if not args.no_plain:
@@ -428,6 +456,11 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--queue-params", default=None,
                     help="blocks_per_cu,refill_min,interact_min for the radiography (skips its autotuner)")
+    ap.add_argument("--event-kernel", default="queue", choices=("queue", "bank96", "bank128"),
+                    help="event kernel of the radiography with --queue-params")
+    ap.add_argument("--served-params", default=None,
+                    help="event_kernel,blocks_per_cu,refill_min,interact_min of the served stream, e.g. "
+                         "bank128,3,64,48 (skips its autotuner)")
     ap.add_argument("--cpu-seconds", type=float, default=16.0)
     ap.add_argument("--no-ct", action="store_true")
     ap.add_argument("--ct-views", type=int, default=360)
@@ -461,11 +494,11 @@ def main():
     jobs = ph = None
     if not args.no_served:
         jobs, ph, sv = served_stream(ctx, args.served_jobs, args.served_seed, args.served_load, args.served_cap,
-                                     plain=not args.no_plain)
+                                     plain=not args.no_plain, params=args.served_params)
         out.update(value=sv["value"], steps=sv["rounds"], warmup=1, ms_per_step=sv["served_s"] / sv["rounds"] * 1e3,
                    speedup_vs_plain=sv["speedup_vs_plain"], latency=sv["latency"], roofline=sv.pop("roofline"))
         out["config"] = {"workload": "served dose-job stream: " + sv["workload"],
-                         "kernel": "served = continuous batching, k_transport_queue_round per round (hipGraph); "
+                         "kernel": "served = continuous batching, " + out["roofline"]["kernel"] + " per round (hipGraph); "
                                    "plain = k_transport_plain, one launch per job, FIFO; same histories per job, "
                                    "hence matched sigma",
                          "step": "one round of the continuous-batching loop"}
