@@ -271,8 +271,8 @@ def served_stream(ctx, n_jobs, seed, load, cap, plain=True, params=None):
 
 
 def radiography(ctx, args):
-    """Config 2 on the queue kernel (autotuned), the plain kernel, and the
-    queue kernel with free flights and local majorants off."""
+    """Config 2 on the tuned event kernel (autotuned), the plain kernel, and
+    the event kernel with free flights and local majorants off."""
     import midsx_amd as M
     from midsx_amd import scenes
     kw = dict(shape=(256, 256, 160), voxel_cm=0.125, seed=2, image=(256, 256))
@@ -320,8 +320,8 @@ def radiography(ctx, args):
         ctx.set_flights(True)
         ctx.set_local_majorants(True)
         out["ablation_majorant_steps"] = {
-            "what": "queue kernel with background free flights and local majorants off (the plain kernel's "
-                    "Woodcock steps, event-regrouped and batched), same launch parameters",
+            "what": "the tuned event kernel with background free flights and local majorants off (the plain "
+                    "kernel's Woodcock steps, event-regrouped and batched), same launch parameters",
             "value": total / awall, "ms_per_step": awall / args.steps * 1e3,
             "steps_per_history": sum(s["steps"] for s in astats) / total,
             "speedup_vs_plain": (total / awall) / out["plain"]["value"]}
