@@ -298,7 +298,13 @@ struct mdx_ctx {
   DBuf d_work;  // queue kernel: histories claimed, one counter per job of a launch
   DBuf d_jobs, d_prefix;
   DBuf d_dose32;  // staging of mdx_get_voxel_dose_f32 (on the I/O stream)
-  DBuf d_roi_req, d_roi_out;  // mdx_roi_stats requests and sums
+  // Pinned host mailbox that kernels write small results into directly (the
+  // round's counters, ROI sums) and read ROI requests from: no DMA copy, so
+  // these reads never queue behind a large result copy on the copy engines.
+  // Layout: [counters MDX_NCOUNTER u64][ROI requests][ROI sums].
+  void* h_mail = nullptr;
+  void* d_mail = nullptr;  // the mailbox's address on the device
+  size_t mail_roi = 0;     // ROI boxes the mailbox holds
 
   // Round buffer of the graph path: [n_jobs, total, prefix[kRoundMax + 1], jobs[kRoundMax]]
   bool use_graphs = false;
@@ -862,11 +868,41 @@ struct DevArray {
 };
 
 
+constexpr size_t kMailCounters = MDX_NCOUNTER * sizeof(unsigned long long);
+constexpr size_t kMailRoiOff = (kMailCounters + 255) & ~size_t(255);
+// The mailbox's parts, at their host (dev = false) or device address.
+mdx_roi_req* mail_roi_req(mdx_ctx* c, bool dev) {
+  return reinterpret_cast<mdx_roi_req*>(static_cast<char*>(dev ? c->d_mail : c->h_mail) + kMailRoiOff);
+}
+double* mail_roi_out(mdx_ctx* c, bool dev) { return reinterpret_cast<double*>(mail_roi_req(c, dev) + c->mail_roi); }
+
+// The mailbox with room for n ROI boxes (grown by reallocation: the device
+// is idle then, as mdx_roi_stats requires).
+void ensure_mail(mdx_ctx* c, size_t n) {
+  if (c->h_mail && n <= c->mail_roi) return;
+  const size_t cap = std::max<size_t>(n, 2 * c->mail_roi);
+  const size_t bytes = kMailRoiOff + cap * (sizeof(mdx_roi_req) + 3 * sizeof(double));
+  void* p = nullptr;
+  void* dp = nullptr;
+  hip_check(hipHostMalloc(&p, bytes, hipHostMallocDefault), "hipHostMalloc");
+  const hipError_t e = hipHostGetDevicePointer(&dp, p, 0);
+  if (e != hipSuccess) {
+    (void)hipHostFree(p);
+    hip_check(e, "hipHostGetDevicePointer");
+  }
+  if (c->h_mail) (void)hipHostFree(c->h_mail);
+  c->h_mail = p;
+  c->d_mail = dp;
+  c->mail_roi = cap;
+}
+
 void read_stats(mdx_ctx* c, mdx_stats* stats) {
   if (!stats) return;
   std::memset(stats, 0, sizeof *stats);
-  unsigned long long h[MDX_NCOUNTER];
-  hip_check(hipMemcpyAsync(h, c->d_counters.p, sizeof h, hipMemcpyDeviceToHost, c->stream), "counters");
+  const unsigned long long* h = static_cast<const unsigned long long*>(c->h_mail);
+  hip_check(mdx_launch::copy_u64(c->d_counters.as<unsigned long long>(), static_cast<unsigned long long*>(c->d_mail),
+                                 MDX_NCOUNTER, c->stream),
+            "counters");
   hip_check(hipStreamSynchronize(c->stream), "counters");
   for (int i = 0; i < MDX_NCOUNTER && i < MDX_STAT_NCOUNTERS; ++i) stats->counters[i] = h[i];
   float ms = 0;
@@ -1090,6 +1126,7 @@ int mdx_create(int device, mdx_ctx** out) {
     c->d_counters.alloc(MDX_NCOUNTER * sizeof(unsigned long long));
     c->d_work.alloc(kRoundMax * sizeof(unsigned long long));
     hip_check(mdx_launch::scratch_address(device, &c->scratch), "hipGetSymbolAddress(scratch)");
+    ensure_mail(c.get(), 256);
     *out = c.release();
   });
 }
@@ -1109,8 +1146,10 @@ int mdx_destroy(mdx_ctx* c) {
     hipEvent_t e0 = c->ev0, e1 = c->ev1;
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     void* pinned = c->h_round;
+    void* mail = c->h_mail;
     delete c;
     if (pinned) (void)hipHostFree(pinned);
+    if (mail) (void)hipHostFree(mail);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipStreamDestroy(st);
@@ -2366,7 +2405,9 @@ int mdx_roi_stats(mdx_ctx* c, int n, const int* jobs, const int* boxes, double* 
     need(!c->inflight, "a round is in flight (mdx_wait first)");
     if (n == 0) return;
     set_device(c);
-    std::vector<mdx_roi_req> req(n);
+    hip_check(hipStreamSynchronize(c->stream), "roi");  // the mailbox may be regrown only with the device idle
+    ensure_mail(c, (size_t)n);
+    mdx_roi_req* req = mail_roi_req(c, false);
     for (int b = 0; b < n; ++b) {
       Job& O = job_at(c, domain_owner(c, jobs[b]));
       need(O.voxel_dose && !O.grids.empty(), "the job's voxel dose is not enabled");
@@ -2384,12 +2425,11 @@ int mdx_roi_stats(mdx_ctx* c, int n, const int* jobs, const int* boxes, double* 
       }
       flush_clears_over(c, G.d_dose.p, G.d_dose.bytes);
     }
-    c->d_roi_req.upload(req.data(), req.size(), c->stream);
-    c->d_roi_out.ensure((size_t)3 * n * sizeof(double));
-    hip_check(mdx_launch::roi_stats(c->d_roi_req.as<mdx_roi_req>(), n, c->d_roi_out.as<double>(), c->stream), "roi");
-    hip_check(hipMemcpyAsync(out, c->d_roi_out.p, (size_t)3 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream),
-              "read roi");
+    // the kernel reads the requests from and writes the sums to the pinned
+    // mailbox directly: no copy engine involved
+    hip_check(mdx_launch::roi_stats(mail_roi_req(c, true), n, mail_roi_out(c, true), c->stream), "roi");
     hip_check(hipStreamSynchronize(c->stream), "roi");
+    std::memcpy(out, mail_roi_out(c, false), (size_t)3 * n * sizeof(double));
   });
 }
 

@@ -65,6 +65,8 @@ hipError_t kahn(const mdx_tables_dev* T, int slot, int n, const float* E, uint64
 hipError_t zero(double* p, size_t n, hipStream_t st);
 hipError_t zero_u32(uint32_t* p, size_t n, hipStream_t st);
 hipError_t zero_u64(unsigned long long* p, size_t n, hipStream_t st);
+// dst[i] = src[i] for i < n (dst may be pinned host memory).
+hipError_t copy_u64(const unsigned long long* src, unsigned long long* dst, int n, hipStream_t st);
 // Clears the ranges listed in device memory as {count, (address, bytes) x
 // count}; addresses 16-byte aligned, sizes multiples of 4.
 hipError_t zero_list(const uint64_t* list, hipStream_t st);

@@ -1971,6 +1971,12 @@ __global__ void __launch_bounds__(256) k_roi_stats(const mdx_roi_req* __restrict
   if (threadIdx.x < 3) out[3 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
 }
 
+// A few 64-bit words from device memory to pinned host memory (the round's
+// event counters), written by the kernel instead of a copy engine.
+__global__ void k_copy_u64(const unsigned long long* __restrict__ src, unsigned long long* __restrict__ dst, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
+
 // Plain streaming copy (16 B per lane per iteration): the HBM reference for
 // scaling traffic counters.
 __global__ void k_copy16(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
@@ -2202,6 +2208,12 @@ hipError_t copy16(const void* src, void* dst, size_t n16, int blocks, hipStream_
   if (n16 == 0) return hipSuccess;
   hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(256), 0, st, static_cast<const float4*>(src),
                      static_cast<float4*>(dst), n16);
+  return hipGetLastError();
+}
+
+hipError_t copy_u64(const unsigned long long* src, unsigned long long* dst, int n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_u64, dim3(1), dim3(64), 0, st, src, dst, n);
   return hipGetLastError();
 }
 

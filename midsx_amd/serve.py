@@ -154,6 +154,9 @@ class Server:
         self.counters = {}
         self.host_s = dict(admit=0.0, read=0.0, wait=0.0, release=0.0, launch=0.0, roi=0.0)
         self.round_log = []  # per round: the launch's event counters and kernel time
+        # per loop pass: host seconds in each phase, the voxels of the jobs it
+        # admitted and the jobs it finished
+        self.pass_log = []
 
     # ------------------------------------------------------------------ helpers
     def _pinned(self, n):
@@ -247,6 +250,7 @@ class Server:
         t0 = time.perf_counter()
         clock = lambda: time.perf_counter() - t0
         while order or waiting or active or to_read or copying:
+            h0 = dict(self.host_s)
             now = clock()
             while order and arrivals[order[0]] <= now:
                 waiting.append(order.popleft())
@@ -338,6 +342,9 @@ class Server:
                 for a in to_read:  # conversion queued ahead of the next round; the copies run beside it
                     ctx.job_select(a.slot)
                     ctx.stage_voxel_dose_f32(0, a.done)
+            self.pass_log.append(dict({k: self.host_s[k] - h0[k] for k in h0}, launched=launched,
+                                      admitted_voxels=[int(np.prod(a.job["shape"])) for a in admitted],
+                                      finished=len(to_read)))
             if idle and order:
                 time.sleep(max(0.0, min(arrivals[order[0]] - clock(), 0.05)))
         self.makespan = clock()
