@@ -250,6 +250,21 @@ __device__ __forceinline__ void curve_eval2(const mdx_tables_dev* T, const mdx_c
   v2 = curve_finish(T, c2, X2, a2);
 }
 
+// Three curves with their table loads issued together (one chain of
+// dependent loads instead of three).
+__device__ __forceinline__ void curve_eval3(const mdx_tables_dev* T, const mdx_curve& c1, float X1,
+                                            const mdx_curve& c2, float X2, const mdx_curve& c3, float X3,
+                                            float& v1, float& v2, float& v3) {
+  const int k1 = curve_hash(T, c1, X1), k2 = curve_hash(T, c2, X2), k3 = curve_hash(T, c3, X3);
+  CurveAt a1 = curve_nodes(T, c1, k1), a2 = curve_nodes(T, c2, k2), a3 = curve_nodes(T, c3, k3);
+  curve_fix(T, c1, X1, a1);
+  curve_fix(T, c2, X2, a2);
+  curve_fix(T, c3, X3, a3);
+  v1 = curve_finish(T, c1, X1, a1);
+  v2 = curve_finish(T, c2, X2, a2);
+  v3 = curve_finish(T, c3, X3, a3);
+}
+
 // ------------------------------------------------------------------ geometry
 // computational_domain.cpp:16-21 (inclusive bounds)
 __device__ __forceinline__ bool in_domain(const mdx_domain_dev* D, float x, float y, float z) {

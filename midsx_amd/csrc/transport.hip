@@ -271,12 +271,17 @@ __device__ void generate(const mdx_source_dev* S, Philox& rng, float& px, float&
 // acceptance rate above 0.1) the last candidate is taken, so a corrupt table
 // cannot hang the kernel.
 constexpr uint32_t kMaxKahnTrials = 4096;
+// The argument of S(x_max) in Kahn's S-test at energy E.
+__device__ __forceinline__ float compton_xmax(float E) { return kAlphaSqrt2 * (E * kInvMc2); }
+
+// s_max: S(compton_xmax(E)) when the caller has it (evaluated beside other
+// table lookups), else a negative value.
 __device__ float compton(const mdx_tables_dev* T, const mdx_material_dev& M, Philox& rng, float E, float& mu,
-                         uint32_t& trials, uint32_t& s_tests) {
+                         uint32_t& trials, uint32_t& s_tests, float s_max = -1.0f) {
   const float k = E * kInvMc2;
   const float ik = frcp(k), k2 = 1.0f + 2.0f * k;
   const float r1p = k2 * frcp(9.0f + 2.0f * k);
-  const float is_max = frcp(curve_eval(T, M.sf, kAlphaSqrt2 * k));
+  const float is_max = frcp(s_max >= 0.0f ? s_max : curve_eval(T, M.sf, compton_xmax(E)));
   // Trial it draws (r1, r2, r3) for Kahn's test and, once that passes (or
   // the trial cap is reached), r4 for the S(x)/S_max test.  The Kahn draws
   // loop on their own so that a wave runs the table lookup of the S-test
@@ -1468,7 +1473,9 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
             else if (c8 != 255u && (c8 & MDX_MACRO_UNIFORM)) sl = (int)(c8 & 0x7Fu);
             else sl = (int)lslot[gld(D->grids[g].ids + lin)];
             if (sl != cslot) {
-              p.stot = bank_sigma(T, lmat, lmaj, W, slot, p, sl);
+              // a local majorant of class k is material k's own total cross
+              // section at this energy: the same value, no second lookup
+              p.stot = sl == (int)fl_get(p.flags, 4, 255u) ? p.smax : bank_sigma(T, lmat, lmaj, W, slot, p, sl);
               cslot = sl;
             }
             if (!limited && real_collision(u2, delta_threshold(p.stot, p.smax))) {
@@ -1488,7 +1495,7 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
             if (nr == 0u || p.E < 1.0e3f) ncls = 255;
             if (ncls != (int)fl_get(p.flags, 4, 255u)) {
               p.flags = fl_set(p.flags, 4, 255u, (uint32_t)ncls);
-              p.smax = bank_sigma(T, lmat, lmaj, W, slot, p, ncls);
+              p.smax = ncls == cslot ? p.stot : bank_sigma(T, lmat, lmaj, W, slot, p, ncls);
             }
             p.dsafe = ncls == 255 ? INFINITY : (float)nr * D->grids[0].mcell;
           }
@@ -1552,8 +1559,10 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
         Philox rng;
         rng.init(jseed, jstream, (uint64_t)p.hlo | ((uint64_t)p.hhi << 32));
         rng.c0 = p.c0;
-        float sc, si;
-        curve_eval2(T, M.coh, p.E, M.incoh, p.E, sc, si);
+        // the Compton sampler's S(x_max) is looked up beside the partial
+        // cross sections (it depends only on the energy and the material)
+        float sc, si, smx;
+        curve_eval3(T, M.coh, p.E, M.incoh, p.E, M.sf, compton_xmax(p.E), sc, si, smx);
         const int type = select_type(sc, si, p.stot, rng.uniform());
         float dep = 0.0f;
         bool alive_ = true;
@@ -1566,7 +1575,7 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
           ++c_comp;
           ni = ni < 3 ? ni + 1 : 3;
           float mu;
-          const float Ep = compton(T, M, rng, p.E, mu, c_trial, c_stest);
+          const float Ep = compton(T, M, rng, p.E, mu, c_trial, c_stest, smx);
           rotate(p.dx, p.dy, p.dz, mu, 2.0f * kPiF * rng.uniform());
           dep = p.E - Ep;
           p.E = Ep;
@@ -1639,8 +1648,10 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
   if (threadIdx.x < MDX_NCOUNTER && cnt[threadIdx.x]) atomicAdd(counters + threadIdx.x, cnt[threadIdx.x]);
 }
 
+// Waves per SIMD the registers must allow: the single-job kernel with
+// 96-photon banks fits four blocks per CU in LDS, the others three.
 template <bool MULTI, int B>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(!MULTI && B <= 96 ? 4 : 3)))
 k_transport_bank(const mdx_tables_dev* __restrict__ T, const mdx_job_dev* __restrict__ jobs,
                  const uint64_t* __restrict__ prefix, int n_jobs, uint64_t total, unsigned long long* __restrict__ work,
                  unsigned long long* __restrict__ counters, int refill_min, int interact_min) {
