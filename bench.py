@@ -10,9 +10,11 @@ as the reference computes it, quantity.cpp:37-45).  The continuous-batching
 loop runs each job until every ROI voxel is at or below 1 % (checked on the
 device after every round); the plain kernel then serves the same jobs with
 the same history counts first-in first-out, one launch per job, so both
-sides reach the same sigma.  `value` is the served histories/s with every
-job queued at the start; then p50/p99 latency under Poisson arrivals at 70 %
-of each service's own capacity.  Phantoms are built before the clock starts
+sides reach the same sigma.  A step is one pass of the whole stream:
+--warmup untimed passes, then --steps timed ones; `value` is the served
+histories/s over the timed passes with every job of a pass queued at its
+start; then p50/p99 latency under Poisson arrivals at 70 % of each
+service's own capacity.  Phantoms are built before the clock starts
 (they are the clients' data, in pinned host memory); uploads, tally zeroing,
 transport, the ROI checks and the read-back of every job's dose and image
 are timed.  The round kernel is the event kernel the autotuner picks for the
@@ -166,9 +168,11 @@ def set_event_kernel(ctx, name):
         ctx.set_event_kernel(M.EVENT_BANK, int(name[4:]))
 
 
-def served_stream(ctx, n_jobs, seed, load, cap, plain=True, params=None):
+def served_stream(ctx, n_jobs, seed, load, cap, steps, warmup, plain=True, params=None):
     """The served job stream at 1 % ROI error: the continuous-batching loop
-    against plain FIFO on the same history counts."""
+    against plain FIFO on the same history counts.  A step is one pass of
+    the whole stream (every job served to its ROI target): warmup untimed
+    passes, then steps timed ones."""
     from midsx_amd import scenes
     from midsx_amd.serve import Server, run_plain, poisson_arrivals, latency_percentiles, warm_read_buffers
     jobs = [dict(j, histories=cap, rse_target=RSE_TARGET) for j in scenes.draw_dose_jobs(n_jobs, seed)]
@@ -200,22 +204,37 @@ def served_stream(ctx, n_jobs, seed, load, cap, plain=True, params=None):
         tuned = ctx.autotune(probe_histories=4000000, rounds=True)  # the rounds' multi-job kernels
     ctx.job_release(pj)
     ctx.job_select(keep_job)
-    # an untimed pass first: the device memory pool and the pinned result
+    # untimed passes first: the device memory pool and the pinned result
     # buffers are then warm, as in a service that has been running
-    progress("served stream: tuned %s; warm pass" % (tuned,))
-    Server(ctx, phantoms=ph).run(jobs)
-    progress("served stream: timed pass")
-    srv = Server(ctx, phantoms=ph)
-    t = time.perf_counter()
-    res = srv.run(jobs)
-    served_s = time.perf_counter() - t
-    hist = [r.histories for r in res]
-    H = sum(hist)
-    served_rse = [r.roi_rse for r in res]
-    capped = sum(1 for h in hist if h >= cap)
-    progress("served stream: %.3f s for %d histories; latency pass" % (served_s, H))
+    progress("served stream: tuned %s; %d warm-up passes" % (tuned, warmup))
+    for _ in range(warmup):
+        Server(ctx, phantoms=ph).run(jobs)
+    passes = []  # (server, results, seconds) of each timed pass
+    for i in range(steps):
+        srv = Server(ctx, phantoms=ph)
+        t = time.perf_counter()
+        res = srv.run(jobs)
+        passes.append((srv, res, time.perf_counter() - t))
+        progress("served stream: timed pass %d of %d, %.3f s for %d histories" % (
+            i + 1, steps, passes[-1][2], sum(r.histories for r in res)))
+    served_s = sum(p[2] for p in passes)
+    H = sum(r.histories for p in passes for r in p[1])
+    pass_s = served_s / steps
+    hist = [r.histories for r in passes[-1][1]]  # the last pass's per-job histories (what plain runs)
+    served_rse = [r.roi_rse for p in passes for r in p[1]]
+    capped = sum(1 for p in passes for r in p[1] if r.histories >= cap)
+    rounds = [st for p in passes for st in p[0].round_log]
+    counters = {}
+    for p in passes:
+        for k, v in p[0].counters.items():
+            counters[k] = counters.get(k, 0) + v
+    host_s = {}
+    for p in passes:
+        for k, v in p[0].host_s.items():
+            host_s[k] = host_s.get(k, 0.0) + v / steps
+    progress("served stream: %d passes, %.3f s for %d histories; latency pass" % (steps, served_s, H))
     # latency under Poisson arrivals at `load` of each service's own capacity
-    arr = poisson_arrivals(n_jobs, load * n_jobs / served_s, seed + 1)
+    arr = poisson_arrivals(n_jobs, load * n_jobs / pass_s, seed + 1)
     p50, p99 = latency_percentiles(Server(ctx, phantoms=ph).run(jobs, arr))
     plain_s = q50 = q99 = None
     plain_rse = [float("nan")]
@@ -235,31 +254,35 @@ def served_stream(ctx, n_jobs, seed, load, cap, plain=True, params=None):
     kernel = event_kernel_symbol(ctx) + "_round"
     prof = "%s_served_dose_stream.md" % kernel
     traffic_ph = profile_value(prof, "traffic_bytes_per_history")
-    rounds = srv.round_log
     traffic = traffic_ph * float(np.mean([r["histories"] for r in rounds])) if traffic_ph else None
+    Hp = sum(hist)
     return jobs, ph, {
         "workload": "%d dose jobs (seed %d), phantoms %s voxels at 0.5-2 mm, each run to <= %g relative "
                     "standard error on its 4 ROI dose voxels (1 cm^3, on the beam axis 1-2 cm deep); "
-                    "histories per job min %.3g, median %.3g, max %.3g, total %d" % (
+                    "histories per job min %.3g, median %.3g, max %.3g, %d per pass; a step is one pass of the "
+                    "whole stream" % (
                         n_jobs, seed, sorted(set(j["shape"][0] for j in jobs)), RSE_TARGET, min(hist),
-                        float(np.median(hist)), max(hist), H),
+                        float(np.median(hist)), max(hist), Hp),
         "value": H / served_s,
-        "plain_value": H / plain_s if plain_s else None,
-        "speedup_vs_plain": plain_s / served_s if plain_s else None,
+        "plain_value": Hp / plain_s if plain_s else None,
+        "speedup_vs_plain": (H / served_s) / (Hp / plain_s) if plain_s else None,
         "jobs_at_cap": capped,
+        "passes": steps,
+        "warmup_passes": warmup,
         "served_s": served_s,
+        "pass_s": [round(p[2], 4) for p in passes],
         "plain_s": plain_s,
-        "kernel_s": srv.kernel_ms / 1e3,
+        "kernel_s": sum(p[0].kernel_ms for p in passes) / 1e3,
         "plain_kernel_s": pinfo["kernel_ms"] / 1e3,
-        "rounds": srv.rounds,
-        "host_s": {k: round(v, 4) for k, v in srv.host_s.items()},
-        "steps_per_history": srv.counters["steps"] / H,
-        "events_per_history": {k: srv.counters[k] / H for k in ("steps", "real", "rayleigh", "compton", "photo",
-                                                                 "kahn_trials", "deposits", "surface_hits")},
-        "plain_steps_per_history": pinfo["counters"]["steps"] / H,
+        "rounds": len(rounds),
+        "host_s_per_pass": {k: round(v, 4) for k, v in host_s.items()},
+        "steps_per_history": counters["steps"] / H,
+        "events_per_history": {k: counters[k] / H for k in ("steps", "real", "rayleigh", "compton", "photo",
+                                                             "kahn_trials", "deposits", "surface_hits")},
+        "plain_steps_per_history": pinfo["counters"]["steps"] / Hp,
         "roi_rse": {"served_max": float(np.max(served_rse)), "served_median": float(np.median(served_rse)),
                     "plain_max": float(np.max(plain_rse)), "plain_median": float(np.median(plain_rse))},
-        "latency": {"load": load, "served_arrival_rate_jobs_per_s": load * n_jobs / served_s, "p50_s": p50,
+        "latency": {"load": load, "served_arrival_rate_jobs_per_s": load * n_jobs / pass_s, "p50_s": p50,
                     "p99_s": p99, "plain_arrival_rate_jobs_per_s": load * n_jobs / plain_s if plain_s else None,
                     "plain_fifo_p50_s": q50, "plain_fifo_p99_s": q99},
         "roofline": dict(kernel=kernel, **roofline(
@@ -286,12 +309,13 @@ def radiography(ctx, args):
     elif not args.no_autotune:
         tuned = ctx.autotune(probe_histories=4000000)  # launches of ~3 ms: stable timings
     progress("radiography: queue kernel")
-    stats, wall, first = timed_steps(ctx, M.KERNEL_QUEUE, n, args.steps, args.warmup, 0)
-    total = n * args.steps
+    steps, warm = args.radiography_steps, args.radiography_warmup
+    stats, wall, first = timed_steps(ctx, M.KERNEL_QUEUE, n, steps, warm, 0)
+    total = n * steps
     out = {
         "workload": "radiography: 256x256x160 uint8 phantom at 1.25 mm, 15-80 keV Kramers cone beam, "
-                    "256x256 image + FP64 voxel dose, %d steps of %d histories" % (args.steps, n),
-        "value": total / wall, "unit": "histories/s", "ms_per_step": wall / args.steps * 1e3,
+                    "256x256 image + FP64 voxel dose, %d steps of %d histories" % (steps, n),
+        "value": total / wall, "unit": "histories/s", "ms_per_step": wall / steps * 1e3,
         "queue_params": dict(zip(("blocks_per_cu", "refill_min", "interact_min"), ctx.queue_params())),
         "event_kernel": event_kernel_name(ctx),
     }
@@ -308,21 +332,21 @@ def radiography(ctx, args):
         out["autotune_probe_ms"] = tuned["probe_ms"]
     if not args.no_plain:
         progress("radiography: plain kernel")
-        pstats, pwall, first = timed_steps(ctx, M.KERNEL_PLAIN, n, args.steps, args.warmup, first)
-        out["plain"] = {"value": total / pwall, "ms_per_step": pwall / args.steps * 1e3,
+        pstats, pwall, first = timed_steps(ctx, M.KERNEL_PLAIN, n, steps, warm, first)
+        out["plain"] = {"value": total / pwall, "ms_per_step": pwall / steps * 1e3,
                         "steps_per_history": sum(s["steps"] for s in pstats) / total}
         out["speedup_vs_plain"] = out["value"] / out["plain"]["value"]
         # what batching and regrouping alone give: majorant steps everywhere
         progress("radiography: ablation")
         ctx.set_flights(False)
         ctx.set_local_majorants(False)
-        astats, awall, first = timed_steps(ctx, M.KERNEL_QUEUE, n, args.steps, args.warmup, first)
+        astats, awall, first = timed_steps(ctx, M.KERNEL_QUEUE, n, steps, warm, first)
         ctx.set_flights(True)
         ctx.set_local_majorants(True)
         out["ablation_majorant_steps"] = {
             "what": "the tuned event kernel with background free flights and local majorants off (the plain "
                     "kernel's Woodcock steps, event-regrouped and batched), same launch parameters",
-            "value": total / awall, "ms_per_step": awall / args.steps * 1e3,
+            "value": total / awall, "ms_per_step": awall / steps * 1e3,
             "steps_per_history": sum(s["steps"] for s in astats) / total,
             "speedup_vs_plain": (total / awall) / out["plain"]["value"]}
     ctx.job_release(job)
@@ -442,8 +466,11 @@ def precision_study(ctx, n_hist, step=10 ** 8):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10, help="radiography steps of --histories-per-step")
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=10,
+                    help="timed passes of the served stream (a step: every job served to its ROI target)")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed passes of the served stream before them")
+    ap.add_argument("--radiography-steps", type=int, default=10, help="radiography steps of --histories-per-step")
+    ap.add_argument("--radiography-warmup", type=int, default=2)
     ap.add_argument("--histories-per-step", type=int, default=10 ** 7)
     ap.add_argument("--served-jobs", type=int, default=32)
     ap.add_argument("--served-seed", type=int, default=2026)
@@ -493,15 +520,19 @@ def main():
     }
     jobs = ph = None
     if not args.no_served:
+        if args.steps < 1 or args.warmup < 0:
+            ap.error("--steps must be at least 1 and --warmup at least 0")
         jobs, ph, sv = served_stream(ctx, args.served_jobs, args.served_seed, args.served_load, args.served_cap,
-                                     plain=not args.no_plain, params=args.served_params)
-        out.update(value=sv["value"], steps=sv["rounds"], warmup=1, ms_per_step=sv["served_s"] / sv["rounds"] * 1e3,
+                                     args.steps, args.warmup, plain=not args.no_plain, params=args.served_params)
+        out.update(value=sv["value"], steps=args.steps, warmup=args.warmup,
+                   ms_per_step=sv["served_s"] / args.steps * 1e3,
                    speedup_vs_plain=sv["speedup_vs_plain"], latency=sv["latency"], roofline=sv.pop("roofline"))
         out["config"] = {"workload": "served dose-job stream: " + sv["workload"],
                          "kernel": "served = continuous batching, " + out["roofline"]["kernel"] + " per round (hipGraph); "
                                    "plain = k_transport_plain, one launch per job, FIFO; same histories per job, "
                                    "hence matched sigma",
-                         "step": "one round of the continuous-batching loop"}
+                         "step": "one pass of the served stream: all %d jobs served to their ROI target"
+                                 % args.served_jobs}
         out["served"] = sv
     if not args.no_radiography:
         out["radiography"] = radiography(ctx, args)
