@@ -133,6 +133,24 @@ __device__ __forceinline__ T gld(const T* p) {
   return *(const MDX_GLOBAL T*)p;
 }
 
+// Descriptors that no kernel writes while it runs (a job's domain, source and
+// tallies), reached through a pointer that is the same in every lane of the
+// wave: made a wave-uniform pointer into the constant address space, so that
+// once inlined their loads are scalar loads into SGPRs through the scalar
+// cache rather than per-lane vector loads into VGPRs.  The empty asm keeps the
+// pointer opaque, so that the loads stay where they are used instead of being
+// hoisted out of a loop and held in registers across it.
+#define MDX_CONST __attribute__((address_space(4)))
+
+template <class T>
+__device__ __forceinline__ const T* wave_const(const T* p) {
+  uint64_t b = (uint64_t)(uintptr_t)p;
+  uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
+  uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
+  asm volatile("" : "+s"(lo), "+s"(hi));
+  return (const T*)(const MDX_CONST T*)(((uint64_t)hi << 32) | lo);
+}
+
 typedef float mdx_v4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ mdx_f4 gld(const mdx_f4* p) {
@@ -291,22 +309,29 @@ __device__ __forceinline__ int trunc_quotient(float q, float s, float inv_s) {
 // (position - origin) / spacing.  A point exactly on a grid's far face maps to
 // the last voxel (the reference's at() throws there).  Returns the grid, or -1
 // for the background voxel.
+// The voxel of grid G holding a point of its box (offsets qx, qy, qz >= 0
+// from its origin): linear index, and optionally the voxel's macro cell.
+__device__ __forceinline__ uint64_t grid_voxel(const mdx_grid_dev& G, float qx, float qy, float qz,
+                                               uint32_t* cell = nullptr) {
+  int i = trunc_quotient(qx, G.spacing[0], G.inv_spacing[0]);
+  int j = trunc_quotient(qy, G.spacing[1], G.inv_spacing[1]);
+  int k = trunc_quotient(qz, G.spacing[2], G.inv_spacing[2]);
+  i = i >= G.n[0] ? G.n[0] - 1 : i;
+  j = j >= G.n[1] ? G.n[1] - 1 : j;
+  k = k >= G.n[2] ? G.n[2] - 1 : k;
+  if (cell)  // the voxel's macro cell (local majorants)
+    *cell = (uint32_t)(i >> MDX_MACRO_SHIFT) +
+            (uint32_t)G.mcx * ((uint32_t)(j >> MDX_MACRO_SHIFT) + (uint32_t)G.mcy * (uint32_t)(k >> MDX_MACRO_SHIFT));
+  return (uint64_t)i + (uint64_t)G.n[0] * ((uint64_t)j + (uint64_t)G.n[1] * (uint64_t)k);
+}
+
 __device__ __forceinline__ int voxel_lookup(const mdx_domain_dev* D, float x, float y, float z, uint64_t& lin,
                                             uint32_t* cell = nullptr) {
   for (int g = 0; g < D->n_grids; ++g) {
     const mdx_grid_dev& G = D->grids[g];
     float qx = x - G.origin[0], qy = y - G.origin[1], qz = z - G.origin[2];
     if (qx >= 0.0f && qy >= 0.0f && qz >= 0.0f && qx <= G.dim[0] && qy <= G.dim[1] && qz <= G.dim[2]) {
-      int i = trunc_quotient(qx, G.spacing[0], G.inv_spacing[0]);
-      int j = trunc_quotient(qy, G.spacing[1], G.inv_spacing[1]);
-      int k = trunc_quotient(qz, G.spacing[2], G.inv_spacing[2]);
-      i = i >= G.n[0] ? G.n[0] - 1 : i;
-      j = j >= G.n[1] ? G.n[1] - 1 : j;
-      k = k >= G.n[2] ? G.n[2] - 1 : k;
-      lin = (uint64_t)i + (uint64_t)G.n[0] * ((uint64_t)j + (uint64_t)G.n[1] * (uint64_t)k);
-      if (cell)  // the voxel's macro cell (local majorants)
-        *cell = (uint32_t)(i >> MDX_MACRO_SHIFT) +
-                (uint32_t)G.mcx * ((uint32_t)(j >> MDX_MACRO_SHIFT) + (uint32_t)G.mcy * (uint32_t)(k >> MDX_MACRO_SHIFT));
+      lin = grid_voxel(G, qx, qy, qz, cell);
       return g;
     }
   }

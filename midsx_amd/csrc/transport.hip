@@ -452,6 +452,14 @@ k_transport_plain(const mdx_tables_dev* __restrict__ T, const mdx_domain_dev* __
 // go to global FP64 atomics for several.
 // A lane's value read by the whole wave (v_readlane: a scalar result, no LDS
 // round trip as a shuffle takes); src must be wave-uniform and active.
+// The value of one lane (a wave-uniform lane index), known to the compiler
+// as wave-uniform.
+__device__ __forceinline__ unsigned long long readlane_u64(unsigned long long v, int src) {
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, src);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), src);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
 __device__ __forceinline__ double readlane_f64(double v, int src) {
   const unsigned long long b = __double_as_longlong(v);
   const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)b, src);
@@ -849,7 +857,8 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
             int jn = -1;
             for (int k = 0; k < n_jobs; ++k) {
               const int j = (jc_last + 1 + k) % n_jobs;
-              if (__hip_atomic_load(work + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < jobs[j].n) {
+              const int left = __hip_atomic_load(work + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < jobs[j].n;
+              if (__builtin_amdgcn_readfirstlane(left)) {  // the same counter in every lane: wave-uniform
                 jn = j;
                 break;
               }
@@ -866,7 +875,7 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
           const uint64_t nj = jobs[jc].n;
           unsigned long long base = 0;
           if (lane == leader) base = atomicAdd(work + jc, (unsigned long long)__popcll(fmask));
-          base = __shfl(base, leader);
+          base = readlane_u64(base, leader);  // the leader's claim, wave-uniform
           if (base >= nj) {
             jc_last = jc;
             jc = -1;
@@ -1039,7 +1048,7 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
       if (pend) {
         pend = false;
         ++c_real;
-        const mdx_material_dev& M = lmat[cslot];
+        const mdx_material_dev& M = T->mat[cslot];
         Step r{qx, qy, qz, dx, dy, dz, qs, E, nc, ni, 0, 0, 0.0f, true};
         float sc, si;
         curve_eval2(T, M.coh, E, M.incoh, E, sc, si);
@@ -1184,19 +1193,6 @@ enum { kEvStep = 0, kEvFly = 1, kEvInt = 2, kEvFree = 3, kEvTypes = 4 };
 // CU) than the table lookups it saves.
 constexpr int kBankCache = 1;
 
-// A wave's job in the bank kernel: domain and source in LDS (read by every
-// step); the tally descriptors are read from global memory with wave-uniform
-// addresses (scalar loads), which keeps three blocks' banks within a CU's LDS.
-struct BankJob {
-  mdx_domain_dev D;
-  mdx_source_dev S;
-};
-
-__device__ __forceinline__ void load_bank_job(BankJob* dst, const mdx_job_dev& j) {
-  wave_copy(&dst->D, j.D, (int)sizeof(mdx_domain_dev));
-  wave_copy(&dst->S, j.S, (int)sizeof(mdx_source_dev));
-}
-
 template <int B>
 struct WaveBank {
   mdx_v4 pe[B];   // position, energy
@@ -1206,7 +1202,6 @@ struct WaveBank {
   mdx_v4 aux[B];
   uint32_t hlo[B], hhi[B], c0[B], nstep[B];  // history index, Philox block counter, step counter
   uint32_t flags[B];  // see fl_get / fl_set
-  uint32_t qlin[B];   // voxel of the pending interaction
   float sc[kBankCache][B];
   uint8_t list[kEvTypes][B];
 };
@@ -1235,7 +1230,7 @@ __device__ __forceinline__ void step_uniforms(uint32_t key, uint32_t hlo, uint32
 // One photon in a lane while an event runs on it.
 struct Ph {
   float px, py, pz, E, dx, dy, dz, smax, dsafe, stot, tfl, qs;
-  uint32_t hlo, hhi, c0, nstep, flags, qlin;
+  uint32_t hlo, hhi, c0, nstep, flags;
 };
 
 template <int B>
@@ -1246,7 +1241,7 @@ __device__ __forceinline__ Ph bank_load(const WaveBank<B>* W, int s) {
   p.dx = b.x; p.dy = b.y; p.dz = b.z; p.smax = b.w;
   p.dsafe = c.x; p.stot = c.y; p.tfl = c.z; p.qs = c.w;
   p.hlo = W->hlo[s]; p.hhi = W->hhi[s]; p.c0 = W->c0[s]; p.nstep = W->nstep[s];
-  p.flags = W->flags[s]; p.qlin = W->qlin[s];
+  p.flags = W->flags[s];
   return p;
 }
 
@@ -1256,7 +1251,7 @@ __device__ __forceinline__ void bank_store(WaveBank<B>* W, int s, const Ph& p) {
   W->dm[s] = mdx_v4{p.dx, p.dy, p.dz, p.smax};
   W->aux[s] = mdx_v4{p.dsafe, p.stot, p.tfl, p.qs};
   W->hlo[s] = p.hlo; W->hhi[s] = p.hhi; W->c0[s] = p.c0; W->nstep[s] = p.nstep;
-  W->flags[s] = p.flags; W->qlin[s] = p.qlin;
+  W->flags[s] = p.flags;
 }
 
 // A wave's own LDS writes made visible to its other lanes before they read.
@@ -1269,12 +1264,12 @@ __device__ __forceinline__ void wave_sync_lds() {
 // Total cross section of material slot (or the global majorant, slot 255)
 // at the photon's energy, from its bank cache when valid.
 template <int B>
-__device__ __forceinline__ float bank_sigma(const mdx_tables_dev* T, const mdx_material_dev* lmat, const mdx_curve& lmaj,
+__device__ __forceinline__ float bank_sigma(const mdx_tables_dev* T, const mdx_curve* ltot, const mdx_curve& maj,
                                             WaveBank<B>* W, int s, Ph& p, int slot) {
   const int ce = slot == 255 ? kBankCache - 1 : (slot < kBankCache - 1 ? slot : -1);  // -1: not cached
   const uint32_t valid = fl_get(p.flags, 24, 63u);
   if (ce >= 0 && (valid >> ce & 1u)) return W->sc[ce][s];
-  const float v = curve_eval(T, slot == 255 ? lmaj : lmat[slot].total, p.E);
+  const float v = curve_eval(T, slot == 255 ? maj : ltot[slot], p.E);
   if (ce >= 0) {
     W->sc[ce][s] = v;
     p.flags = fl_set(p.flags, 24, 63u, valid | (1u << ce));
@@ -1293,18 +1288,11 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
   // that hit one record summed first (no LDS pre-reduction: the LDS holds
   // photon banks)
   __shared__ unsigned long long cnt[MDX_NCOUNTER];
-  __shared__ mdx_material_dev lmat[MDX_MAX_MATERIALS];
-  __shared__ mdx_curve lmaj;
+  __shared__ mdx_curve ltot[MDX_MAX_MATERIALS];  // the materials' total cross sections
   __shared__ uint8_t lslot[256];
-  __shared__ BankJob desc[MULTI ? kWaves : 1];
   __shared__ WaveBank<B> banks[kWaves];
-  block_copy(lmat, T->mat, T->n_mat * (int)sizeof(mdx_material_dev));
+  for (int m = threadIdx.x; m < T->n_mat; m += blockDim.x) ltot[m] = T->mat[m].total;
   block_copy(lslot, T->id_to_slot, 256);
-  block_copy(&lmaj, &T->majorant, (int)sizeof(mdx_curve));
-  if (!MULTI) {
-    block_copy(&desc[0].D, jobs[0].D, (int)sizeof(mdx_domain_dev));
-    block_copy(&desc[0].S, jobs[0].S, (int)sizeof(mdx_source_dev));
-  }
   if (threadIdx.x < MDX_NCOUNTER) cnt[threadIdx.x] = 0;
   __syncthreads();
 
@@ -1323,8 +1311,14 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
   uint64_t jseed = 0, jfirst = 0, jn = total;
   uint32_t jstream = 0;
   bool jflights = false;
+  // the current job's descriptors (wave-uniform pointers, read through the
+  // scalar cache)
+  const mdx_domain_dev* jdom = nullptr;
+  const mdx_source_dev* jsrc = nullptr;
   const mdx_tallies_dev* jtl = nullptr;
   auto take_job = [&](int j) {
+    jdom = jobs[j].D;
+    jsrc = jobs[j].S;
     jtl = jobs[j].TL;
     jseed = jobs[j].seed;
     jfirst = jobs[j].first;
@@ -1342,27 +1336,23 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
       if (prefix[mid] <= h) lo = mid; else hi = mid - 1;
     }
     jc = lo;
-    load_bank_job(&desc[wsl], jobs[jc]);
   }
   take_job(jc);
   wave_sync_lds();
   refill_min = refill_min > B ? B : refill_min;
   for (;;) {
-    const BankJob* jd;
-    {
-      int o = 0;
-      asm volatile("" : "+s"(o));
-      jd = MULTI ? &desc[wsl] + o : &desc[0] + o;
-    }
-    const mdx_domain_dev* D = &jd->D;
-    const mdx_source_dev* S = &jd->S;
-    const mdx_tallies_dev* TL = jtl;
+    const mdx_domain_dev* D = wave_const(jdom);
+    const mdx_source_dev* S = wave_const(jsrc);
+    const mdx_tallies_dev* TL = wave_const(jtl);
     const int alive = n_step + n_fly + n_int;
     if (MULTI && exhausted && alive == 0) {  // the wave's job is done here: the next one with histories left
       int jnext = -1;
       for (int k = 0; k < n_jobs; ++k) {
         const int j = (jc + 1 + k) % n_jobs;
-        if (__hip_atomic_load(work + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < jobs[j].n) {
+        // every lane reads the same counter; its first lane's answer is the
+        // wave's (kept wave-uniform, so that the loop stays scalar)
+        const int left = __hip_atomic_load(work + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < jobs[j].n;
+        if (__builtin_amdgcn_readfirstlane(left)) {
           jnext = j;
           break;
         }
@@ -1370,7 +1360,6 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
       if (jnext < 0) {
         done_all = true;
       } else {
-        load_bank_job(&desc[wsl], jobs[jnext]);
         jc = jnext;
         take_job(jc);
         exhausted = false;
@@ -1403,7 +1392,7 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
       // ---- new source photons for the k free slots
       unsigned long long base = 0;
       if (lane == 0) base = atomicAdd(MULTI ? work + jc : work, (unsigned long long)k);
-      base = __shfl(base, 0);
+      base = readlane_u64(base, 0);  // lane 0's claim, wave-uniform
       if (base + (uint64_t)k >= jn) exhausted = true;
       if (act && base + (uint64_t)lane < jn) {
         const uint64_t h = jfirst + base + (uint64_t)lane;
@@ -1417,13 +1406,12 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
         p.c0 = rng.c0;
         p.nstep = 0;
         p.flags = fl_set(fl_set(0u, 4, 255u, 255u), 12, 255u, 255u);
-        p.smax = curve_eval(T, lmaj, p.E);
+        p.smax = curve_eval(T, wave_const(T)->majorant, p.E);
         W->sc[kBankCache - 1][slot] = p.smax;
         p.flags = fl_set(p.flags, 24, 63u, 1u << (kBankCache - 1));
         p.dsafe = INFINITY;
         p.stot = 0.0f;
         p.qs = 0.0f;
-        p.qlin = 0;
         p.tfl = next_tally_event(TL, p.px, p.py, p.pz, p.dx, p.dy, p.dz);
         uint64_t lin0;
         const bool fly = jflights && in_domain(D, p.px, p.py, p.pz) && voxel_lookup(D, p.px, p.py, p.pz, lin0) < 0;
@@ -1475,13 +1463,12 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
             if (sl != cslot) {
               // a local majorant of class k is material k's own total cross
               // section at this energy: the same value, no second lookup
-              p.stot = sl == (int)fl_get(p.flags, 4, 255u) ? p.smax : bank_sigma(T, lmat, lmaj, W, slot, p, sl);
+              p.stot = sl == (int)fl_get(p.flags, 4, 255u) ? p.smax : bank_sigma(T, ltot, wave_const(T)->majorant, W, slot, p, sl);
               cslot = sl;
             }
             if (!limited && real_collision(u2, delta_threshold(p.stot, p.smax))) {
               next = kEvInt;
               p.qs = s;
-              p.qlin = (uint32_t)lin;
               p.flags = fl_set(p.flags, 20, 15u, (uint32_t)(g + 1));
             } else {
               p.tfl -= s;
@@ -1495,7 +1482,7 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
             if (nr == 0u || p.E < 1.0e3f) ncls = 255;
             if (ncls != (int)fl_get(p.flags, 4, 255u)) {
               p.flags = fl_set(p.flags, 4, 255u, (uint32_t)ncls);
-              p.smax = ncls == cslot ? p.stot : bank_sigma(T, lmat, lmaj, W, slot, p, ncls);
+              p.smax = ncls == cslot ? p.stot : bank_sigma(T, ltot, wave_const(T)->majorant, W, slot, p, ncls);
             }
             p.dsafe = ncls == 255 ? INFINITY : (float)nr * D->grids[0].mcell;
           }
@@ -1504,7 +1491,7 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
           // cut at the next grid face or the domain's boundary
           const int sl = D->bg_slot;
           if (sl != cslot) {
-            p.stot = bank_sigma(T, lmat, lmaj, W, slot, p, sl);
+            p.stot = bank_sigma(T, ltot, wave_const(T)->majorant, W, slot, p, sl);
             cslot = sl;
           }
           const float si = -__log2f(u1) * (0.69314718055994531f * frcp(p.stot));
@@ -1516,7 +1503,6 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
             p.pz = z0 + si * p.dz;
             next = kEvInt;
             p.qs = si;
-            p.qlin = 0;
             p.flags = fl_set(p.flags, 20, 15u, 0u);
           } else if (dgrid <= dexit) {  // up to the grid's face; majorant steps from there
             p.px = x0 + dgrid * p.dx;
@@ -1552,7 +1538,7 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
         Ph p = bank_load(W, slot);
         ++c_real;
         const int cslot = (int)fl_get(p.flags, 12, 255u);
-        const mdx_material_dev& M = lmat[cslot];
+        const mdx_material_dev& M = T->mat[cslot];
         int nc = (int)fl_get(p.flags, 0, 3u), ni = (int)fl_get(p.flags, 2, 3u);
         Step r{p.px - p.qs * p.dx, p.py - p.qs * p.dy, p.pz - p.qs * p.dz, p.dx, p.dy, p.dz, p.qs, p.E,
                nc, ni, 0, 0, 0.0f, true};
@@ -1585,7 +1571,7 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
             p.dsafe = INFINITY;
           }
           p.flags = fl_set(fl_set(p.flags, 4, 255u, (uint32_t)mcls), 24, 63u, 0u);  // a new energy: cache invalid
-          p.smax = bank_sigma(T, lmat, lmaj, W, slot, p, mcls);
+          p.smax = bank_sigma(T, ltot, wave_const(T)->majorant, W, slot, p, mcls);
           p.flags = fl_set(p.flags, 12, 255u, 255u);
         } else {
           ++c_photo;
@@ -1601,7 +1587,10 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
         if (dep != 0.0f) {
           ++c_dep;
           if (qg >= 0) {
-            voxel_deposit(D->grids[qg], p.qlin, dep);
+            // the voxel of the interaction point (the step that parked it
+            // found the same one: the position is unchanged since)
+            const mdx_grid_dev& G = D->grids[qg];
+            voxel_deposit(G, grid_voxel(G, p.px - G.origin[0], p.py - G.origin[1], p.pz - G.origin[2]), dep);
           } else if (D->bg_dose) {
             gadd(D->bg_dose, (double)dep);
             gadd(D->bg_dose + 1, (double)dep * (double)dep);
@@ -1648,10 +1637,10 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
   if (threadIdx.x < MDX_NCOUNTER && cnt[threadIdx.x]) atomicAdd(counters + threadIdx.x, cnt[threadIdx.x]);
 }
 
-// Waves per SIMD the registers must allow: the single-job kernel with
-// 96-photon banks fits four blocks per CU in LDS, the others three.
+// Blocks of four waves with banks of 96 or 128 photons each fit four to a
+// CU's LDS; the registers are held to four waves per SIMD to match.
 template <bool MULTI, int B>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(!MULTI && B <= 96 ? 4 : 3)))
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
 k_transport_bank(const mdx_tables_dev* __restrict__ T, const mdx_job_dev* __restrict__ jobs,
                  const uint64_t* __restrict__ prefix, int n_jobs, uint64_t total, unsigned long long* __restrict__ work,
                  unsigned long long* __restrict__ counters, int refill_min, int interact_min) {
@@ -1660,7 +1649,7 @@ k_transport_bank(const mdx_tables_dev* __restrict__ T, const mdx_job_dev* __rest
 
 // Graph form (round buffer as for k_transport_queue_round).
 template <int B>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
 k_transport_bank_round(const mdx_tables_dev* __restrict__ T, const char* __restrict__ round,
                        unsigned long long* __restrict__ work, unsigned long long* __restrict__ counters,
                        int refill_min, int interact_min) {
@@ -2080,9 +2069,11 @@ static hipError_t launch_bank(int bank, unsigned blocks, const mdx_tables_dev* T
   if (bank == 96)
     hipLaunchKernelGGL((k_transport_bank<MULTI, 96>), dim3(blocks), dim3(kBlock), 0, st, T, jobs, prefix, n_jobs,
                        total, work, counters, refill_min, interact_min);
-  else
+  else if (bank == 128)
     hipLaunchKernelGGL((k_transport_bank<MULTI, 128>), dim3(blocks), dim3(kBlock), 0, st, T, jobs, prefix, n_jobs,
                        total, work, counters, refill_min, interact_min);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
@@ -2106,9 +2097,11 @@ hipError_t transport_bank_round(const mdx_tables_dev* T, const char* round, unsi
   if (bank == 96)
     hipLaunchKernelGGL(k_transport_bank_round<96>, dim3((unsigned)grid_blocks), dim3(kBlock), 0, st, T, round, work,
                        counters, refill_min, interact_min);
-  else
+  else if (bank == 128)
     hipLaunchKernelGGL(k_transport_bank_round<128>, dim3((unsigned)grid_blocks), dim3(kBlock), 0, st, T, round, work,
                        counters, refill_min, interact_min);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
