@@ -423,6 +423,10 @@ int mdx_sample_source(mdx_ctx* ctx, int n, uint64_t seed, float* dir, float* ene
  * key[2 i], key[2 i + 1]; words 2, Philox-2x32-10 of counter[2 i ..] under
  * key[i]; out holds n * words words. */
 int mdx_philox(mdx_ctx* ctx, int n, int words, const uint32_t* counter, const uint32_t* key, uint32_t* out);
+/* Particle::rotate (particle.cpp:7-45) as the transport kernels apply it: the
+ * direction dir[3 i ..] (unit) turned by polar cosine mu[i] and azimuth
+ * phi[i] in [0, 2 pi) into out[3 i ..], on the device. */
+int mdx_rotate(mdx_ctx* ctx, int n, const float* dir, const float* mu, const float* phi, float* out);
 
 #ifdef __cplusplus
 }

@@ -49,6 +49,7 @@ EXPORTS = [
     "mdx_read_staged_dose_f32_async", "mdx_read_done", "mdx_copy_probe", "mdx_air_muen", "mdx_set_air_muen_table",
     "mdx_set_local_majorants", "mdx_get_macro_cells", "mdx_delta_decision", "mdx_philox",
     "mdx_sample_source", "mdx_run_jobs_noalloc", "mdx_roi_stats", "mdx_set_event_kernel", "mdx_get_event_kernel",
+    "mdx_rotate",
 ]
 
 
@@ -238,6 +239,16 @@ class Context:
         out = np.empty(a.size, dtype=np.int32)
         self._call(self.lib.mdx_delta_decision, self._h, ctypes.c_int(a.size), _ptr(a, ctypes.c_float),
                    _ptr(b, ctypes.c_float), _ptr(d, ctypes.c_float), _ptr(lim, ctypes.c_int), _ptr(out, ctypes.c_int))
+        return out
+
+    def rotate(self, dirs, mu, phi):
+        """The device's direction change (particle.cpp:7-45): unit dirs (n, 3)
+        turned by polar cosine mu and azimuth phi."""
+        d = _f32(dirs).reshape(-1, 3)
+        m, p = _f32(mu), _f32(phi)
+        out = np.empty_like(d)
+        self._call(self.lib.mdx_rotate, self._h, ctypes.c_int(m.size), _ptr(d, ctypes.c_float),
+                   _ptr(m, ctypes.c_float), _ptr(p, ctypes.c_float), _ptr(out, ctypes.c_float))
         return out
 
     def sample_source(self, n, seed=1):

@@ -265,7 +265,7 @@ struct mdx_ctx {
   std::vector<std::unique_ptr<mdx::MaterialTables>> mats;
   mdx::Table majorant;
   mdx::Packed packed;
-  DBuf d_xs, d_nodes, d_hash, d_rE, d_rinv, d_roff, d_rrows, d_rhash, d_rguide, d_tables;
+  DBuf d_xs, d_nodes, d_bins, d_rE, d_rinv, d_roff, d_rrows, d_rhash, d_rguide, d_tables;
   bool tables_ready = false;
   int tables_gen = 0;  // bumped by every table build
   // per slot k: the slots whose total cross section k's bounds from above on
@@ -773,7 +773,7 @@ void build_and_upload(mdx_ctx* c, const std::vector<int>& ids) {
   mdx::Packed& P = c->packed;
   c->d_xs.upload(P.xs.data(), P.xs.size(), c->stream);
   c->d_nodes.upload(P.nodes.data(), P.nodes.size(), c->stream);
-  c->d_hash.upload(P.hash.data(), P.hash.size(), c->stream);
+  c->d_bins.upload(P.bins.data(), P.bins.size(), c->stream);
   c->d_rE.upload(P.rita_E.data(), P.rita_E.size(), c->stream);
   c->d_rinv.upload(P.rita_escale.data(), P.rita_escale.size(), c->stream);
   c->d_roff.upload(P.rita_row_off.data(), P.rita_row_off.size(), c->stream);
@@ -783,7 +783,7 @@ void build_and_upload(mdx_ctx* c, const std::vector<int>& ids) {
   mdx_tables_dev T{};
   T.xs = c->d_xs.as<float>();
   T.nodes = c->d_nodes.as<mdx_node>();
-  T.hash = c->d_hash.as<int32_t>();
+  T.bins = c->d_bins.as<mdx_node>();
   T.rita_E = c->d_rE.as<float>();
   T.rita_escale = c->d_rinv.as<float>();
   T.rita_row_off = c->d_roff.as<int32_t>();
@@ -1864,7 +1864,7 @@ int mdx_job_device_bytes(mdx_ctx* c, int job, uint64_t* bytes) {
 int mdx_tables_device_bytes(mdx_ctx* c, uint64_t* bytes) {
   return guard([&] {
     need(c && bytes, "NULL argument");
-    *bytes = c->d_xs.bytes + c->d_nodes.bytes + c->d_hash.bytes + c->d_rE.bytes + c->d_rinv.bytes + c->d_roff.bytes +
+    *bytes = c->d_xs.bytes + c->d_nodes.bytes + c->d_bins.bytes + c->d_rE.bytes + c->d_rinv.bytes + c->d_roff.bytes +
              c->d_rrows.bytes + c->d_tables.bytes;
   });
 }
@@ -2381,6 +2381,19 @@ int mdx_delta_decision(mdx_ctx* c, int n, const float* s_tot, const float* s_max
     hip_check(mdx_launch::delta(n, a.p(), b.p(), d.p(), l.p(), o.p(), c->stream), "delta");
     hip_check(hipMemcpyAsync(real, o.p(), n * sizeof(int), hipMemcpyDeviceToHost, c->stream), "read");
     hip_check(hipStreamSynchronize(c->stream), "delta");
+  });
+}
+
+int mdx_rotate(mdx_ctx* c, int n, const float* dir, const float* mu, const float* phi, float* out) {
+  return guard([&] {
+    need(c && dir && mu && phi && out && n >= 0, "bad argument");
+    if (n == 0) return;
+    set_device(c);
+    DevArray<float> d(dir, 3 * (size_t)n, c->stream), m(mu, n, c->stream), p(phi, n, c->stream);
+    DevArray<float> o(nullptr, 3 * (size_t)n, c->stream);
+    hip_check(mdx_launch::rotate(n, d.p(), m.p(), p.p(), o.p(), c->stream), "rotate");
+    hip_check(hipMemcpyAsync(out, o.p(), 3 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->stream), "read");
+    hip_check(hipStreamSynchronize(c->stream), "rotate");
   });
 }
 

@@ -192,12 +192,12 @@ __device__ __forceinline__ float log_ratio(float X, float xk) {
   return logf(r) + e * frcp(X);
 }
 
-// A curve lookup is two dependent loads: the hash bin of ln X gives the
-// largest node index k with x_k at or below the bin's lower edge, clamped to
-// [0, n-2], and the node pair (k, k+1) then brackets X unless the bin holds
-// further nodes, in which case a short walk over the abscissae follows.  The
-// result is the largest k in [0, n-2] with x_k <= X (0 below x_0), the same
-// index as upper_bound - 1 clamped to the last span.
+// A curve lookup reads one bin record: the hash bin of ln X holds the record
+// of the largest node index k with x_k at or below the bin's lower edge,
+// clamped to [0, n-2], together with x_{k+1} and k.  That node brackets X
+// unless the bin holds further nodes, in which case a short walk over the
+// abscissae follows.  The result is the largest k in [0, n-2] with x_k <= X
+// (0 below x_0), the same index as upper_bound - 1 clamped to the last span.
 struct CurveAt {
   int k;
   mdx_node r;  // node k
@@ -206,12 +206,19 @@ struct CurveAt {
 
 // The bin comes from the hardware log2: a point within its error of a bin
 // edge may land in the neighbouring bin, whose node curve_fix then corrects.
-__device__ __forceinline__ int curve_hash(const mdx_tables_dev* T, const mdx_curve& c, float X) {
+__device__ __forceinline__ CurveAt curve_bin(const mdx_tables_dev* T, const mdx_curve& c, float X) {
   constexpr float kLn2 = 0.69314718055994531f;
   int b = (int)((__log2f(X) * kLn2 - c.lnx0) * c.inv_dlnx);
   b = b < 0 ? 0 : (b >= c.nh ? c.nh - 1 : b);
-  const int k = gld(T->hash + c.hoff + b);
-  return k > c.n - 2 ? c.n - 2 : k;
+  const MDX_GLOBAL float* p = (const MDX_GLOBAL float*)(T->bins + c.hoff + b);
+  const mdx_v4 a = *(const MDX_GLOBAL mdx_v4*)p;
+  const mdx_v4 q = *(const MDX_GLOBAL mdx_v4*)(p + 4);
+  CurveAt r;
+  r.k = __float_as_int(q.w);
+  r.r.a_hi = a.x; r.r.a_lo = a.y; r.r.c1 = a.z; r.r.c2 = a.w;
+  r.r.c3 = q.x; r.r.x = q.y; r.r.pad0 = 0.0f; r.r.pad1 = 0.0f;
+  r.xn = q.z;
+  return r;
 }
 
 __device__ __forceinline__ CurveAt curve_nodes(const mdx_tables_dev* T, const mdx_curve& c, int k) {
@@ -252,7 +259,7 @@ __device__ __forceinline__ float curve_finish(const mdx_tables_dev* T, const mdx
 
 // Evaluates a packed curve (see mdx_types.h).
 __device__ __forceinline__ float curve_eval(const mdx_tables_dev* T, const mdx_curve& c, float X) {
-  CurveAt a = curve_nodes(T, c, curve_hash(T, c, X));
+  CurveAt a = curve_bin(T, c, X);
   curve_fix(T, c, X, a);
   return curve_finish(T, c, X, a);
 }
@@ -260,8 +267,7 @@ __device__ __forceinline__ float curve_eval(const mdx_tables_dev* T, const mdx_c
 // Two curves at once, their loads issued side by side.
 __device__ __forceinline__ void curve_eval2(const mdx_tables_dev* T, const mdx_curve& c1, float X1,
                                             const mdx_curve& c2, float X2, float& v1, float& v2) {
-  const int k1 = curve_hash(T, c1, X1), k2 = curve_hash(T, c2, X2);
-  CurveAt a1 = curve_nodes(T, c1, k1), a2 = curve_nodes(T, c2, k2);
+  CurveAt a1 = curve_bin(T, c1, X1), a2 = curve_bin(T, c2, X2);
   curve_fix(T, c1, X1, a1);
   curve_fix(T, c2, X2, a2);
   v1 = curve_finish(T, c1, X1, a1);
@@ -273,8 +279,7 @@ __device__ __forceinline__ void curve_eval2(const mdx_tables_dev* T, const mdx_c
 __device__ __forceinline__ void curve_eval3(const mdx_tables_dev* T, const mdx_curve& c1, float X1,
                                             const mdx_curve& c2, float X2, const mdx_curve& c3, float X3,
                                             float& v1, float& v2, float& v3) {
-  const int k1 = curve_hash(T, c1, X1), k2 = curve_hash(T, c2, X2), k3 = curve_hash(T, c3, X3);
-  CurveAt a1 = curve_nodes(T, c1, k1), a2 = curve_nodes(T, c2, k2), a3 = curve_nodes(T, c3, k3);
+  CurveAt a1 = curve_bin(T, c1, X1), a2 = curve_bin(T, c2, X2), a3 = curve_bin(T, c3, X3);
   curve_fix(T, c1, X1, a1);
   curve_fix(T, c2, X2, a2);
   curve_fix(T, c3, X3, a3);

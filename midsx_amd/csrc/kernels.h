@@ -54,6 +54,7 @@ hipError_t select(int n, const float* sc, const float* si, const float* stt, con
                   hipStream_t st);
 hipError_t delta(int n, const float* stt, const float* sm, const float* u, const int* limited, int* real,
                  hipStream_t st);
+hipError_t rotate(int n, const float* dir, const float* mu, const float* phi, float* out, hipStream_t st);
 // Directions (3 per photon) and energies of the source photons of histories
 // [0, n) under seed, as the transport kernels generate them.
 hipError_t source(const mdx_source_dev* S, int n, uint64_t seed, float* dir, float* E, hipStream_t st);

@@ -35,7 +35,7 @@
 //         spans (interpolators.cpp:3-31).
 struct alignas(32) mdx_node {
   float a_hi, a_lo, c1, c2;
-  float c3, x, pad0, pad1;
+  float c3, x, pad0, pad1;  // in a bin record: pad0 = x_{k+1}, pad1 = k (int32 bits)
 };
 
 struct mdx_curve {
@@ -85,7 +85,7 @@ struct alignas(32) mdx_rita_row {
 struct mdx_tables_dev {
   const float* xs;          // node abscissae (all curves)
   const mdx_node* nodes;    // node records (all curves)
-  const int32_t* hash;      // bracket hash (all curves)
+  const mdx_node* bins;     // bracket hash (all curves): per bin, the record of its node k (see build_hash)
   const float* rita_E;      // RITA energies (all materials), rounded down to FP32
   const float* rita_escale; // per energy i: 1 / ln(E_{i+1}/E_i) and ln(rita_E[i] / E_i)
   const int32_t* rita_row_off;  // per energy: first row; one extra entry per material

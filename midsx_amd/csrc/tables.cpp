@@ -520,12 +520,19 @@ static void build_hash(Packed& p, mdx_curve& c, double lo, double hi) {
   c.nh = nh;
   c.lnx0 = (float)std::log(lo);
   c.inv_dlnx = (float)(nh / (std::log(hi) - std::log(lo)));
-  c.hoff = (int)p.hash.size();
+  c.hoff = (int)p.bins.size();
   int k = 0;
   for (int b = 0; b < nh; ++b) {
     double xb = std::exp(std::log(lo) + b / (double)c.inv_dlnx);
     while (k + 1 < c.n && xs[k + 1] <= xb) ++k;
-    p.hash.push_back(k);
+    // the bin's record: node k (clamped to the last span) with the next
+    // node's abscissa and k itself in its padding, so that a lookup whose
+    // X lies in [x_k, x_{k+1}) reads one record
+    const int kc = k > c.n - 2 ? c.n - 2 : k;
+    mdx_node r = p.nodes[c.off + kc];
+    r.pad0 = xs[kc + 1];
+    std::memcpy(&r.pad1, &kc, sizeof(float));
+    p.bins.push_back(r);
   }
 }
 

@@ -89,7 +89,7 @@ std::vector<double> merge_grids(const std::vector<const std::vector<double>*>& g
 struct Packed {
   std::vector<float> xs;
   std::vector<mdx_node> nodes;
-  std::vector<int32_t> hash;
+  std::vector<mdx_node> bins;  // per hash bin: node k of the bin with x_{k+1} and k (see build_hash)
   std::vector<float> rita_E, rita_escale;
   std::vector<int32_t> rita_row_off;
   std::vector<mdx_rita_row> rita_rows;

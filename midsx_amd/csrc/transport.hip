@@ -1852,6 +1852,17 @@ __global__ void k_select(int n, const float* sc, const float* si, const float* s
   type[i] = select_type(sc[i], si[i], st[i], u[i]);
 }
 
+// The direction change every scattering event of the transport kernels makes.
+__global__ void k_rotate(int n, const float* dir, const float* mu, const float* phi, float* out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float dx = dir[3 * i], dy = dir[3 * i + 1], dz = dir[3 * i + 2];
+  rotate(dx, dy, dz, mu[i], phi[i]);
+  out[3 * i] = dx;
+  out[3 * i + 1] = dy;
+  out[3 * i + 2] = dz;
+}
+
 // The delta-scatter decision of both transport kernels (limited: the step
 // was cut where a local majorant's bound ends, which takes no collision
 // test) and the raw Philox blocks, for known-answer tests.
@@ -2142,6 +2153,11 @@ hipError_t voxel(const mdx_tables_dev* T, const mdx_domain_dev* D, int n, const 
 hipError_t select(int n, const float* sc, const float* si, const float* stt, const float* u, int* type,
                   hipStream_t st) {
   hipLaunchKernelGGL(k_select, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, sc, si, stt, u, type);
+  return hipGetLastError();
+}
+
+hipError_t rotate(int n, const float* dir, const float* mu, const float* phi, float* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_rotate, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, dir, mu, phi, out);
   return hipGetLastError();
 }
 

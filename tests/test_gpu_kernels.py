@@ -8,6 +8,8 @@
 * RITA inverse-CDF sampling: within 2 FP32 units (at max(|mu|, 1 + mu)) of
   the FP64 sampler on the same uniforms, every sample;
 * the delta-scatter decision: bit-exact against IEEE FP32;
+* the direction change of a scattering event: against the reference's FP64
+  quaternion rotation within the hardware sine / cosine error;
 * Kahn + S(x) Compton sampling: chi^2 against Klein-Nishina x S(x).
 """
 import numpy as np
@@ -252,6 +254,36 @@ def test_delta_decision_bit_exact(ctx):
     real64 = ~(u[rnd].astype(np.float64) < thr64[rnd])
     differ = real64 != got[rnd].astype(bool)
     assert np.all(np.abs(u[rnd][differ] - thr64[rnd][differ]) <= 2 * np.spacing(np.float32(1.0)))
+
+
+def test_rotation_matches_quaternion_form(ctx):
+    """particle.cpp:7-45 on the device (the closed form of the reference's
+    quaternion product, hardware sine and cosine for the azimuth) against the
+    oracle's FP64 quaternion rotation of the same FP32 inputs: random
+    directions, axis-aligned ones and ties between |d_i| (the rotation axis
+    comes from the smallest component, x before y before z), polar cosines
+    including +-1.  Every component within 4e-6 (the hardware sine and cosine
+    are accurate to a few 1e-7 over [0, 2 pi)); |d'| = 1 and d.d' = mu."""
+    rng = np.random.default_rng(17)
+    d = rng.normal(size=(3000, 3))
+    special = [(1, 0, 0), (0, 1, 0), (0, 0, 1), (-1, 0, 0), (0, 0, -1), (1, 1, 1), (1, 1, 0), (0, 1, 1),
+               (1, 0, 1), (-1, 1, 1), (1, -1, 2), (2, 1, 1), (1, 2, 1), (1, 1, 2), (3, -3, 1)]
+    d[:len(special)] = special
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    d = d.astype(np.float32)
+    n = len(d)
+    mu = rng.uniform(-1, 1, n).astype(np.float32)
+    mu[::7] = 1.0
+    mu[3::7] = -1.0
+    phi = rng.uniform(0, 2 * np.pi, n).astype(np.float32)
+    phi[5::11] = 0.0
+    got = ctx.rotate(d, mu, phi).astype(np.float64)
+    ref = np.array([oracle.rotate(d[i].astype(np.float64), float(np.arccos(np.float64(mu[i]))), float(phi[i]))
+                    for i in range(n)])
+    err = np.abs(got - ref).max(axis=1)
+    assert err.max() < 4e-6, (err.max(), np.argmax(err))
+    assert np.abs(np.linalg.norm(got, axis=1) - 1).max() < 1e-6
+    assert np.abs(np.sum(got * d.astype(np.float64), axis=1) - mu).max() < 4e-6
 
 
 def kn_s(m, E, mu):
