@@ -187,7 +187,7 @@ def test_budget_holds_with_staged_reads_in_flight():
 
 def test_round_plan_shares_equally():
     ctx = FakeCtx()
-    srv = Server(ctx, budget_bytes=10 ** 9, round_histories=1000, min_slice=10, on_result=None)
+    srv = Server(ctx, budget_bytes=10 ** 9, round_histories=1000, min_slice=10, on_result=None, size_weight=0.0)
 
     class A:
         def __init__(self, h, done):
@@ -199,6 +199,25 @@ def test_round_plan_shares_equally():
     assert take[1] == 50 and take[2] == 10
     assert take[0] == take[3] and take.sum() >= 1000
     assert take.sum() <= 1000 + 4 * 10
+
+
+def test_round_plan_weighted_by_phantom_size():
+    """size_weight 1: shares in proportion to the phantom's voxels (a job
+    that needs fewer passes the rest on; at least min_slice each, which
+    blurs the proportion by a few slices)."""
+    ctx = FakeCtx()
+    srv = Server(ctx, budget_bytes=10 ** 9, round_histories=9000, min_slice=10, on_result=None, size_weight=1.0)
+
+    class A:
+        def __init__(self, h, done, edge):
+            self.job = {"histories": h, "shape": (edge, edge, edge)}
+            self.done = done
+            self.need = h
+
+    take = srv._plan([A(10 ** 6, 0, 16), A(10 ** 6, 0, 32), A(10 ** 6, 0, 16), A(100, 0, 64)])
+    assert take[3] == 100
+    assert abs(take[1] / take[0] - 8) < 0.2 and take[0] == take[2] and take[0] >= 10
+    assert 9000 <= take.sum() <= 9000 + 4 * 10
 
 
 def test_precision_jobs_stop_at_their_target():
