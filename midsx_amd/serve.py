@@ -134,7 +134,7 @@ class Server:
     makespan by 6 % at the same p50 / p99 latency)."""
 
     def __init__(self, ctx, budget_bytes=None, round_histories=30_000_000, min_slice=100_000, max_active=512,
-                 graphs=True, phantoms=None, on_result=None, dose="f32", admit_slice=0.01,
+                 graphs=True, phantoms=None, on_result=None, dose="f32", admit_slice=0.005,
                  admit_voxels_per_s=1.0e10, first_slice=1_000_000, size_weight=0.5):
         self.ctx = ctx
         if budget_bytes is None:
