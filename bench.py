@@ -395,7 +395,7 @@ def radiography(ctx, args):
     return out
 
 
-def ct_scan(ctx, n_views, scale, plain_every, seed=4):
+def ct_scan(ctx, n_views, scale, plain_every, seed=4, params=None):
     """CT: 512 x 512 x 256 phantom at 0.8 mm (water, lung, adipose, bone),
     n_views views about its z axis, 120 kVp, a 512 x 64 detector per view,
     per-view histories U[2e6, 2e7] x scale.  All views in one multi-job launch
@@ -410,9 +410,14 @@ def ct_scan(ctx, n_views, scale, plain_every, seed=4):
     seeds = [10_000 + v for v in range(n_views)]
     H = sum(n)
     ctx.job_select(views[0])
-    # one view, probed as the multi-job launch the views run in: launches of
-    # ~15 ms time stably
-    tuned = ctx.autotune(probe_histories=4000000, rounds=True)
+    if params:  # fixed: event kernel, blocks per CU, refill, interact
+        ev, b, r, i = params.split(",")
+        set_event_kernel(ctx, ev)
+        ctx.set_queue_params(int(b), int(r), int(i))
+        tuned = dict(blocks_per_cu=int(b), refill_min=int(r), interact_min=int(i), event_kernel=ev, fixed=True)
+    else:
+        # one view, probed as the multi-job launch the views run in
+        tuned = ctx.autotune(probe_histories=4000000, rounds=True)
     ctx.set_graphs(True)
     ctx.run_jobs(views[:8], [100000] * 8, [1 << 40] * 8, seeds[:8])  # warm (capture)
     for j in [owner] + views:
@@ -537,6 +542,8 @@ def main():
     ap.add_argument("--ct-scale", type=float, default=1.0,
                     help="fraction of the CT workload's per-view histories (1.0: the full ~4e9)")
     ap.add_argument("--ct-plain-every", type=int, default=10)
+    ap.add_argument("--ct-params", default=None,
+                    help="event_kernel,blocks_per_cu,refill_min,interact_min of the CT launch (skips its autotuner)")
     ap.add_argument("--copy-probe-mb", type=int, default=0)
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--precision-histories", type=int, default=10 ** 9)
@@ -580,7 +587,7 @@ def main():
     if not args.no_radiography:
         out["radiography"] = radiography(ctx, args)
     if not args.no_ct:
-        out["ct"] = ct_scan(ctx, args.ct_views, args.ct_scale, args.ct_plain_every)
+        out["ct"] = ct_scan(ctx, args.ct_views, args.ct_scale, args.ct_plain_every, params=args.ct_params)
     if not args.no_precision:
         out["precision"] = precision_study(ctx, args.precision_histories)
     if args.copy_probe_mb:
