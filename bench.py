@@ -240,7 +240,7 @@ def served_stream(ctx, n_jobs, seed, load, cap, steps, warmup, plain=True, param
         tuned = dict(blocks_per_cu=int(b), refill_min=int(r), interact_min=int(i), event_kernel=ev, fixed=True)
     else:
         # the rounds' multi-job kernels, probed as launches of ~1.5e7
-        # histories (a round runs 3e7)
+        # histories (a round runs 4.5e7)
         tuned = ctx.autotune(probe_histories=15_000_000, rounds=True)
     ctx.job_release(pj)
     ctx.job_select(keep_job)

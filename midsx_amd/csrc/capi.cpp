@@ -128,6 +128,7 @@ struct DBuf {
 // (a word of one repeated byte sets one bit); threads take the layers of
 // cells along z.  Also the IDs present anywhere.
 void scan_cells(const uint8_t* ids, const int dims[3], std::vector<uint64_t>& cells, int mc[3], bool* present) {
+  static_assert(MDX_MACRO_EDGE == 8, "a cell's row of voxels is read as one 8-byte word");
   for (int a = 0; a < 3; ++a) mc[a] = (dims[a] + MDX_MACRO_EDGE - 1) >> MDX_MACRO_SHIFT;
   const size_t nc = (size_t)mc[0] * mc[1] * mc[2];
   cells.assign(nc * 4, 0);

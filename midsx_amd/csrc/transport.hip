@@ -964,7 +964,9 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
         thr = delta_threshold(stot, smax);
         cslot = slot;
       }
-      bool replay = escaped;  // replay this step's segment for the tallies
+      // replay this step's segment for the tallies: an escaping step only
+      // when it reaches the next tally event (a shorter one scores nothing)
+      bool replay = escaped && tfl - s <= 0.0f;
       float seg = s;
       if (escaped) {
         ++c_esc;
@@ -998,7 +1000,7 @@ __device__ __forceinline__ void transport_queue_body(const mdx_tables_dev* __res
           py = y0 + seg * dy;
           pz = z0 + seg * dz;
           ++c_esc;
-          replay = true;
+          replay = tfl - seg <= 0.0f;  // an escaping step short of the next tally event scores nothing
           alive = false;
         }
       } else if (!limited && real_collision(u2, thr)) {  // real interaction: park it for the interaction phase
@@ -1448,7 +1450,7 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
           escaped = !in_domain(D, p.px, p.py, p.pz);
           if (escaped) {
             ++c_esc;
-            replay = true;
+            replay = p.tfl - s <= 0.0f;  // an escaping step short of the next tally event scores nothing
             alive_ = false;
           } else {
             uint32_t cell = 0;
@@ -1519,7 +1521,7 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
             p.py = y0 + seg * p.dy;
             p.pz = z0 + seg * p.dz;
             ++c_esc;
-            replay = true;
+            replay = p.tfl - seg <= 0.0f;
             alive_ = false;
           }
         }
@@ -1552,17 +1554,17 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
         const int type = select_type(sc, si, p.stot, rng.uniform());
         float dep = 0.0f;
         bool alive_ = true;
+        // both scatters draw the azimuth after their polar sampler, so one
+        // rotation below serves both (one copy of it in the divergent code)
+        float mu = 1.0f;
         if (type == 1) {
           ++c_ray;
           nc = nc < 3 ? nc + 1 : 3;
-          const float mu = rita_sample(T, M, p.E, rng.uniform());
-          rotate(p.dx, p.dy, p.dz, mu, 2.0f * kPiF * rng.uniform());
+          mu = rita_sample(T, M, p.E, rng.uniform());
         } else if (type == 2) {
           ++c_comp;
           ni = ni < 3 ? ni + 1 : 3;
-          float mu;
           const float Ep = compton(T, M, rng, p.E, mu, c_trial, c_stest, smx);
-          rotate(p.dx, p.dy, p.dz, mu, 2.0f * kPiF * rng.uniform());
           dep = p.E - Ep;
           p.E = Ep;
           int mcls = (int)fl_get(p.flags, 4, 255u);
@@ -1578,6 +1580,7 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
           dep = p.E;
           alive_ = false;
         }
+        if (alive_) rotate(p.dx, p.dy, p.dz, mu, 2.0f * kPiF * rng.uniform());
         p.c0 = rng.c0;
         p.flags = fl_set(fl_set(p.flags, 0, 3u, (uint32_t)nc), 2, 3u, (uint32_t)ni);
         r.ncf = nc;

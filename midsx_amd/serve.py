@@ -118,7 +118,11 @@ class Server:
     budget_bytes: device memory the resident jobs may hold (default: half of
     the free memory at construction).  round_histories: histories per round,
     shared by the admitted jobs in proportion to (phantom voxels)^size_weight
-    (at least min_slice each, never more than a job has left).  phantoms: callable job -> uint8 grid (default:
+    (at least min_slice each, never more than a job has left); longer rounds
+    spend less of the stream in kernel tails, launches and checks between
+    rounds, shorter ones stop precision jobs closer to their target (4.5e7
+    served a 32-job stream at 1 % ROI error in 4 % less time than 3e7 and
+    in less than 2e7 or 6e7).  phantoms: callable job -> uint8 grid (default:
     `scenes.dose_phantom`), so that a client can hand over its own data.
     dose: format of the dose read back ("f32" or "f64", see `read_dose`);
     on_result(job, a, b, image) receives it (the buffers are reused).
@@ -133,7 +137,7 @@ class Server:
     trailing the last one (0: equal shares; 0.5 cut a 32-job stream's
     makespan by 6 % at the same p50 / p99 latency)."""
 
-    def __init__(self, ctx, budget_bytes=None, round_histories=30_000_000, min_slice=100_000, max_active=512,
+    def __init__(self, ctx, budget_bytes=None, round_histories=45_000_000, min_slice=100_000, max_active=512,
                  graphs=True, phantoms=None, on_result=None, dose="f32", admit_slice=0.005,
                  admit_voxels_per_s=1.0e10, first_slice=1_000_000, size_weight=0.5):
         self.ctx = ctx
