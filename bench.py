@@ -249,8 +249,13 @@ def served_stream(ctx, n_jobs, seed, load, cap, steps, warmup, plain=True, param
     # untimed passes first: the device memory pool and the pinned result
     # buffers are then warm, as in a service that has been running
     progress("served stream: tuned %s; %d warm-up passes" % (tuned, warmup))
+    # every pass's rounds and histories in launch order (warm-up, timed,
+    # latency pass), so that profiles can attribute their dispatches
+    pass_rounds = []
     for _ in range(warmup):
-        Server(ctx, phantoms=ph).run(jobs)
+        srv = Server(ctx, phantoms=ph)
+        res = srv.run(jobs)
+        pass_rounds.append(dict(kind="warmup", rounds=srv.rounds, histories=sum(r.histories for r in res)))
     passes = []  # (server, results, seconds) of each timed pass
     for i in range(steps):
         srv = Server(ctx, phantoms=ph)
@@ -277,7 +282,12 @@ def served_stream(ctx, n_jobs, seed, load, cap, steps, warmup, plain=True, param
     progress("served stream: %d passes, %.3f s for %d histories; latency pass" % (steps, served_s, H))
     # latency under Poisson arrivals at `load` of each service's own capacity
     arr = poisson_arrivals(n_jobs, load * n_jobs / pass_s, seed + 1)
-    p50, p99 = latency_percentiles(Server(ctx, phantoms=ph).run(jobs, arr))
+    for srv, res, _ in passes:
+        pass_rounds.append(dict(kind="timed", rounds=srv.rounds, histories=sum(r.histories for r in res)))
+    srv = Server(ctx, phantoms=ph)
+    res = srv.run(jobs, arr)
+    pass_rounds.append(dict(kind="latency", rounds=srv.rounds, histories=sum(r.histories for r in res)))
+    p50, p99 = latency_percentiles(res)
     plain_s = q50 = q99 = None
     plain_rse = [float("nan")]
     pinfo = {"kernel_ms": float("nan"), "counters": {"steps": float("nan")}}
@@ -317,6 +327,7 @@ def served_stream(ctx, n_jobs, seed, load, cap, steps, warmup, plain=True, param
         "kernel_s": sum(p[0].kernel_ms for p in passes) / 1e3,
         "plain_kernel_s": pinfo["kernel_ms"] / 1e3,
         "rounds": len(rounds),
+        "pass_rounds": pass_rounds,
         "host_s_per_pass": {k: round(v, 4) for k, v in host_s.items()},
         "steps_per_history": counters["steps"] / H,
         "events_per_history": {k: counters[k] / H for k in ("steps", "real", "rayleigh", "compton", "photo",
