@@ -22,7 +22,9 @@ stream (k_transport_bank_round, photon banks per wave, or
 k_transport_queue_round, a photon per lane), confirmed against its
 neighbouring parameters on untimed passes of the stream itself; the
 roofline is that kernel's over the timed rounds: algorithmic bytes from its
-int64 event counters over its HIP-event time.
+int64 event counters over its HIP-event time, and its voxel-dose atomics
+per second against what the library's scattered-deposit probe sustains on a
+512^3 grid of dose records (`atomic_probe`).
 
 Beside it: the radiography workload (256 x 256 x 160 phantom at 1.25 mm,
 15-80 keV Kramers cone beam, 256 x 256 image, FP64 voxel dose; 10 steps of
@@ -83,6 +85,21 @@ def copy_probe(ctx, mb):
     return {"bytes_read": nbytes, "bytes_written": nbytes, "ms": ms, "GB_per_s": 2 * nbytes / ms / 1e6}
 
 
+def atomic_probe(ctx, deposits=200_000_000):
+    """The library's scattered-deposit probe (three device-scope atomics per
+    deposit at hashed 32-byte voxel records) on grids of the served stream's
+    smallest and largest dose arrays: the atomic rate the chip sustains for
+    the dose update pattern, against which the transport kernels' own
+    deposit rate is put."""
+    out = []
+    for n in (128, 512):
+        records = n ** 3
+        ms = ctx.atomic_probe(records, deposits)
+        out.append({"grid": "%d^3" % n, "record_bytes": 32 * records, "deposits": deposits, "ms": ms,
+                    "atomics_per_s": 3.0 * deposits / (ms / 1e3)})
+    return out
+
+
 def cpu_baseline(jobs, phantoms, seconds, affinity):
     """The FP64 C++/OpenMP restatement of the reference's CPU path on a
     bounded sample of the served stream: one job of each phantom size, each
@@ -139,9 +156,12 @@ def roofline(stats_list, traffic=None, source=None):
     kern_ms = np.array([s["kernel_ms"] for s in stats_list])
     byt = np.array([algorithmic_bytes(s) for s in stats_list], dtype=np.float64)
     achieved = byt.sum() / (kern_ms.sum() / 1e3) / 1e9
+    dep = float(sum(s["deposits"] for s in stats_list))
     out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel_ms_avg": float(kern_ms.mean()),
-           "algorithmic_bytes_per_launch": float(byt.mean())}
+           "algorithmic_bytes_per_launch": float(byt.mean()),
+           # the voxel-dose atomics (three per deposit) per second of kernel time
+           "dose_atomics_per_s": 3.0 * dep / (kern_ms.sum() / 1e3)}
     if source:
         out["traffic_source"] = source
     return out
@@ -556,6 +576,7 @@ def main():
     ap.add_argument("--ct-params", default=None,
                     help="event_kernel,blocks_per_cu,refill_min,interact_min of the CT launch (skips its autotuner)")
     ap.add_argument("--copy-probe-mb", type=int, default=0)
+    ap.add_argument("--no-atomic-probe", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--precision-histories", type=int, default=10 ** 9)
     args = ap.parse_args()
@@ -603,6 +624,12 @@ def main():
         out["precision"] = precision_study(ctx, args.precision_histories)
     if args.copy_probe_mb:
         out["copy_probe"] = copy_probe(ctx, args.copy_probe_mb)
+    if not args.no_atomic_probe:
+        out["atomic_probe"] = atomic_probe(ctx)
+        if out.get("roofline"):  # the served kernel's deposit atomics against the probe on a 512^3 grid
+            peak = out["atomic_probe"][-1]["atomics_per_s"]
+            out["roofline"]["dose_atomics_probe_per_s"] = peak
+            out["roofline"]["dose_atomics_frac"] = out["roofline"]["dose_atomics_per_s"] / peak
     if not args.no_cpu_baseline and jobs is not None:
         out["cpu_baseline"] = cpu_baseline(jobs, ph, args.cpu_seconds, affinity)
     print(json.dumps(out), flush=True)

@@ -334,6 +334,14 @@ int mdx_mem_info(mdx_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
 /* A streaming device-to-device copy of `bytes` (read and write once each),
  * timed: the reference kernel for HBM bandwidth and traffic counters. */
 int mdx_copy_probe(mdx_ctx* ctx, uint64_t bytes, double* ms);
+/* No counterpart in the reference: a measurement probe like mdx_copy_probe.
+ * `deposits` scattered voxel deposits into `records` zeroed 32-byte voxel
+ * records at hashed indices, each the three device-scope atomics of a
+ * transport kernel's deposit (sum, sum of squares, count), timed over the
+ * second of two launches: the achievable atomic rate for a dose grid of
+ * that size.  counted (may be NULL): the records' deposit counts summed on
+ * the host after both launches (2 x deposits), read back at the end. */
+int mdx_atomic_probe(mdx_ctx* ctx, uint64_t records, uint64_t deposits, double* ms, uint64_t* counted);
 /* Pinned host memory for fast tally and dose reads (hipHostMalloc). */
 int mdx_host_alloc(mdx_ctx* ctx, uint64_t bytes, void** out);
 int mdx_host_free(void* p);

@@ -46,7 +46,7 @@ EXPORTS = [
     "mdx_autotune", "mdx_job_alias_domain", "mdx_set_graphs",
     "mdx_run_jobs_async", "mdx_wait", "mdx_mem_info", "mdx_host_alloc", "mdx_host_free",
     "mdx_set_flights", "mdx_set_rita_builder", "mdx_set_dose_precision", "mdx_get_voxel_dose_f32", "mdx_stage_voxel_dose_f32",
-    "mdx_read_staged_dose_f32_async", "mdx_read_done", "mdx_copy_probe", "mdx_air_muen", "mdx_set_air_muen_table",
+    "mdx_read_staged_dose_f32_async", "mdx_read_done", "mdx_copy_probe", "mdx_atomic_probe", "mdx_air_muen", "mdx_set_air_muen_table",
     "mdx_set_local_majorants", "mdx_get_macro_cells", "mdx_delta_decision", "mdx_philox",
     "mdx_sample_source", "mdx_run_jobs_noalloc", "mdx_roi_stats", "mdx_set_event_kernel", "mdx_get_event_kernel",
     "mdx_rotate",
@@ -581,6 +581,17 @@ class Context:
         ms = ctypes.c_double()
         self._call(self.lib.mdx_copy_probe, self._h, ctypes.c_uint64(nbytes), ctypes.byref(ms))
         return ms.value
+
+    def atomic_probe(self, records, deposits, check=False):
+        """Milliseconds of `deposits` scattered voxel deposits (three
+        device-scope atomics each) into `records` 32-byte voxel records; with
+        check, also the deposit count the records hold after the probe's two
+        launches (2 x deposits; read back to the host)."""
+        ms = ctypes.c_double()
+        n = ctypes.c_uint64()
+        self._call(self.lib.mdx_atomic_probe, self._h, ctypes.c_uint64(records), ctypes.c_uint64(deposits),
+                   ctypes.byref(ms), ctypes.byref(n) if check else None)
+        return (ms.value, n.value) if check else ms.value
 
     def air_muen(self, energy):
         """mu_en / rho of dry air (cm^2/g) at energies in eV."""

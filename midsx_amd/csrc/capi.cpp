@@ -1993,6 +1993,37 @@ int mdx_copy_probe(mdx_ctx* c, uint64_t bytes, double* ms) {
   });
 }
 
+int mdx_atomic_probe(mdx_ctx* c, uint64_t records, uint64_t deposits, double* ms, uint64_t* counted) {
+  return guard([&] {
+    need(c && ms, "NULL argument");
+    need(records > 0 && deposits > 0, "records and deposits must be positive");
+    set_device(c);
+    DBuf r;
+    r.alloc(records * 32);
+    hip_check(mdx_launch::zero(r.as<double>(), records * 4, c->stream), "zero");
+    const int blocks = c->num_cu * 8;
+    hip_check(mdx_launch::atomic_probe(r.as<double>(), records, deposits, 1, blocks, c->stream), "probe");  // warm
+    hip_check(hipEventRecord(c->ev0, c->stream), "hipEventRecord");
+    hip_check(mdx_launch::atomic_probe(r.as<double>(), records, deposits, 2, blocks, c->stream), "probe");
+    hip_check(hipEventRecord(c->ev1, c->stream), "hipEventRecord");
+    hip_check(hipStreamSynchronize(c->stream), "probe");
+    float t = 0;
+    hip_check(hipEventElapsedTime(&t, c->ev0, c->ev1), "hipEventElapsedTime");
+    *ms = t;
+    if (counted) {  // the records' deposit counts summed on the host (both launches)
+      std::vector<double> h(4 * (size_t)records);
+      hip_check(hipMemcpy(h.data(), r.p, records * 32, hipMemcpyDeviceToHost), "read");
+      uint64_t n = 0;
+      for (uint64_t i = 0; i < records; ++i) {
+        uint64_t k;
+        std::memcpy(&k, &h[4 * i + 2], 8);
+        n += k;
+      }
+      *counted = n;
+    }
+  });
+}
+
 int mdx_host_alloc(mdx_ctx* c, uint64_t bytes, void** out) {
   return guard([&] {
     need(c && out, "NULL argument");

@@ -80,6 +80,9 @@ hipError_t rita_build(const double* lx, const double* ly, int n_ff, double alpha
 int rita_cap();
 // Streaming copy of n16 16-byte words (HBM reference for traffic counters).
 hipError_t copy16(const void* src, void* dst, size_t n16, int blocks, hipStream_t st);
+// n_dep scattered voxel deposits (three device-scope atomics each) into the
+// 32-byte records rec[0 .. n_rec) (atomic-rate reference).
+hipError_t atomic_probe(double* rec, uint64_t n_rec, uint64_t n_dep, uint64_t seed, int blocks, hipStream_t st);
 // Per requested box (n of them, one block each): (deposits, sum, sum^2) of the
 // FP64 voxel records into out[3 i ..].
 hipError_t roi_stats(const mdx_roi_req* reqs, int n, double* out, hipStream_t st);

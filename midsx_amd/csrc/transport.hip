@@ -1998,6 +1998,24 @@ __global__ void k_copy16(const float4* __restrict__ src, float4* __restrict__ ds
     dst[i] = src[i];
 }
 
+// Scattered deposits: deposit i adds (v, v^2, 1) to the 32-byte record of
+// a hashed index among n_rec, with the three device-scope atomics of a voxel
+// deposit.  The reference for the atomic rate of the transport kernels.
+__global__ void k_atomic_probe(double* __restrict__ rec, uint64_t n_rec, uint64_t n_dep, uint64_t seed) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_dep;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t h = (i + seed) * 0x9E3779B97F4A7C15ull;  // splitmix64 finaliser
+    h = (h ^ (h >> 30)) * 0xBF58476D1CE4E5B9ull;
+    h = (h ^ (h >> 27)) * 0x94D049BB133111EBull;
+    h ^= h >> 31;
+    double* p = rec + 4 * __umul64hi(h, n_rec);  // index in [0, n_rec)
+    const double v = 1.0 + (double)(h & 1023u) * (1.0 / 1024.0);
+    gadd(p, v);
+    gadd(p + 1, v * v);
+    gadd(reinterpret_cast<unsigned long long*>(p + 2), 1ull);
+  }
+}
+
 inline unsigned blocks_for(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
 // Static scratch of mdx_run_jobs_noalloc: the launch's event counters, the
@@ -2231,6 +2249,12 @@ hipError_t copy16(const void* src, void* dst, size_t n16, int blocks, hipStream_
   if (n16 == 0) return hipSuccess;
   hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(256), 0, st, static_cast<const float4*>(src),
                      static_cast<float4*>(dst), n16);
+  return hipGetLastError();
+}
+
+hipError_t atomic_probe(double* rec, uint64_t n_rec, uint64_t n_dep, uint64_t seed, int blocks, hipStream_t st) {
+  if (n_dep == 0 || n_rec == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_atomic_probe, dim3((unsigned)blocks), dim3(256), 0, st, rec, n_rec, n_dep, seed);
   return hipGetLastError();
 }
 

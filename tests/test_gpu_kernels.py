@@ -307,3 +307,14 @@ def test_compton_sampling_chi2(ctx, E):
     expect = np.array([integrate.quad(f, edges[i], edges[i + 1])[0] for i in range(50)]) / norm * n
     chi2 = np.sum((counts - expect) ** 2 / expect)
     assert stats.chi2.sf(chi2, 49) > 0.01, chi2
+
+
+def test_atomic_probe_counts_every_deposit(ctx):
+    """The atomic-rate probe: both of its launches land every deposit (the
+    records' counts sum to 2 x deposits) and it is timed; bad sizes fail."""
+    for records, deposits in ((1, 1000), (4096, 300_000), (1 << 20, 1_000_003)):
+        ms, n = ctx.atomic_probe(records, deposits, check=True)
+        assert n == 2 * deposits, (records, deposits, n)
+        assert ms > 0.0
+    with pytest.raises(M.MdxError):
+        ctx.atomic_probe(0, 10)
