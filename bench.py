@@ -181,7 +181,7 @@ def event_kernel_symbol(ctx):
 
 
 def set_event_kernel(ctx, name):
-    """name: queue, bank96 or bank128."""
+    """name: queue, bank96, bank128 or bank152."""
     import midsx_amd as M
     if name == "queue":
         ctx.set_event_kernel(M.EVENT_QUEUE)
@@ -192,7 +192,7 @@ def set_event_kernel(ctx, name):
 def stream_tune(ctx, jobs, ph, tuned):
     """Start-up tuning on the stream itself: the autotuner probes one job,
     while a round mixes every admitted job, so the autotuner's choice and
-    its neighbours (the other bank size, one block per CU more or fewer,
+    its neighbours (the other bank sizes, one block per CU more or fewer,
     interaction batches of 32, 48 and 64) each serve one pass of the stream
     and the fastest is kept."""
     from midsx_amd.serve import Server
@@ -201,7 +201,7 @@ def stream_tune(ctx, jobs, ph, tuned):
     b, r, i = ctx.queue_params()
     cands = [(kind, bank, b, r, i)]
     if kind == M.EVENT_BANK:
-        cands.append((kind, 96 if bank == 128 else 128, b, r, i))
+        cands += [(kind, bs, b, r, i) for bs in M.BANK_SIZES if bs != bank]
     for bb in (b - 1, b + 1):
         if 2 <= bb <= 4:
             cands.append((kind, bank, bb, r, i))
@@ -562,7 +562,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--queue-params", default=None,
                     help="blocks_per_cu,refill_min,interact_min for the radiography (skips its autotuner)")
-    ap.add_argument("--event-kernel", default="queue", choices=("queue", "bank96", "bank128"),
+    ap.add_argument("--event-kernel", default="queue", choices=("queue", "bank96", "bank128", "bank152"),
                     help="event kernel of the radiography with --queue-params")
     ap.add_argument("--served-params", default=None,
                     help="event_kernel,blocks_per_cu,refill_min,interact_min of the served stream, e.g. "

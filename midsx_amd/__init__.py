@@ -25,6 +25,8 @@ TALLY_AACUBOID, TALLY_DISC, TALLY_RECT, TALLY_IMAGE = 0, 1, 2, 3
 XS_TOTAL, XS_COHERENT, XS_INCOHERENT, XS_PHOTOELECTRIC, SF_INCOHERENT, FF_COHERENT = 0, 1, 2, 3, 4, 5
 KERNEL_PLAIN, KERNEL_QUEUE = 0, 1
 EVENT_QUEUE, EVENT_BANK = 0, 1
+BANK_SIZES = (96, 128, 152)  # photons per wave bank of EVENT_BANK
+BANK_MAX_HISTORY = 1 << 42  # EVENT_BANK launches: first + n at most this
 TUNE_ROUNDS, TUNE_EVENTS = 1, 2
 DOSE_F64, DOSE_F32, DOSE_F32C, DOSE_F32C_LDS = 0, 1, 2, 3
 NQ, NBUCKET, NFIELD = 4, 4, 3
@@ -386,7 +388,8 @@ class Context:
 
     def set_event_kernel(self, kind=EVENT_QUEUE, bank_size=128):
         """The event form of KERNEL_QUEUE runs: EVENT_QUEUE (lane-resident
-        photons) or EVENT_BANK (per-wave photon banks of bank_size in LDS)."""
+        photons) or EVENT_BANK (per-wave photon banks of bank_size in LDS, one
+        of BANK_SIZES; history indices below BANK_MAX_HISTORY)."""
         self._call(self.lib.mdx_set_event_kernel, self._h, ctypes.c_int(kind), ctypes.c_int(bank_size))
 
     def event_kernel(self):

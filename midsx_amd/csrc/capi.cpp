@@ -933,6 +933,12 @@ int domain_owner(mdx_ctx* c, int id) {
 }
 
 // Fills the device job records of one launch and validates every job.
+// The bank kernel keeps 42 bits of a history index (MDX_BANK_MAX_HISTORY).
+void check_bank_range(const mdx_ctx* c, uint64_t first, uint64_t n) {
+  if (c->event_kernel == MDX_EVENT_BANK && (first > MDX_BANK_MAX_HISTORY || n > MDX_BANK_MAX_HISTORY - first))
+    throw std::out_of_range("history indices of the bank event kernel end at 2^42 (first + n)");
+}
+
 uint64_t fill_round(mdx_ctx* c, int n_jobs, const int* ids, const uint64_t* first, const uint64_t* n,
                     const uint64_t* seed, mdx_job_dev* jd, uint64_t* prefix) {
   KeepCur kc(c);
@@ -951,6 +957,7 @@ uint64_t fill_round(mdx_ctx* c, int n_jobs, const int* ids, const uint64_t* firs
     jd[i].D = D;
     jd[i].S = c->J().d_source.as<mdx_source_dev>();
     jd[i].TL = c->J().d_tallies.as<mdx_tallies_dev>();
+    check_bank_range(c, first[i], n[i]);
     jd[i].first = first[i];
     jd[i].n = n[i];
     jd[i].seed = seed[i];
@@ -1576,7 +1583,7 @@ int mdx_set_event_kernel(mdx_ctx* c, int kind, int bank_size) {
   return guard([&] {
     need(c, "NULL context");
     need(kind == MDX_EVENT_QUEUE || kind == MDX_EVENT_BANK, "unknown event kernel");
-    need(bank_size == 96 || bank_size == 128, "bank_size must be 96 or 128");
+    need(bank_size == 96 || bank_size == 128 || bank_size == 152, "bank_size must be 96, 128 or 152");
     c->event_kernel = kind;
     c->bank_size = bank_size;
   });
@@ -1674,7 +1681,7 @@ int mdx_autotune(mdx_ctx* c, uint64_t probe_histories, int flags, int* blocks_pe
     if ((flags & MDX_TUNE_EVENTS) && !lds_dose) {
       int kb = 96, bb = 2, rb = 64, ib = 32;
       double tb = 1e300;
-      for (int bs : {96, 128})
+      for (int bs : {96, 128, 152})
         for (int v : {2, 3, 4}) {
           double t = probe(MDX_EVENT_BANK, bs, v, rb, ib);
           if (t < tb) { tb = t; kb = bs; bb = v; }
@@ -1914,6 +1921,7 @@ int mdx_run_jobs_noalloc(mdx_ctx* c, int n_jobs, const int* ids, const uint64_t*
         need(c->J().d_tallies.p != nullptr, "tallies not set (mdx_set_tallies)");
         jd[i].S = c->J().d_source.as<mdx_source_dev>();
         jd[i].TL = c->J().d_tallies.as<mdx_tallies_dev>();
+        check_bank_range(c, first[i], n[i]);
         jd[i].first = first[i];
         jd[i].n = n[i];
         jd[i].seed = seed[i];

@@ -212,6 +212,9 @@ struct mdx_job_dev {
 // counters of one launch live in a static device array (see transport.hip),
 // which holds at most this many jobs.
 #define MDX_NOALLOC_MAX_JOBS 256
+// History indices the bank event kernel (MDX_EVENT_BANK) takes: first + n
+// of every slice at most this (a bank slot keeps 42 bits of the index).
+#define MDX_BANK_MAX_HISTORY (1ull << 42)
 
 // One box of a job's voxel grid whose FP64 voxel records mdx_roi_stats sums:
 // voxel index ranges [lo, hi) per axis of a grid with nx, ny voxels in x, y.

@@ -1190,28 +1190,38 @@ k_transport_queue_round(const mdx_tables_dev* __restrict__ T, const char* __rest
 // A wave holds the photons of one job at a time (MULTI: the next job with
 // histories left once its own has run out and its bank is empty).
 enum { kEvStep = 0, kEvFly = 1, kEvInt = 2, kEvFree = 3, kEvTypes = 4 };
-// Cached total cross sections per bank slot: the global majorant only.
-// Caching material cross sections as well costs more LDS (i.e. waves per
-// CU) than the table lookups it saves.
-constexpr int kBankCache = 1;
+
+// A photon's slot is 64 bytes, so that a wave's bank holds 152 of them at
+// four blocks per CU (banks of 96 or 128 leave LDS unused); nothing that a
+// load from the tables or the grid can give again is kept: the safe
+// distance of the local majorant is its radius (8 bits of the flags) times
+// the job's macro-cell edge, and no cross section is cached beside the two
+// below.  The high word of the history index shares a word with the step
+// counter, which limits the bank kernel to history indices below 2^42.
+constexpr uint32_t kStepBits = 22;  // step counter bits: kMaxSteps = 2^20 fits
+constexpr uint32_t kStepMask = (1u << kStepBits) - 1u;
+static_assert(kMaxSteps <= kStepMask, "the step counter fits its bits");
+static_assert(MDX_BANK_MAX_HISTORY == 1ull << (32 + (32 - kStepBits)), "history index bits of a bank slot");
 
 template <int B>
 struct WaveBank {
   mdx_v4 pe[B];   // position, energy
   mdx_v4 dm[B];   // direction, majorant of the next step
-  // safe distance, sigma_tot of the cached slot, distance to the next tally
-  // event, length of the step that ended in the pending interaction
+  // sigma_tot of the cached slot, distance to the next tally event, length
+  // of the step that ended in the pending interaction, history index (low
+  // word, as its bits)
   mdx_v4 aux[B];
-  uint32_t hlo[B], hhi[B], c0[B], nstep[B];  // history index, Philox block counter, step counter
+  uint32_t c0[B];     // Philox block counter
+  uint32_t hn[B];     // step counter | high word of the history index << kStepBits
   uint32_t flags[B];  // see fl_get / fl_set
-  float sc[kBankCache][B];
   uint8_t list[kEvTypes][B];
 };
 
 // flags: bits 0-1 coherent scatters (saturating at 3), 2-3 incoherent, 4-11
 // majorant class (255: global), 12-19 slot of the cached sigma_tot (255:
-// none), 20-23 grid of the pending interaction + 1 (0: background), 24 ..
-// 24 + kBankCache - 1 valid cache entries.
+// none), 20-23 grid of the pending interaction + 1 (0: background), 24-31
+// safe radius of the class in macro cells (the safe distance when the class
+// is not the global majorant's).
 __device__ __forceinline__ uint32_t fl_get(uint32_t f, int sh, uint32_t m) { return (f >> sh) & m; }
 __device__ __forceinline__ uint32_t fl_set(uint32_t f, int sh, uint32_t m, uint32_t v) {
   return (f & ~(m << sh)) | ((v & m) << sh);
@@ -1235,15 +1245,20 @@ struct Ph {
   uint32_t hlo, hhi, c0, nstep, flags;
 };
 
+// mcell: the job's macro-cell edge (the safe distance's unit).
 template <int B>
-__device__ __forceinline__ Ph bank_load(const WaveBank<B>* W, int s) {
+__device__ __forceinline__ Ph bank_load(const WaveBank<B>* W, int s, float mcell) {
   Ph p;
   const mdx_v4 a = W->pe[s], b = W->dm[s], c = W->aux[s];
   p.px = a.x; p.py = a.y; p.pz = a.z; p.E = a.w;
   p.dx = b.x; p.dy = b.y; p.dz = b.z; p.smax = b.w;
-  p.dsafe = c.x; p.stot = c.y; p.tfl = c.z; p.qs = c.w;
-  p.hlo = W->hlo[s]; p.hhi = W->hhi[s]; p.c0 = W->c0[s]; p.nstep = W->nstep[s];
+  p.stot = c.x; p.tfl = c.y; p.qs = c.z; p.hlo = __float_as_uint(c.w);
+  const uint32_t hn = W->hn[s];
+  p.nstep = hn & kStepMask;
+  p.hhi = hn >> kStepBits;
+  p.c0 = W->c0[s];
   p.flags = W->flags[s];
+  p.dsafe = fl_get(p.flags, 4, 255u) == 255u ? INFINITY : (float)fl_get(p.flags, 24, 255u) * mcell;
   return p;
 }
 
@@ -1251,8 +1266,9 @@ template <int B>
 __device__ __forceinline__ void bank_store(WaveBank<B>* W, int s, const Ph& p) {
   W->pe[s] = mdx_v4{p.px, p.py, p.pz, p.E};
   W->dm[s] = mdx_v4{p.dx, p.dy, p.dz, p.smax};
-  W->aux[s] = mdx_v4{p.dsafe, p.stot, p.tfl, p.qs};
-  W->hlo[s] = p.hlo; W->hhi[s] = p.hhi; W->c0[s] = p.c0; W->nstep[s] = p.nstep;
+  W->aux[s] = mdx_v4{p.stot, p.tfl, p.qs, __uint_as_float(p.hlo)};
+  W->hn[s] = p.nstep | (p.hhi << kStepBits);
+  W->c0[s] = p.c0;
   W->flags[s] = p.flags;
 }
 
@@ -1264,19 +1280,10 @@ __device__ __forceinline__ void wave_sync_lds() {
 }
 
 // Total cross section of material slot (or the global majorant, slot 255)
-// at the photon's energy, from its bank cache when valid.
-template <int B>
+// at energy E.
 __device__ __forceinline__ float bank_sigma(const mdx_tables_dev* T, const mdx_curve* ltot, const mdx_curve& maj,
-                                            WaveBank<B>* W, int s, Ph& p, int slot) {
-  const int ce = slot == 255 ? kBankCache - 1 : (slot < kBankCache - 1 ? slot : -1);  // -1: not cached
-  const uint32_t valid = fl_get(p.flags, 24, 63u);
-  if (ce >= 0 && (valid >> ce & 1u)) return W->sc[ce][s];
-  const float v = curve_eval(T, slot == 255 ? maj : ltot[slot], p.E);
-  if (ce >= 0) {
-    W->sc[ce][s] = v;
-    p.flags = fl_set(p.flags, 24, 63u, valid | (1u << ce));
-  }
-  return v;
+                                            float E, int slot) {
+  return curve_eval(T, slot == 255 ? maj : ltot[slot], E);
 }
 
 template <bool MULTI, int B>
@@ -1409,8 +1416,6 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
         p.nstep = 0;
         p.flags = fl_set(fl_set(0u, 4, 255u, 255u), 12, 255u, 255u);
         p.smax = curve_eval(T, wave_const(T)->majorant, p.E);
-        W->sc[kBankCache - 1][slot] = p.smax;
-        p.flags = fl_set(p.flags, 24, 63u, 1u << (kBankCache - 1));
         p.dsafe = INFINITY;
         p.stot = 0.0f;
         p.qs = 0.0f;
@@ -1423,7 +1428,7 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
     } else if (ev == kEvStep || ev == kEvFly) {
       // ---- one Woodcock step or one background free flight (physics_engine.cpp:38-66)
       if (act) {
-        Ph p = bank_load(W, slot);
+        Ph p = bank_load(W, slot, D->grids[0].mcell);
         ++c_step;
         float u1, u2;
         step_uniforms(step_key(jseed, jstream, p.hhi), p.hlo, p.nstep, u1, u2);
@@ -1465,7 +1470,7 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
             if (sl != cslot) {
               // a local majorant of class k is material k's own total cross
               // section at this energy: the same value, no second lookup
-              p.stot = sl == (int)fl_get(p.flags, 4, 255u) ? p.smax : bank_sigma(T, ltot, wave_const(T)->majorant, W, slot, p, sl);
+              p.stot = sl == (int)fl_get(p.flags, 4, 255u) ? p.smax : bank_sigma(T, ltot, wave_const(T)->majorant, p.E, sl);
               cslot = sl;
             }
             if (!limited && real_collision(u2, delta_threshold(p.stot, p.smax))) {
@@ -1484,16 +1489,17 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
             if (nr == 0u || p.E < 1.0e3f) ncls = 255;
             if (ncls != (int)fl_get(p.flags, 4, 255u)) {
               p.flags = fl_set(p.flags, 4, 255u, (uint32_t)ncls);
-              p.smax = ncls == cslot ? p.stot : bank_sigma(T, ltot, wave_const(T)->majorant, W, slot, p, ncls);
+              p.smax = ncls == cslot ? p.stot : bank_sigma(T, ltot, wave_const(T)->majorant, p.E, ncls);
             }
             p.dsafe = ncls == 255 ? INFINITY : (float)nr * D->grids[0].mcell;
+            p.flags = fl_set(p.flags, 24, 255u, nr);
           }
         } else {
           // background free flight at the background's own cross section,
           // cut at the next grid face or the domain's boundary
           const int sl = D->bg_slot;
           if (sl != cslot) {
-            p.stot = bank_sigma(T, ltot, wave_const(T)->majorant, W, slot, p, sl);
+            p.stot = bank_sigma(T, ltot, wave_const(T)->majorant, p.E, sl);
             cslot = sl;
           }
           const float si = -__log2f(u1) * (0.69314718055994531f * frcp(p.stot));
@@ -1537,7 +1543,7 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
     } else {
       // ---- the parked real interactions (physics_engine.cpp:62-78)
       if (act) {
-        Ph p = bank_load(W, slot);
+        Ph p = bank_load(W, slot, D->grids[0].mcell);
         ++c_real;
         const int cslot = (int)fl_get(p.flags, 12, 255u);
         const mdx_material_dev& M = T->mat[cslot];
@@ -1572,8 +1578,8 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
             mcls = 255;
             p.dsafe = INFINITY;
           }
-          p.flags = fl_set(fl_set(p.flags, 4, 255u, (uint32_t)mcls), 24, 63u, 0u);  // a new energy: cache invalid
-          p.smax = bank_sigma(T, ltot, wave_const(T)->majorant, W, slot, p, mcls);
+          p.flags = fl_set(p.flags, 4, 255u, (uint32_t)mcls);
+          p.smax = bank_sigma(T, ltot, wave_const(T)->majorant, p.E, mcls);
           p.flags = fl_set(p.flags, 12, 255u, 255u);
         } else {
           ++c_photo;
@@ -1640,7 +1646,7 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
   if (threadIdx.x < MDX_NCOUNTER && cnt[threadIdx.x]) atomicAdd(counters + threadIdx.x, cnt[threadIdx.x]);
 }
 
-// Blocks of four waves with banks of 96 or 128 photons each fit four to a
+// Blocks of four waves with banks of 96, 128 or 152 photons each fit four to a
 // CU's LDS; the registers are held to four waves per SIMD to match.
 template <bool MULTI, int B>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
@@ -2104,6 +2110,9 @@ static hipError_t launch_bank(int bank, unsigned blocks, const mdx_tables_dev* T
   else if (bank == 128)
     hipLaunchKernelGGL((k_transport_bank<MULTI, 128>), dim3(blocks), dim3(kBlock), 0, st, T, jobs, prefix, n_jobs,
                        total, work, counters, refill_min, interact_min);
+  else if (bank == 152)
+    hipLaunchKernelGGL((k_transport_bank<MULTI, 152>), dim3(blocks), dim3(kBlock), 0, st, T, jobs, prefix, n_jobs,
+                       total, work, counters, refill_min, interact_min);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
@@ -2131,6 +2140,9 @@ hipError_t transport_bank_round(const mdx_tables_dev* T, const char* round, unsi
                        counters, refill_min, interact_min);
   else if (bank == 128)
     hipLaunchKernelGGL(k_transport_bank_round<128>, dim3((unsigned)grid_blocks), dim3(kBlock), 0, st, T, round, work,
+                       counters, refill_min, interact_min);
+  else if (bank == 152)
+    hipLaunchKernelGGL(k_transport_bank_round<152>, dim3((unsigned)grid_blocks), dim3(kBlock), 0, st, T, round, work,
                        counters, refill_min, interact_min);
   else
     return hipErrorInvalidValue;

@@ -12,7 +12,8 @@ import midsx_amd as M
 # Context.set_event_kernel
 EVENT_FORMS = [pytest.param((M.EVENT_QUEUE, 128), id="queue"),
                pytest.param((M.EVENT_BANK, 96), id="bank96"),
-               pytest.param((M.EVENT_BANK, 128), id="bank128")]
+               pytest.param((M.EVENT_BANK, 128), id="bank128"),
+               pytest.param((M.EVENT_BANK, 152), id="bank152")]
 
 
 def batch_mean_sigma(samples):

@@ -104,3 +104,23 @@ def test_multi_job_launch_with_an_empty_slice(events):
         assert np.allclose(m, a, rtol=1e-12, atol=1e-9), (j, np.max(np.abs(m - a)))
     assert not np.any(snapshot(ctx, jobs[1]))
     ctx.close()
+
+
+@pytest.mark.parametrize("events", [p for p in EVENT_FORMS if p.values[0][0] == M.EVENT_BANK])
+def test_bank_history_index_limit(events):
+    """The bank kernel keeps 42 bits of a history index: a launch that ends
+    at 2^42 runs, one past it is refused before anything runs (the queue
+    kernel has no such limit)."""
+    ctx, K = slab_ctx(("events", events))
+    top = M.BANK_MAX_HISTORY
+    ctx.reset_tallies()
+    st = ctx.run(300, seed=9, first=top - 300, kernel=K)
+    assert st["histories"] == 300 and st["photo"] + st["escapes"] == 300
+    with pytest.raises(M.MdxError) as e:
+        ctx.run(300, seed=9, first=top - 299, kernel=K)
+    assert e.value.status == M.ERR_RANGE
+    with pytest.raises(M.MdxError):
+        ctx.run_jobs([ctx.job], [300], [top], [9])
+    ctx.set_event_kernel(M.EVENT_QUEUE)
+    assert ctx.run(300, seed=9, first=top, kernel=K)["histories"] == 300
+    ctx.close()
