@@ -181,7 +181,7 @@ def event_kernel_symbol(ctx):
 
 
 def set_event_kernel(ctx, name):
-    """name: queue, bank96, bank128 or bank152."""
+    """name: queue, bank96, bank128 or bank160."""
     import midsx_amd as M
     if name == "queue":
         ctx.set_event_kernel(M.EVENT_QUEUE)
@@ -562,7 +562,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--queue-params", default=None,
                     help="blocks_per_cu,refill_min,interact_min for the radiography (skips its autotuner)")
-    ap.add_argument("--event-kernel", default="queue", choices=("queue", "bank96", "bank128", "bank152"),
+    ap.add_argument("--event-kernel", default="queue", choices=("queue", "bank96", "bank128", "bank160"),
                     help="event kernel of the radiography with --queue-params")
     ap.add_argument("--served-params", default=None,
                     help="event_kernel,blocks_per_cu,refill_min,interact_min of the served stream, e.g. "

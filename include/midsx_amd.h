@@ -201,7 +201,7 @@ int mdx_get_queue_params(mdx_ctx* ctx, int* blocks_per_cu, int* refill_min, int*
 /* The event form MDX_KERNEL_QUEUE runs (mdx_run, mdx_run_jobs and their
  * variants): MDX_EVENT_QUEUE, every lane carrying one photon through
  * wave-level refill / step / interaction phases, or MDX_EVENT_BANK, every
- * wave keeping a bank of bank_size photons (96, 128 or 152) in LDS with one
+ * wave keeping a bank of bank_size photons (96, 128 or 160) in LDS with one
  * list of bank slots per next event (Woodcock step, background flight, real
  * interaction, free) and running each event on up to 64 photons of one
  * list at a time.  Same physics and tallies; the per-block LDS dose mode

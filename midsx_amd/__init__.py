@@ -25,7 +25,7 @@ TALLY_AACUBOID, TALLY_DISC, TALLY_RECT, TALLY_IMAGE = 0, 1, 2, 3
 XS_TOTAL, XS_COHERENT, XS_INCOHERENT, XS_PHOTOELECTRIC, SF_INCOHERENT, FF_COHERENT = 0, 1, 2, 3, 4, 5
 KERNEL_PLAIN, KERNEL_QUEUE = 0, 1
 EVENT_QUEUE, EVENT_BANK = 0, 1
-BANK_SIZES = (96, 128, 152)  # photons per wave bank of EVENT_BANK
+BANK_SIZES = (96, 128, 160)  # photons per wave bank of EVENT_BANK
 BANK_MAX_HISTORY = 1 << 42  # EVENT_BANK launches: first + n at most this
 TUNE_ROUNDS, TUNE_EVENTS = 1, 2
 DOSE_F64, DOSE_F32, DOSE_F32C, DOSE_F32C_LDS = 0, 1, 2, 3

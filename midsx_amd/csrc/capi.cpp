@@ -1583,7 +1583,7 @@ int mdx_set_event_kernel(mdx_ctx* c, int kind, int bank_size) {
   return guard([&] {
     need(c, "NULL context");
     need(kind == MDX_EVENT_QUEUE || kind == MDX_EVENT_BANK, "unknown event kernel");
-    need(bank_size == 96 || bank_size == 128 || bank_size == 152, "bank_size must be 96, 128 or 152");
+    need(bank_size == 96 || bank_size == 128 || bank_size == 160, "bank_size must be 96, 128 or 160");
     c->event_kernel = kind;
     c->bank_size = bank_size;
   });
@@ -1681,7 +1681,7 @@ int mdx_autotune(mdx_ctx* c, uint64_t probe_histories, int flags, int* blocks_pe
     if ((flags & MDX_TUNE_EVENTS) && !lds_dose) {
       int kb = 96, bb = 2, rb = 64, ib = 32;
       double tb = 1e300;
-      for (int bs : {96, 128, 152})
+      for (int bs : {96, 128, 160})
         for (int v : {2, 3, 4}) {
           double t = probe(MDX_EVENT_BANK, bs, v, rb, ib);
           if (t < tb) { tb = t; kb = bs; bb = v; }

@@ -1191,13 +1191,16 @@ k_transport_queue_round(const mdx_tables_dev* __restrict__ T, const char* __rest
 // histories left once its own has run out and its bank is empty).
 enum { kEvStep = 0, kEvFly = 1, kEvInt = 2, kEvFree = 3, kEvTypes = 4 };
 
-// A photon's slot is 64 bytes, so that a wave's bank holds 152 of them at
+// A photon's slot is 60 bytes, so that a wave's bank holds 160 of them at
 // four blocks per CU (banks of 96 or 128 leave LDS unused); nothing that a
 // load from the tables or the grid can give again is kept: the safe
 // distance of the local majorant is its radius (8 bits of the flags) times
 // the job's macro-cell edge, and no cross section is cached beside the two
-// below.  The high word of the history index shares a word with the step
-// counter, which limits the bank kernel to history indices below 2^42.
+// below.  The distance to the next tally event is dead while a photon waits
+// for its interaction (the interaction recomputes it), so that word holds
+// the length of the step that ended there meanwhile.  The high word of the
+// history index shares a word with the step counter, which limits the bank
+// kernel to history indices below 2^42.
 constexpr uint32_t kStepBits = 22;  // step counter bits: kMaxSteps = 2^20 fits
 constexpr uint32_t kStepMask = (1u << kStepBits) - 1u;
 static_assert(kMaxSteps <= kStepMask, "the step counter fits its bits");
@@ -1207,11 +1210,10 @@ template <int B>
 struct WaveBank {
   mdx_v4 pe[B];   // position, energy
   mdx_v4 dm[B];   // direction, majorant of the next step
-  // sigma_tot of the cached slot, distance to the next tally event, length
-  // of the step that ended in the pending interaction, history index (low
-  // word, as its bits)
+  // sigma_tot of the cached slot, distance to the next tally event (while
+  // parked for an interaction: length of the step that ended there),
+  // history index (low word) and Philox block counter (as their bits)
   mdx_v4 aux[B];
-  uint32_t c0[B];     // Philox block counter
   uint32_t hn[B];     // step counter | high word of the history index << kStepBits
   uint32_t flags[B];  // see fl_get / fl_set
   uint8_t list[kEvTypes][B];
@@ -1252,11 +1254,10 @@ __device__ __forceinline__ Ph bank_load(const WaveBank<B>* W, int s, float mcell
   const mdx_v4 a = W->pe[s], b = W->dm[s], c = W->aux[s];
   p.px = a.x; p.py = a.y; p.pz = a.z; p.E = a.w;
   p.dx = b.x; p.dy = b.y; p.dz = b.z; p.smax = b.w;
-  p.stot = c.x; p.tfl = c.y; p.qs = c.z; p.hlo = __float_as_uint(c.w);
+  p.stot = c.x; p.tfl = c.y; p.qs = c.y; p.hlo = __float_as_uint(c.z); p.c0 = __float_as_uint(c.w);
   const uint32_t hn = W->hn[s];
   p.nstep = hn & kStepMask;
   p.hhi = hn >> kStepBits;
-  p.c0 = W->c0[s];
   p.flags = W->flags[s];
   p.dsafe = fl_get(p.flags, 4, 255u) == 255u ? INFINITY : (float)fl_get(p.flags, 24, 255u) * mcell;
   return p;
@@ -1266,9 +1267,8 @@ template <int B>
 __device__ __forceinline__ void bank_store(WaveBank<B>* W, int s, const Ph& p) {
   W->pe[s] = mdx_v4{p.px, p.py, p.pz, p.E};
   W->dm[s] = mdx_v4{p.dx, p.dy, p.dz, p.smax};
-  W->aux[s] = mdx_v4{p.stot, p.tfl, p.qs, __uint_as_float(p.hlo)};
+  W->aux[s] = mdx_v4{p.stot, p.tfl, __uint_as_float(p.hlo), __uint_as_float(p.c0)};
   W->hn[s] = p.nstep | (p.hhi << kStepBits);
-  W->c0[s] = p.c0;
   W->flags[s] = p.flags;
 }
 
@@ -1418,7 +1418,6 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
         p.smax = curve_eval(T, wave_const(T)->majorant, p.E);
         p.dsafe = INFINITY;
         p.stot = 0.0f;
-        p.qs = 0.0f;
         p.tfl = next_tally_event(TL, p.px, p.py, p.pz, p.dx, p.dy, p.dz);
         uint64_t lin0;
         const bool fly = jflights && in_domain(D, p.px, p.py, p.pz) && voxel_lookup(D, p.px, p.py, p.pz, lin0) < 0;
@@ -1475,7 +1474,7 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
             }
             if (!limited && real_collision(u2, delta_threshold(p.stot, p.smax))) {
               next = kEvInt;
-              p.qs = s;
+              p.tfl = s;  // parked: the word holds the step's length (loaded as qs)
               p.flags = fl_set(p.flags, 20, 15u, (uint32_t)(g + 1));
             } else {
               p.tfl -= s;
@@ -1510,7 +1509,7 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
             p.py = y0 + si * p.dy;
             p.pz = z0 + si * p.dz;
             next = kEvInt;
-            p.qs = si;
+            p.tfl = si;
             p.flags = fl_set(p.flags, 20, 15u, 0u);
           } else if (dgrid <= dexit) {  // up to the grid's face; majorant steps from there
             p.px = x0 + dgrid * p.dx;
@@ -1646,7 +1645,7 @@ __device__ __forceinline__ void transport_bank_body(const mdx_tables_dev* __rest
   if (threadIdx.x < MDX_NCOUNTER && cnt[threadIdx.x]) atomicAdd(counters + threadIdx.x, cnt[threadIdx.x]);
 }
 
-// Blocks of four waves with banks of 96, 128 or 152 photons each fit four to a
+// Blocks of four waves with banks of 96, 128 or 160 photons each fit four to a
 // CU's LDS; the registers are held to four waves per SIMD to match.
 template <bool MULTI, int B>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
@@ -2110,8 +2109,8 @@ static hipError_t launch_bank(int bank, unsigned blocks, const mdx_tables_dev* T
   else if (bank == 128)
     hipLaunchKernelGGL((k_transport_bank<MULTI, 128>), dim3(blocks), dim3(kBlock), 0, st, T, jobs, prefix, n_jobs,
                        total, work, counters, refill_min, interact_min);
-  else if (bank == 152)
-    hipLaunchKernelGGL((k_transport_bank<MULTI, 152>), dim3(blocks), dim3(kBlock), 0, st, T, jobs, prefix, n_jobs,
+  else if (bank == 160)
+    hipLaunchKernelGGL((k_transport_bank<MULTI, 160>), dim3(blocks), dim3(kBlock), 0, st, T, jobs, prefix, n_jobs,
                        total, work, counters, refill_min, interact_min);
   else
     return hipErrorInvalidValue;
@@ -2141,8 +2140,8 @@ hipError_t transport_bank_round(const mdx_tables_dev* T, const char* round, unsi
   else if (bank == 128)
     hipLaunchKernelGGL(k_transport_bank_round<128>, dim3((unsigned)grid_blocks), dim3(kBlock), 0, st, T, round, work,
                        counters, refill_min, interact_min);
-  else if (bank == 152)
-    hipLaunchKernelGGL(k_transport_bank_round<152>, dim3((unsigned)grid_blocks), dim3(kBlock), 0, st, T, round, work,
+  else if (bank == 160)
+    hipLaunchKernelGGL(k_transport_bank_round<160>, dim3((unsigned)grid_blocks), dim3(kBlock), 0, st, T, round, work,
                        counters, refill_min, interact_min);
   else
     return hipErrorInvalidValue;

@@ -13,7 +13,7 @@ import midsx_amd as M
 EVENT_FORMS = [pytest.param((M.EVENT_QUEUE, 128), id="queue"),
                pytest.param((M.EVENT_BANK, 96), id="bank96"),
                pytest.param((M.EVENT_BANK, 128), id="bank128"),
-               pytest.param((M.EVENT_BANK, 152), id="bank152")]
+               pytest.param((M.EVENT_BANK, 160), id="bank160")]
 
 
 def batch_mean_sigma(samples):
