@@ -19,8 +19,9 @@ service's own capacity.  Phantoms are built before the clock starts
 transport, the ROI checks and the read-back of every job's dose and image
 are timed.  The round kernel is the event kernel the autotuner picks for the
 stream (k_transport_bank_round, photon banks per wave, or
-k_transport_queue_round, a photon per lane), confirmed against its
-neighbouring parameters on untimed passes of the stream itself; the
+k_transport_queue_round, a photon per lane), refined by a coordinate
+search over bank size, blocks per CU and interaction batch on untimed
+passes of the stream itself; the
 roofline is that kernel's over the timed rounds: algorithmic bytes from its
 int64 event counters over its HIP-event time, and its voxel-dose atomics
 per second against what the library's scattered-deposit probe sustains on a
@@ -191,39 +192,47 @@ def set_event_kernel(ctx, name):
 
 def stream_tune(ctx, jobs, ph, tuned):
     """Start-up tuning on the stream itself: the autotuner probes one job,
-    while a round mixes every admitted job, so the autotuner's choice and
-    its neighbours (the other bank sizes, one block per CU more or fewer,
-    interaction batches of 32, 48 and 64) each serve one pass of the stream
-    and the fastest is kept."""
+    while a round mixes every admitted job, so from the autotuner's choice a
+    coordinate search over the bank size, the blocks per CU (2-4) and the
+    interaction batch (32, 48, 64) runs each candidate for one pass of the
+    stream, keeps the fastest and sweeps again until a sweep changes
+    nothing (at most three sweeps)."""
     from midsx_amd.serve import Server
     import midsx_amd as M
     kind, bank = ctx.event_kernel()
     b, r, i = ctx.queue_params()
-    cands = [(kind, bank, b, r, i)]
-    if kind == M.EVENT_BANK:
-        cands += [(kind, bs, b, r, i) for bs in M.BANK_SIZES if bs != bank]
-    for bb in (b - 1, b + 1):
-        if 2 <= bb <= 4:
-            cands.append((kind, bank, bb, r, i))
-    for ii in (32, 48, 64):
-        if ii != i:
-            cands.append((kind, bank, b, r, ii))
     Server(ctx, phantoms=ph).run(jobs)  # warm: memory pool and pinned buffers
-    times = []
-    for k, bs, bb, rr, ii in cands:
-        ctx.set_event_kernel(k, bs)
-        ctx.set_queue_params(bb, rr, ii)
-        t = time.perf_counter()
-        res = Server(ctx, phantoms=ph).run(jobs)
-        times.append((time.perf_counter() - t) / sum(x.histories for x in res))
-    k, bs, bb, rr, ii = cands[int(np.argmin(times))]
+    timed = {}
+
+    def cost(c):
+        if c not in timed:
+            k, bs, bb, rr, ii = c
+            ctx.set_event_kernel(k, bs)
+            ctx.set_queue_params(bb, rr, ii)
+            t = time.perf_counter()
+            res = Server(ctx, phantoms=ph).run(jobs)
+            timed[c] = (time.perf_counter() - t) / sum(x.histories for x in res)
+        return timed[c]
+
+    best = (kind, bank, b, r, i)
+    axes = [(1, M.BANK_SIZES if kind == M.EVENT_BANK else (bank,)), (2, (2, 3, 4)), (4, (32, 48, 64))]
+    for _ in range(3):
+        start = best
+        for pos, values in axes:
+            for v in values:
+                c = best[:pos] + (v,) + best[pos + 1:]
+                if cost(c) < cost(best):
+                    best = c
+        if best == start:
+            break
+    k, bs, bb, rr, ii = best
     ctx.set_event_kernel(k, bs)
     ctx.set_queue_params(bb, rr, ii)
     return dict(tuned, blocks_per_cu=bb, refill_min=rr, interact_min=ii,
                 event_kernel="bank" if k == M.EVENT_BANK else "queue", bank_size=bs,
                 autotuner=dict(event_kernel=tuned["event_kernel"], bank_size=tuned["bank_size"],
                                blocks_per_cu=tuned["blocks_per_cu"], interact_min=tuned["interact_min"]),
-                stream_candidates=len(cands))
+                stream_candidates=len(timed))
 
 
 def served_stream(ctx, n_jobs, seed, load, cap, steps, warmup, plain=True, params=None):
