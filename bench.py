@@ -390,6 +390,8 @@ def radiography(ctx, args):
         ctx.set_queue_params(*[int(v) for v in args.queue_params.split(",")])
     elif not args.no_autotune:
         tuned = ctx.autotune(probe_histories=4000000)  # launches of ~3 ms: stable timings
+        # refined on launches of the timed size (history indices above the timed ones)
+        tuned = launch_tune(ctx, lambda: ctx.run(n, seed=2, first=1 << 41, kernel=M.KERNEL_QUEUE)["kernel_ms"], tuned)
     progress("radiography: queue kernel")
     steps, warm = args.radiography_steps, args.radiography_warmup
     stats, wall, first = timed_steps(ctx, M.KERNEL_QUEUE, n, steps, warm, 0)
@@ -435,6 +437,46 @@ def radiography(ctx, args):
     return out
 
 
+def launch_tune(ctx, run, tuned):
+    """From the autotuner's choice, a coordinate search over the bank size,
+    the blocks per CU (2-4) and the interaction batch (32, 48, 64) on
+    run() (a reduced launch of the workload itself; returns its kernel
+    milliseconds), sweeping until a sweep changes nothing (at most three
+    sweeps); leaves the fastest set."""
+    import midsx_amd as M
+    kind, bank = ctx.event_kernel()
+    b, r, i = ctx.queue_params()
+    timed = {}
+
+    def cost(c):
+        if c not in timed:
+            k, bs, bb, rr, ii = c
+            ctx.set_event_kernel(k, bs)
+            ctx.set_queue_params(bb, rr, ii)
+            timed[c] = run()
+        return timed[c]
+
+    best = (kind, bank, b, r, i)
+    axes = [(1, M.BANK_SIZES if kind == M.EVENT_BANK else (bank,)), (2, (2, 3, 4)), (4, (32, 48, 64))]
+    for _ in range(3):
+        start = best
+        for pos, values in axes:
+            for v in values:
+                c = best[:pos] + (v,) + best[pos + 1:]
+                if cost(c) < cost(best):
+                    best = c
+        if best == start:
+            break
+    k, bs, bb, rr, ii = best
+    ctx.set_event_kernel(k, bs)
+    ctx.set_queue_params(bb, rr, ii)
+    return dict(tuned, blocks_per_cu=bb, refill_min=rr, interact_min=ii,
+                event_kernel="bank" if k == M.EVENT_BANK else "queue", bank_size=bs,
+                autotuner=dict(event_kernel=tuned["event_kernel"], bank_size=tuned.get("bank_size"),
+                               blocks_per_cu=tuned["blocks_per_cu"], interact_min=tuned["interact_min"]),
+                launch_candidates=len(timed))
+
+
 def ct_scan(ctx, n_views, scale, plain_every, seed=4, params=None):
     """CT: 512 x 512 x 256 phantom at 0.8 mm (water, lung, adipose, bone),
     n_views views about its z axis, 120 kVp, a 512 x 64 detector per view,
@@ -456,8 +498,12 @@ def ct_scan(ctx, n_views, scale, plain_every, seed=4, params=None):
         ctx.set_queue_params(int(b), int(r), int(i))
         tuned = dict(blocks_per_cu=int(b), refill_min=int(r), interact_min=int(i), event_kernel=ev, fixed=True)
     else:
-        # one view, probed as the multi-job launch the views run in
+        # one view, probed as the multi-job launch the views run in, then
+        # refined on every view at a tenth of its histories (the probes of
+        # one view do not separate banks of 128 and 160 reliably)
         tuned = ctx.autotune(probe_histories=4000000, rounds=True)
+        small = [max(1, h // 10) for h in n]
+        tuned = launch_tune(ctx, lambda: ctx.run_jobs(views, small, [1 << 41] * n_views, seeds)["kernel_ms"], tuned)
     ctx.set_graphs(True)
     ctx.run_jobs(views[:8], [100000] * 8, [1 << 40] * 8, seeds[:8])  # warm (capture)
     for j in [owner] + views:
