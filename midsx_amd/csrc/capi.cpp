@@ -2005,6 +2005,7 @@ int mdx_atomic_probe(mdx_ctx* c, uint64_t records, uint64_t deposits, double* ms
   return guard([&] {
     need(c && ms, "NULL argument");
     need(records > 0 && deposits > 0, "records and deposits must be positive");
+    need(records <= (1ull << 34), "records must be at most 2^34 (512 GiB of records)");
     set_device(c);
     DBuf r;
     r.alloc(records * 32);
