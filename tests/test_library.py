@@ -229,3 +229,20 @@ def test_macro_table_host():
     c.set_local_majorants(False)
     assert c.macro_cells() is None
     c.close()
+
+
+def test_event_kernel_choices_host():
+    """The event forms and bank sizes the library takes, on a host-only
+    context (no device work): banks of 96, 128 or 160 photons; any other
+    size or an unknown form is refused and leaves the setting unchanged."""
+    c = M.Context(-1)
+    for bs in M.BANK_SIZES:
+        c.set_event_kernel(M.EVENT_BANK, bs)
+        assert c.event_kernel() == (M.EVENT_BANK, bs)
+    for bad in ((M.EVENT_BANK, 152), (M.EVENT_BANK, 64), (7, 128)):
+        with pytest.raises(M.MdxError) as e:
+            c.set_event_kernel(*bad)
+        assert e.value.status == M.ERR_INVALID
+        assert c.event_kernel() == (M.EVENT_BANK, M.BANK_SIZES[-1])
+    assert M.BANK_MAX_HISTORY == 1 << 42
+    c.close()
