@@ -1673,8 +1673,8 @@ int mdx_autotune(mdx_ctx* c, uint64_t probe_histories, int flags, int* blocks_pe
       if (t < cur) { cur = t; b = v; }
     }
     // the bank kernel (not for the per-block LDS dose mode, which always runs
-    // on the queue kernel): both bank sizes at two to four blocks per CU
-    // (LDS and registers fit four blocks of either), then its refill and
+    // on the queue kernel): every bank size at two to four blocks per CU
+    // (LDS and registers fit four blocks of each), then its refill and
     // interaction batches
     const Job& O = job_at(c, domain_owner(c, job));
     const bool lds_dose = O.voxel_dose && O.dose_mode == MDX_DOSE_F32C_LDS && !rounds;
