@@ -192,47 +192,20 @@ def set_event_kernel(ctx, name):
 
 def stream_tune(ctx, jobs, ph, tuned):
     """Start-up tuning on the stream itself: the autotuner probes one job,
-    while a round mixes every admitted job, so from the autotuner's choice a
-    coordinate search over the bank size, the blocks per CU (2-4) and the
-    interaction batch (32, 48, 64) runs each candidate for one pass of the
-    stream, keeps the fastest and sweeps again until a sweep changes
-    nothing (at most three sweeps)."""
+    while a round mixes every admitted job, so from the autotuner's choice
+    the coordinate search of launch_tune runs each candidate for one pass of
+    the stream (seconds per history) and keeps the fastest."""
     from midsx_amd.serve import Server
-    import midsx_amd as M
-    kind, bank = ctx.event_kernel()
-    b, r, i = ctx.queue_params()
     Server(ctx, phantoms=ph).run(jobs)  # warm: memory pool and pinned buffers
-    timed = {}
 
-    def cost(c):
-        if c not in timed:
-            k, bs, bb, rr, ii = c
-            ctx.set_event_kernel(k, bs)
-            ctx.set_queue_params(bb, rr, ii)
-            t = time.perf_counter()
-            res = Server(ctx, phantoms=ph).run(jobs)
-            timed[c] = (time.perf_counter() - t) / sum(x.histories for x in res)
-        return timed[c]
+    def one_pass():
+        t = time.perf_counter()
+        res = Server(ctx, phantoms=ph).run(jobs)
+        return (time.perf_counter() - t) / sum(x.histories for x in res)
 
-    best = (kind, bank, b, r, i)
-    axes = [(1, M.BANK_SIZES if kind == M.EVENT_BANK else (bank,)), (2, (2, 3, 4)), (4, (32, 48, 64))]
-    for _ in range(3):
-        start = best
-        for pos, values in axes:
-            for v in values:
-                c = best[:pos] + (v,) + best[pos + 1:]
-                if cost(c) < cost(best):
-                    best = c
-        if best == start:
-            break
-    k, bs, bb, rr, ii = best
-    ctx.set_event_kernel(k, bs)
-    ctx.set_queue_params(bb, rr, ii)
-    return dict(tuned, blocks_per_cu=bb, refill_min=rr, interact_min=ii,
-                event_kernel="bank" if k == M.EVENT_BANK else "queue", bank_size=bs,
-                autotuner=dict(event_kernel=tuned["event_kernel"], bank_size=tuned["bank_size"],
-                               blocks_per_cu=tuned["blocks_per_cu"], interact_min=tuned["interact_min"]),
-                stream_candidates=len(timed))
+    out = launch_tune(ctx, one_pass, tuned)
+    out["stream_candidates"] = out.pop("launch_candidates")
+    return out
 
 
 def served_stream(ctx, n_jobs, seed, load, cap, steps, warmup, plain=True, params=None):
