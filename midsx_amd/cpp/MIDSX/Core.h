@@ -30,6 +30,7 @@
 #include <cmath>
 #include <fstream>
 #include <functional>
+#include <iostream>
 #include <map>
 #include <memory>
 #include <sstream>
@@ -78,6 +79,12 @@ struct Vec3 {
   double y() const { return v[1]; }
   double z() const { return v[2]; }
   double operator[](int i) const { return v[i]; }
+  // and back to a caller's 3-vector type constructible from three doubles
+  // (where the reference returns an Eigen::Vector3d)
+  template <class V, class = decltype(V(0.0, 0.0, 0.0)), class = decltype(std::declval<const V&>().x())>
+  operator V() const {  // NOLINT: implicit on purpose
+    return V(v[0], v[1], v[2]);
+  }
 };
 
 // ------------------------------------------------------------------ quantities
@@ -141,6 +148,27 @@ enum class VectorVolumeQuantityType { EnergyDeposition, IncidentEnergy };
 enum class CountVolumeQuantityType { NumberOfPhotons, NumberOfInteractions };
 enum class VectorSurfaceQuantityType { IncidentEnergy, EntranceCosine };
 enum class CountSurfaceQuantityType { NumberOfPhotons };
+
+// The quantity names drivers print (volume_quantity.h:18-33,
+// surface_quantity.h:19-33): each enumerator's own name.
+namespace VectorVolumeQuantityHelper {
+inline std::string toString(VectorVolumeQuantityType type) {
+  return type == VectorVolumeQuantityType::EnergyDeposition ? "EnergyDeposition" : "IncidentEnergy";
+}
+}  // namespace VectorVolumeQuantityHelper
+namespace CountVolumeQuantityHelper {
+inline std::string toString(CountVolumeQuantityType type) {
+  return type == CountVolumeQuantityType::NumberOfPhotons ? "NumberOfPhotons" : "NumberOfInteractions";
+}
+}  // namespace CountVolumeQuantityHelper
+namespace VectorSurfaceQuantityHelper {
+inline std::string toString(VectorSurfaceQuantityType type) {
+  return type == VectorSurfaceQuantityType::IncidentEnergy ? "IncidentEnergy" : "EntranceCosine";
+}
+}  // namespace VectorSurfaceQuantityHelper
+namespace CountSurfaceQuantityHelper {
+inline std::string toString(CountSurfaceQuantityType) { return "NumberOfPhotons"; }
+}  // namespace CountSurfaceQuantityHelper
 
 // Buckets of *_quantity.cpp measure(): primary, single incoherent, single
 // coherent, multiple (the order of the device result layout).
@@ -396,6 +424,14 @@ class MonoenergeticSpectrum : public EnergySpectrum {
 class PolyenergeticSpectrum : public EnergySpectrum {
  public:
   explicit PolyenergeticSpectrum(std::vector<std::pair<double, double>> probabilities) : p_(std::move(probabilities)) {}
+  // source.cpp:42-50 takes an Eigen::MatrixXd of (energy, probability) rows:
+  // any matrix with rows() and operator()(row, col) is read the same way
+  template <class M, class = decltype(std::declval<const M&>().rows()),
+            class = decltype(std::declval<const M&>()(0, 0))>
+  explicit PolyenergeticSpectrum(const M& probabilities) {
+    for (long i = 0; i < (long)probabilities.rows(); ++i)
+      p_.emplace_back((double)probabilities(i, 0), (double)probabilities(i, 1));
+  }
   void describe(mdx_source& s, std::vector<double>& e, std::vector<double>& w) const override {
     e.clear();
     w.clear();
@@ -508,11 +544,26 @@ class PhotonSource {
 };
 
 namespace SourceHelpers {
+// The rows of a two-column CSV file: a vector of (first, second) pairs that
+// also converts to any matrix type constructible from (rows, cols) with
+// operator()(row, col), e.g. the Eigen::MatrixXd the reference returns.
+struct CSVRows : std::vector<std::pair<double, double>> {
+  template <class M, class = decltype(std::declval<M&>()(0, 0) = 0.0)>
+  operator M() const {
+    M m((long)size(), 2);
+    for (size_t i = 0; i < size(); ++i) {
+      m((long)i, 0) = (*this)[i].first;
+      m((long)i, 1) = (*this)[i].second;
+    }
+    return m;
+  }
+};
+
 // source.cpp:10-40: comma-separated rows
-inline std::vector<std::pair<double, double>> readCSV(const std::string& file) {
+inline CSVRows readCSV(const std::string& file) {
   std::ifstream in(file);
   if (!in) throw std::invalid_argument("File does not exist");
-  std::vector<std::pair<double, double>> rows;
+  CSVRows rows;
   std::string line;
   while (std::getline(in, line)) {
     std::stringstream ls(line);
@@ -546,6 +597,8 @@ class VoxelGrid {
     for (auto& v : voxels) m[v.materialID] = m[v.materialID] + v.dose;
     return m;
   }
+  // voxel_grid.h:116-118: the grid's extent in cm (voxels x spacing per axis)
+  Vec3 getDimSpace() const { return Vec3(dims[0] * spacing[0], dims[1] * spacing[1], dims[2] * spacing[2]); }
 };
 
 class InteractionData;

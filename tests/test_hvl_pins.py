@@ -225,3 +225,22 @@ def test_cpp_kvp_driver_air_kerma_ratio(tmp_path):
     ref, ref_s = recorded_ratio_kvp(30, "hvl")
     sig = ratio * math.sqrt(3.0 / n)
     assert abs(ratio - ref) < 4 * math.hypot(sig, ref_s), (ratio, ref)
+
+
+def test_cpp_matrix_call_forms(tmp_path):
+    """The reference drivers' call forms that pass or receive Eigen types
+    (readCSV into a matrix, PolyenergeticSpectrum from it, getDimSpace into a
+    3-vector, the quantity-name helpers) compile and run against the header,
+    with matrix and vector types that have only the members those drivers
+    use (host only)."""
+    exe = os.path.join(str(tmp_path), "matrix_call_forms")
+    lib = os.path.join(ROOT, "midsx_amd", "_lib")
+    cmd = ["g++", "-std=c++17", "-O1", "-Wall", "-Werror", "-I", os.path.join(ROOT, "midsx_amd", "cpp"),
+           os.path.join(ROOT, "tests", "cpp", "matrix_call_forms.cpp"), "-o", exe, "-L", lib, "-lmidsx_amd",
+           "-Wl,-rpath," + lib]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout
+    csv = tmp_path / "spectrum.csv"
+    csv.write_text("10000,0.25\n20000,0.5\n30000,0.25\n")
+    r = subprocess.run([exe, str(csv)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=60)
+    assert r.returncode == 0 and "matrix call forms ok" in r.stdout, r.stdout
