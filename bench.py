@@ -101,11 +101,37 @@ def atomic_probe(ctx, deposits=200_000_000):
     return out
 
 
-def cpu_baseline(jobs, phantoms, seconds, affinity):
+def roi_dose(job, mean, var=None, rel=None, n=None):
+    """(dose per history, its standard error) of each of the job's four ROI
+    dose voxels (scenes.dose_job_roi) from per-voxel means per history (x
+    fastest) and either the variances of those means or their relative
+    standard errors, the voxels taken as independent; with n, mean and var
+    hold the voxels' sums and sums of squares over n histories instead.
+    Only the ROI's voxels are read (the arrays may be a whole result)."""
+    from midsx_amd import scenes
+    nx, ny, nz = job["shape"]
+    m3 = np.asarray(mean).reshape(nz, ny, nx)
+    e3 = np.asarray(var if var is not None else rel).reshape(nz, ny, nx)
+    out = []
+    for i0, i1, j0, j1, k0, k1 in scenes.dose_job_roi(job):
+        m = m3[k0:k1, j0:j1, i0:i1].astype(np.float64)
+        e = e3[k0:k1, j0:j1, i0:i1].astype(np.float64)
+        if n is not None:  # sums over n histories: mean and variance of the mean
+            m = m / n
+            e = np.maximum(e / n - m * m, 0.0) / (n - 1)
+        v = e if var is not None else (m * e) ** 2
+        out.append((float(m.sum()), float(np.sqrt(v.sum()))))
+    return out
+
+
+def cpu_baseline(jobs, phantoms, seconds, affinity, roi_gpu=None):
     """The FP64 C++/OpenMP restatement of the reference's CPU path on a
     bounded sample of the served stream: one job of each phantom size, each
     run for the histories that take about seconds / 4 on the host's cores
-    (the OpenMP team of the environment)."""
+    (the OpenMP team of the environment).  With roi_gpu (job index -> the
+    served result's ROI doses), each sampled job's ROI doses from that CPU
+    run are compared with the served ones: z = difference over the combined
+    standard error, per ROI dose voxel."""
     import oracle
     from midsx_amd import scenes
     threads = oracle.max_threads()
@@ -115,7 +141,7 @@ def cpu_baseline(jobs, phantoms, seconds, affinity):
         if j["shape"][0] not in seen:
             seen.add(j["shape"][0])
             picks.append(j)
-    total_n, total_t, parts = 0, 0.0, []
+    total_n, total_t, parts, agreement = 0, 0.0, [], []
     for j in picks:
         progress("CPU baseline: %d^3 job" % j["shape"][0])
         scenes.setup_dose_job(sim, j, ids=phantoms(j))
@@ -132,11 +158,27 @@ def cpu_baseline(jobs, phantoms, seconds, affinity):
         total_n += n
         total_t += dt
         parts.append("%d^3: %d histories in %.1f s" % (j["shape"][0], n, dt))
+        idx = next(k for k, jj in enumerate(jobs) if jj is j)
+        if roi_gpu and idx in roi_gpu:
+            ssum, ssq, _ = sim.voxel_dose(0)
+            cpu = roi_dose(j, ssum, var=ssq, n=n)
+            z = [(g[0] - c[0]) / np.hypot(g[1], c[1]) for g, c in zip(roi_gpu[idx]["roi"], cpu)]
+            agreement.append(dict(job=idx, shape=j["shape"][0], cpu_histories=n,
+                                  served_histories=roi_gpu[idx].get("histories"),
+                                  z=[round(float(x), 3) for x in z], max_abs_z=float(np.max(np.abs(z)))))
+    zs = np.array([z for a in agreement for z in a["z"]])
+    chi2 = float(np.sum(zs ** 2)) if zs.size else None
+    try:
+        from scipy import stats
+        p_chi2 = float(stats.chi2.sf(chi2, zs.size)) if zs.size else None
+    except ImportError:
+        p_chi2 = None
     return dict(value=total_n / total_t, unit="histories/s", cores=threads, kind="port",
                 sample="FP64 C++/OpenMP restatement of runSimulation on one dose job of each phantom size of the "
                        "served stream (" + "; ".join(parts) + ")",
                 omp_max_threads=threads, nproc=os.cpu_count(), affinity_cpus=affinity,
-                omp_proc_bind=os.environ.get("OMP_PROC_BIND"), omp_places=os.environ.get("OMP_PLACES"))
+                omp_proc_bind=os.environ.get("OMP_PROC_BIND"), omp_places=os.environ.get("OMP_PLACES"),
+                dose_agreement=dict(jobs=agreement, chi2=chi2, dof=int(zs.size), p=p_chi2))
 
 
 def timed_steps(ctx, kernel, n, steps, warmup, first):
@@ -286,8 +328,17 @@ def served_stream(ctx, n_jobs, seed, load, cap, steps, warmup, plain=True, param
     arr = poisson_arrivals(n_jobs, load * n_jobs / pass_s, seed + 1)
     for srv, res, _ in passes:
         pass_rounds.append(dict(kind="timed", rounds=srv.rounds, histories=sum(r.histories for r in res)))
-    srv = Server(ctx, phantoms=ph)
+    roi_gpu = {}  # the latency pass's ROI doses per job, for the comparison with the CPU path
+
+    def keep_roi(job, a, b, image):
+        k = next(i for i, jj in enumerate(jobs) if jj is job)
+        roi_gpu[k] = {"roi": roi_dose(job, a, rel=b)}
+
+    srv = Server(ctx, phantoms=ph, on_result=keep_roi)
     res = srv.run(jobs, arr)
+    for r in res:
+        if r.index in roi_gpu:
+            roi_gpu[r.index]["histories"] = r.histories
     pass_rounds.append(dict(kind="latency", rounds=srv.rounds, histories=sum(r.histories for r in res)))
     p50, p99 = latency_percentiles(res)
     plain_s = q50 = q99 = None
@@ -311,6 +362,7 @@ def served_stream(ctx, n_jobs, seed, load, cap, steps, warmup, plain=True, param
     traffic = traffic_ph * float(np.mean([r["histories"] for r in rounds])) if traffic_ph else None
     Hp = sum(hist)
     return jobs, ph, {
+        "_roi_gpu": roi_gpu,
         "workload": "%d dose jobs (seed %d), phantoms %s voxels at 0.5-2 mm, each run to <= %g relative "
                     "standard error on its 4 ROI dose voxels (1 cm^3, on the beam axis 1-2 cm deep); "
                     "histories per job min %.3g, median %.3g, max %.3g, %d per pass; a step is one pass of the "
@@ -631,7 +683,7 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
         "data": "synthetic (seeded ellipsoid phantoms; EPDL2017 tables for air, water, lung, adipose, cortical bone)",
     }
-    jobs = ph = None
+    jobs = ph = roi_gpu = None
     if not args.no_served:
         if args.steps < 1 or args.warmup < 0:
             ap.error("--steps must be at least 1 and --warmup at least 0")
@@ -646,6 +698,7 @@ def main():
                                    "hence matched sigma",
                          "step": "one pass of the served stream: all %d jobs served to their ROI target"
                                  % args.served_jobs}
+        roi_gpu = sv.pop("_roi_gpu")
         out["served"] = sv
     if not args.no_radiography:
         out["radiography"] = radiography(ctx, args)
@@ -662,7 +715,7 @@ def main():
             out["roofline"]["dose_atomics_probe_per_s"] = peak
             out["roofline"]["dose_atomics_frac"] = out["roofline"]["dose_atomics_per_s"] / peak
     if not args.no_cpu_baseline and jobs is not None:
-        out["cpu_baseline"] = cpu_baseline(jobs, ph, args.cpu_seconds, affinity)
+        out["cpu_baseline"] = cpu_baseline(jobs, ph, args.cpu_seconds, affinity, roi_gpu)
     print(json.dumps(out), flush=True)
 
 

@@ -52,3 +52,26 @@ def test_launch_tune_on_the_queue_kernel_keeps_its_bank():
                                                             "blocks_per_cu": 4, "interact_min": 32})
     assert ctx.ev == (M.EVENT_QUEUE, 128)
     assert ctx.qp[0] == 2 and out["event_kernel"] == "queue"
+
+
+def test_roi_dose_sums_and_errors():
+    """ROI doses from per-voxel means with relative errors equal those from
+    the same means with variances, and from sums and sums of squares over n
+    histories; each ROI dose voxel covers the 1 cm cube of scenes.dose_job_roi."""
+    import numpy as np
+    from midsx_amd import scenes
+    job = dict(shape=(40, 40, 40), voxel_cm=0.5)
+    rng = np.random.default_rng(3)
+    n = 5000
+    s = rng.random(40 ** 3) * n
+    s2 = s * s / n * (1 + rng.random(40 ** 3))
+    m = s / n
+    var = np.maximum(s2 / n - m * m, 0) / (n - 1)
+    a = bench.roi_dose(job, s, var=s2, n=n)
+    b = bench.roi_dose(job, m, var=var)
+    c = bench.roi_dose(job, m.astype(np.float32), rel=(np.sqrt(var) / m).astype(np.float32))
+    assert np.allclose(a, b, rtol=1e-12) and np.allclose(b, c, rtol=1e-5)
+    m3 = m.reshape(40, 40, 40)
+    for (i0, i1, j0, j1, k0, k1), (d, _) in zip(scenes.dose_job_roi(job), b):
+        assert (i1 - i0, j1 - j0, k1 - k0) == (2, 2, 2)
+        assert abs(d - m3[k0:k1, j0:j1, i0:i1].sum()) < 1e-9 * d
