@@ -1169,7 +1169,7 @@ k_transport_queue_round(const mdx_tables_dev* __restrict__ T, const char* __rest
 
 // ------------------------------------------------------------------ bank kernel
 // The event form with photon banks.  Every wave keeps a bank of B photons in
-// LDS (structure of arrays, 80 B each) and a list of bank slots per next
+// LDS (structure of arrays, 60 B each) and a list of bank slots per next
 // event: a Woodcock step inside a grid, a free flight through the
 // background, a parked real interaction, or a free slot.  Each iteration the
 // wave takes up to 64 photons from one list (the fullest step or flight
