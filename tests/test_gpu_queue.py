@@ -235,7 +235,7 @@ def test_autotune_on_aliased_job_clears_shared_dose(rounds):
     assert ctx.voxel_dose(0)[0].sum() > 0
     ctx.job_select(view)
     tuned = ctx.autotune(probe_histories=20000, rounds=rounds)
-    assert tuned["event_kernel"] in ("queue", "bank") and tuned["bank_size"] in (96, 128)
+    assert tuned["event_kernel"] in ("queue", "bank") and tuned["bank_size"] in M.BANK_SIZES
     assert ctx.event_kernel() == (M.EVENT_BANK if tuned["event_kernel"] == "bank" else M.EVENT_QUEUE,
                                   tuned["bank_size"])
     assert ctx.queue_params() == (tuned["blocks_per_cu"], tuned["refill_min"], tuned["interact_min"])
