@@ -246,3 +246,19 @@ def test_event_kernel_choices_host():
         assert c.event_kernel() == (M.EVENT_BANK, M.BANK_SIZES[-1])
     assert M.BANK_MAX_HISTORY == 1 << 42
     c.close()
+
+
+def test_queue_params_limits_host():
+    """Launch parameters on a host-only context: blocks per CU, refill batch
+    and interaction batch each in [1, 64] (a batch is at most one wave's 64
+    lanes); a value outside is refused and leaves the setting unchanged."""
+    c = M.Context(-1)
+    for p in ((1, 1, 1), (64, 64, 64), (4, 64, 48)):
+        c.set_queue_params(*p)
+        assert c.queue_params() == p
+    for bad in ((0, 64, 48), (65, 64, 48), (4, 0, 48), (4, 65, 48), (4, 64, 0), (4, 64, 65)):
+        with pytest.raises(M.MdxError) as e:
+            c.set_queue_params(*bad)
+        assert e.value.status == M.ERR_INVALID
+        assert c.queue_params() == (4, 64, 48)
+    c.close()
